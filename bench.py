@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""H.v throughput benchmark: 18-site Kagome Hubbard, N_up = N_dn = 9, FP64.
+
+One step = one on-the-fly H.v (Siro-Harju split) over the full sector,
+D = 48,620^2 = 2,363,904,400 amplitudes, x = the normalized seed-235 start vector,
+bonds of the 2x3-cell Kagome cluster (KagomeLattice18y, drude_v2_cal_Kagome_randomHop.cu:712-730)
+with matrix element -1 and U = 8.  Inputs are resident in HBM before timing starts.
+
+value = algorithmic bytes (16 D per H.v: x read once, y written once) / mean step time.
+The JSON line also carries the roofline of the H.v kernel (HBM bound, MI355X peak
+8 TB/s) and a CPU baseline: the oracle's H.v (a C restatement of the reference
+kernel, OpenMP over rows) timed on a bounded sample of rows of the same workload.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "H·v HBM GB/s vs peak (18-site Kagome); batched Drude weights/s vs plain @1e-8"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hv_kagome18y_traffic.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per H.v launch from the committed rocprofv3 PMC summary (or None)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
+    """Oracle H.v on a bounded row sample; returns (GB/s by the same 16 B/element count, rows, secs)."""
+    import numpy as np
+    from oracle import oracle as O
+    y = np.empty_like(x_host)  # only the sampled rows are touched
+    rows = 16
+    t0 = time.perf_counter()
+    H_or.apply_rows(x_host, y, 0, rows)
+    dt = time.perf_counter() - t0
+    per_row = max(dt / rows, 1e-6)
+    rows2 = int(min(H_or.dim_up - rows, max(16, budget_s / per_row)))
+    t0 = time.perf_counter()
+    H_or.apply_rows(x_host, y, rows, rows2)
+    dt2 = time.perf_counter() - t0
+    elems = rows2 * dim_dn
+    return 16.0 * elems / dt2 / 1e9, rows2, dt2, O.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sites", type=int, default=18, choices=[12, 18])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the H.v path here is single-GPU (multi-GPU sharding is a design "
+              "note only), so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard, Lanczos
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+
+    torch.cuda.set_device(0)
+    cl = L.KAGOME18Y if args.sites == 18 else L.KAGOME12
+    N = cl.n_sites
+    Q = N // 2
+    U = 8.0 if args.sites == 18 else 4.0
+    ops = L.hubbard_ops(cl, t=-1.0)
+    log(f"[bench] building basis/tables N={N} Q={Q}")
+    hq = H_TBq(N, Q)
+    hq.add_ops(ops)
+    H = Hubbard(hq, hq, U)
+    D = H.dim
+    log(f"[bench] D = {D} ({D * 8 / 1e9:.2f} GB per vector)")
+    x = torch.empty(D, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    Lanczos(hq.count, hq.count).random_vector(x)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        H.Hubbard_mv(x, y)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({args.warmup} steps)")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record()
+        H.Hubbard_mv(x, y)
+        ends[k].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))  # ms
+    mean_ms = sum(per) / len(per)
+    median_ms = per[len(per) // 2]
+    alg_bytes = 16.0 * D
+    gbs = alg_bytes / (mean_ms * 1e-3) / 1e9
+    log(f"[bench] H.v mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, wall/step {wall / args.steps * 1e3:.3f} ms")
+
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seed-235 minstd_rand start vector, the reference's GenRand)",
+        "config": {"workload": f"hubbard_hv_{cl.name}_nup{Q}_ndn{Q}_U{U:g}", "sites": N, "dim": D,
+                   "hv_median_ms": round(median_ms, 4), "hv_mean_ms": round(mean_ms, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import numpy as np
+        from oracle import oracle as O
+        log("[bench] cpu baseline: oracle H.v on a row sample")
+        osp = O.Species(N, Q, ops)
+        Hor = O.Hubbard(osp, osp, U)
+        x_host = np.empty(D, dtype=np.float64)
+        torch.from_numpy(x_host).copy_(x)
+        cg, rows, secs, thr = cpu_baseline(Hor, x_host, hq.count)
+        out["cpu_baseline"] = {"value": round(cg, 3), "unit": "GB/s", "cores": thr, "kind": "port",
+                               "sample": f"oracle H.v (C restatement of Hubbard_mv_), rows {16}..{16 + rows} "
+                                         f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
+        del x_host
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

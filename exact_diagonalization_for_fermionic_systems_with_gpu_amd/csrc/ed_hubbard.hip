@@ -1,0 +1,537 @@
+// ed_hubbard.hip -- the on-the-fly Hubbard H.v (Siro-Harju split) and the fused
+// Lanczos kernels built on it.
+//
+// Reference: struct Hubbard + Hubbard_mv_ (MBL_test/include/HubMatrix_GPU.cuh:30-182).
+//   y[u,d] = sum_s AxUP[s,u] x[AjUP[s,u], d]          (H_up (x) 1, rows mix)
+//          + sum_s AxDN[s,d] x[u, AjDN[s,d]]          (1 (x) H_dn, columns mix)
+//          + U popcount(up_u & dn_d & mask) x[u,d]    (diagonal)
+// with psi row-major [u*dim_dn + d].  Per output element the terms are summed in the
+// reference order: UP slots ascending, DN slots ascending, then U.  Slots that hold
+// an exact 0.0 are dropped (the reference skips them for UP and adds 0*x for DN).
+//
+// Data layout in HBM (built once per operator from the reference's ELL tables):
+//   UP side  -- CSR over rows u: the <= S nonzero slots of one row are shared by all
+//               columns of that row, so a workgroup stages them in LDS and each lane
+//               streams x[col*dim_dn + d] for consecutive d (coalesced 512 B per wave).
+//   DN side  -- sliced ELL over columns d, slices of 64 columns (one wavefront), width
+//               = the slice's longest row, entry k of column d at base + 64k + (d&63):
+//               table loads are coalesced; padding entries point at the own column
+//               with value 0.
+//   diag     -- computed from the two state words (popcount, or a per-site U vector).
+// All element offsets are 64-bit (D = 2.36e9 at 18 sites).
+#include <algorithm>
+#include <vector>
+
+#include "ed_common.hpp"
+#include "ed_hubbard.hpp"
+
+namespace ed {
+
+// ------------------------------------------------------------------------------------
+// compaction of the ELL tables
+// ------------------------------------------------------------------------------------
+template <bool CPLX>
+__global__ void k_row_nnz(int64_t count, int S, const double *__restrict__ Ax, int32_t *__restrict__ nnz)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= count) return;
+    int c = 0;
+    for (int s = 0; s < S; ++s) {
+        const int64_t idx = (int64_t)s * count + a;
+        const bool nz = CPLX ? (Ax[2 * idx] != 0.0 || Ax[2 * idx + 1] != 0.0) : (Ax[idx] != 0.0);
+        c += nz ? 1 : 0;
+    }
+    nnz[a] = c;
+}
+
+// CSR fill (UP side)
+template <bool CPLX>
+__global__ void k_fill_csr(int64_t count, int S, const double *__restrict__ Ax, const int32_t *__restrict__ Aj,
+                           const int64_t *__restrict__ ptr, int32_t *__restrict__ col, double *__restrict__ val)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= count) return;
+    int64_t o = ptr[a];
+    for (int s = 0; s < S; ++s) {
+        const int64_t idx = (int64_t)s * count + a;
+        if (CPLX) {
+            const double re = Ax[2 * idx], im = Ax[2 * idx + 1];
+            if (re == 0.0 && im == 0.0) continue;
+            val[2 * o] = re;
+            val[2 * o + 1] = im;
+        } else {
+            const double v = Ax[idx];
+            if (v == 0.0) continue;
+            val[o] = v;
+        }
+        col[o] = Aj[idx];
+        ++o;
+    }
+}
+
+// sliced-ELL fill (DN side): slice = 64 consecutive columns
+template <bool CPLX>
+__global__ void k_fill_sell(int64_t count, int S, const double *__restrict__ Ax, const int32_t *__restrict__ Aj,
+                            const int64_t *__restrict__ sptr, int32_t *__restrict__ col, double *__restrict__ val)
+{
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= count) return;
+    const int64_t sl = d >> 6;
+    const int lane = (int)(d & 63);
+    const int64_t base = sptr[sl];
+    const int w = (int)((sptr[sl + 1] - base) >> 6);
+    int k = 0;
+    for (int s = 0; s < S; ++s) {
+        const int64_t idx = (int64_t)s * count + d;
+        double re, im = 0.0;
+        if (CPLX) {
+            re = Ax[2 * idx];
+            im = Ax[2 * idx + 1];
+            if (re == 0.0 && im == 0.0) continue;
+        } else {
+            re = Ax[idx];
+            if (re == 0.0) continue;
+        }
+        const int64_t e = base + (int64_t)k * 64 + lane;
+        col[e] = Aj[idx];
+        if (CPLX) {
+            val[2 * e] = re;
+            val[2 * e + 1] = im;
+        } else {
+            val[e] = re;
+        }
+        ++k;
+    }
+    for (; k < w; ++k) {
+        const int64_t e = base + (int64_t)k * 64 + lane;
+        col[e] = (int32_t)d;
+        if (CPLX) {
+            val[2 * e] = 0.0;
+            val[2 * e + 1] = 0.0;
+        } else {
+            val[e] = 0.0;
+        }
+    }
+}
+
+static int compact_species(const SpeciesIn &in, bool cplx, Species &out, bool as_csr, hipStream_t st)
+{
+    out.dim = in.dim;
+    out.state = nullptr;
+    const int64_t n = in.dim;
+    const int B = 256;
+    const unsigned grid = (unsigned)((n + B - 1) / B);
+    int32_t *dnnz = nullptr;
+    ED_TRY(dmalloc((void **)&dnnz, sizeof(int32_t) * (size_t)n));
+    if (cplx) hipLaunchKernelGGL(k_row_nnz<true>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, dnnz);
+    else hipLaunchKernelGGL(k_row_nnz<false>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, dnnz);
+    std::vector<int32_t> nnz((size_t)n);
+    if (hipMemcpyAsync(nnz.data(), dnnz, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        dfree(dnnz);
+        return fail(ED_EHIP, "compact_species: nnz count failed");
+    }
+    dfree(dnnz);
+    const size_t vsz = cplx ? 2 * sizeof(double) : sizeof(double);
+    int maxw = 0;
+    for (int32_t c : nnz) maxw = std::max(maxw, (int)c);
+    out.max_nnz = maxw;
+    if (as_csr) {
+        std::vector<int64_t> ptr((size_t)n + 1);
+        ptr[0] = 0;
+        for (int64_t a = 0; a < n; ++a) ptr[a + 1] = ptr[a] + nnz[a];
+        const int64_t tot = ptr[n];
+        out.nent = tot;
+        ED_TRY(dmalloc((void **)&out.ptr, sizeof(int64_t) * (size_t)(n + 1)));
+        ED_TRY(dmalloc((void **)&out.col, sizeof(int32_t) * (size_t)std::max<int64_t>(tot, 1)));
+        ED_TRY(dmalloc((void **)&out.val, vsz * (size_t)std::max<int64_t>(tot, 1)));
+        ED_HIP(hipMemcpyAsync(out.ptr, ptr.data(), sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+        if (cplx)
+            hipLaunchKernelGGL(k_fill_csr<true>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, in.Aj, out.ptr,
+                               out.col, out.val);
+        else
+            hipLaunchKernelGGL(k_fill_csr<false>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, in.Aj, out.ptr,
+                               out.col, out.val);
+        ED_CHECK_LAUNCH();
+        ED_HIP(hipStreamSynchronize(st)); // ptr (host vector) must outlive the copy
+    } else {
+        const int64_t nsl = (n + 63) / 64;
+        std::vector<int64_t> sptr((size_t)nsl + 1);
+        sptr[0] = 0;
+        for (int64_t s = 0; s < nsl; ++s) {
+            int w = 0;
+            for (int64_t d = s * 64; d < std::min<int64_t>(n, s * 64 + 64); ++d) w = std::max(w, (int)nnz[d]);
+            sptr[s + 1] = sptr[s] + (int64_t)w * 64;
+        }
+        const int64_t tot = sptr[nsl];
+        out.nent = tot;
+        ED_TRY(dmalloc((void **)&out.ptr, sizeof(int64_t) * (size_t)(nsl + 1)));
+        ED_TRY(dmalloc((void **)&out.col, sizeof(int32_t) * (size_t)std::max<int64_t>(tot, 1)));
+        ED_TRY(dmalloc((void **)&out.val, vsz * (size_t)std::max<int64_t>(tot, 1)));
+        ED_HIP(hipMemcpyAsync(out.ptr, sptr.data(), sizeof(int64_t) * (size_t)(nsl + 1), hipMemcpyHostToDevice,
+                              st));
+        if (cplx)
+            hipLaunchKernelGGL(k_fill_sell<true>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, in.Aj, out.ptr,
+                               out.col, out.val);
+        else
+            hipLaunchKernelGGL(k_fill_sell<false>, dim3(grid), dim3(B), 0, st, n, in.S, in.Ax, in.Aj, out.ptr,
+                               out.col, out.val);
+        ED_CHECK_LAUNCH();
+        ED_HIP(hipStreamSynchronize(st));
+    }
+    return ED_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// value helpers (real / complex)
+// ------------------------------------------------------------------------------------
+template <typename V>
+struct VT;
+template <>
+struct VT<double> {
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double scale(double s, double x) { return s * x; }
+    static __device__ __forceinline__ double fma_rr(double a, double x, double acc) { return fma(a, x, acc); }
+    static __device__ __forceinline__ double dotre(double a, double b) { return a * b; }
+    static __device__ __forceinline__ double abs2(double a) { return a * a; }
+};
+template <>
+struct VT<cplx> {
+    static __device__ __forceinline__ cplx zero() { return cplx{0.0, 0.0}; }
+    static __device__ __forceinline__ cplx scale(double s, cplx x) { return cplx{s * x.re, s * x.im}; }
+    static __device__ __forceinline__ cplx fma_rr(double a, cplx x, cplx acc)
+    {
+        return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)};
+    }
+    static __device__ __forceinline__ double dotre(cplx a, cplx b) { return a.re * b.re + a.im * b.im; }
+    static __device__ __forceinline__ double abs2(cplx a) { return a.re * a.re + a.im * a.im; }
+};
+
+// acc += a * x for table value type TV and vector type V
+__device__ __forceinline__ double mac(double a, double x, double acc) { return fma(a, x, acc); }
+__device__ __forceinline__ cplx mac(double a, cplx x, cplx acc)
+{
+    return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)};
+}
+__device__ __forceinline__ cplx mac(cplx a, cplx x, cplx acc)
+{
+    // (a.re + i a.im)(x.re + i x.im)
+    return cplx{fma(a.re, x.re, fma(-a.im, x.im, acc.re)), fma(a.re, x.im, fma(a.im, x.re, acc.im))};
+}
+
+template <typename V>
+__device__ __forceinline__ V ldv(const V *p, int64_t i)
+{
+    return p[i];
+}
+
+template <typename V>
+__device__ __forceinline__ V vscale(double s, V x)
+{
+    return VT<V>::scale(s, x);
+}
+
+__device__ __forceinline__ double diag_value(int32_t su, int32_t sd, double U, int64_t mask,
+                                             const double *__restrict__ Uvec)
+{
+    const uint32_t dbl = (uint32_t)(su & sd) & (uint32_t)mask;
+    if (!Uvec) return U * (double)__popc(dbl);
+    double s = 0.0;
+    uint32_t m = dbl;
+    while (m) {
+        const int i = __ffs(m) - 1;
+        s += Uvec[i];
+        m &= m - 1;
+    }
+    return s;
+}
+
+// ------------------------------------------------------------------------------------
+// H.v kernel.  One workgroup = one row u and a run of CPB = 256*CPT columns.
+// FLAGS: HV_SCALE (x gathered as s*x, s = 1/b_dev[step]), HV_WRITEV (v[i] = s*x[i]),
+//        HV_AXPY (y = Hx + c*w, c = -b_dev[step]), HV_DOT (block partial Re<v,y>).
+// ------------------------------------------------------------------------------------
+enum { HV_SCALE = 1, HV_WRITEV = 2, HV_AXPY = 4, HV_DOT = 8 };
+
+template <typename V, typename TV, int CPT, int FLAGS>
+__global__ __launch_bounds__(256) void k_hv(HvArgs p, const V *__restrict__ x, V *y,
+                                            V *__restrict__ vout, const V *w, const double *__restrict__ bdev,
+                                            int step, double *__restrict__ partial)
+{
+    constexpr int B = 256;
+    constexpr int CPB = B * CPT;
+    constexpr int MAXUP = 512;
+    __shared__ int32_t s_col[MAXUP];
+    __shared__ TV s_val[MAXUP];
+    __shared__ double s_red[B / 64];
+
+    const int64_t u = blockIdx.x / p.blocks_per_row;
+    const int64_t cb = blockIdx.x - u * p.blocks_per_row;
+    const int64_t dim_dn = p.dim_dn;
+    const int64_t row0 = u * dim_dn;
+
+    const int64_t up0 = p.up_ptr[u];
+    const int nup = (int)(p.up_ptr[u + 1] - up0);
+    for (int t = threadIdx.x; t < nup; t += B) {
+        s_col[t] = p.up_col[up0 + t];
+        s_val[t] = reinterpret_cast<const TV *>(p.up_val)[up0 + t];
+    }
+    __syncthreads();
+
+    double s = 1.0, c = 0.0;
+    if (FLAGS & HV_SCALE) s = 1.0 / bdev[step];
+    if (FLAGS & HV_AXPY) c = -bdev[step];
+    const int32_t su = p.state_up[u];
+
+    V acc[CPT];
+    int64_t dcol[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        acc[k] = VT<V>::zero();
+        dcol[k] = cb * CPB + (int64_t)k * B + threadIdx.x;
+    }
+
+    // UP term: rows col_j of x, same columns (coalesced streams)
+    for (int j = 0; j < nup; ++j) {
+        const int64_t rb = (int64_t)s_col[j] * dim_dn;
+        const TV a = s_val[j];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (dcol[k] < dim_dn) {
+                V xv = x[rb + dcol[k]];
+                if (FLAGS & HV_SCALE) xv = vscale(s, xv);
+                acc[k] = mac(a, xv, acc[k]);
+            }
+        }
+    }
+
+    double dot = 0.0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int64_t d = dcol[k];
+        if (d >= dim_dn) continue;
+        // DN term: gathers inside row u
+        const int64_t sl = d >> 6;
+        const int lane = (int)(d & 63);
+        const int64_t base = p.dn_sptr[sl];
+        const int wdt = (int)((p.dn_sptr[sl + 1] - base) >> 6);
+        const TV *dv = reinterpret_cast<const TV *>(p.dn_val);
+        for (int e = 0; e < wdt; ++e) {
+            const int64_t ei = base + (int64_t)e * 64 + lane;
+            V xv = x[row0 + p.dn_col[ei]];
+            if (FLAGS & HV_SCALE) xv = vscale(s, xv);
+            acc[k] = mac(dv[ei], xv, acc[k]);
+        }
+        // diagonal
+        V xd = x[row0 + d];
+        if (FLAGS & HV_SCALE) xd = vscale(s, xd);
+        if (p.has_diag) {
+            const double dg = diag_value(su, p.state_dn[d], p.U, p.mask, p.Uvec);
+            acc[k] = VT<V>::fma_rr(dg, xd, acc[k]);
+        }
+        V out = acc[k];
+        if (FLAGS & HV_AXPY) out = VT<V>::fma_rr(c, w[row0 + d], out);
+        y[row0 + d] = out;
+        if (FLAGS & HV_WRITEV) vout[row0 + d] = xd;
+        if (FLAGS & HV_DOT) dot += VT<V>::dotre(xd, out);
+    }
+    if (FLAGS & HV_DOT) {
+        const double r = block_sum<B>(dot, s_red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = r;
+    }
+}
+
+template <typename V, typename TV, int FLAGS>
+static int launch_hv_cpt(const HvArgs &a0, int cpt, const V *x, V *y, V *vout, const V *w, const double *bdev,
+                         int step, double *partial, hipStream_t st, int64_t *nblocks_out)
+{
+    HvArgs a = a0;
+    const int64_t cpb = 256 * (int64_t)cpt;
+    a.blocks_per_row = (a.dim_dn + cpb - 1) / cpb;
+    const int64_t nb = a.blocks_per_row * a.dim_up;
+    if (nb > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "H.v grid too large (%lld blocks)", (long long)nb);
+    if (nblocks_out) *nblocks_out = nb;
+    dim3 g((unsigned)nb), b(256);
+    switch (cpt) {
+    case 1: hipLaunchKernelGGL((k_hv<V, TV, 1, FLAGS>), g, b, 0, st, a, x, y, vout, w, bdev, step, partial); break;
+    case 2: hipLaunchKernelGGL((k_hv<V, TV, 2, FLAGS>), g, b, 0, st, a, x, y, vout, w, bdev, step, partial); break;
+    default: hipLaunchKernelGGL((k_hv<V, TV, 4, FLAGS>), g, b, 0, st, a, x, y, vout, w, bdev, step, partial); break;
+    }
+    ED_CHECK_LAUNCH();
+    return ED_OK;
+}
+
+int choose_cpt(int64_t dim_up, int64_t dim_dn)
+{
+    // enough workgroups to cover 256 CUs several times over
+    for (int cpt : {4, 2}) {
+        const int64_t bpr = (dim_dn + 256 * cpt - 1) / (256 * cpt);
+        if (bpr * dim_up >= 8192 && 256 * cpt <= dim_dn * 2) return cpt;
+    }
+    return 1;
+}
+
+template <int FLAGS>
+int hv_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout, const void *w, const double *bdev,
+                int step, double *partial, hipStream_t st, int64_t *nblocks)
+{
+    const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
+    if (!cvec) {
+        if (h.cplx_tables) return fail(ED_EINVAL, "complex tables need complex vectors");
+        return launch_hv_cpt<double, double, FLAGS>(h.args, cpt, (const double *)x, (double *)y, (double *)vout,
+                                                    (const double *)w, bdev, step, partial, st, nblocks);
+    }
+    if (h.cplx_tables)
+        return launch_hv_cpt<cplx, cplx, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout,
+                                                (const cplx *)w, bdev, step, partial, st, nblocks);
+    return launch_hv_cpt<cplx, double, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w,
+                                              bdev, step, partial, st, nblocks);
+}
+
+int64_t hv_num_blocks(const Hubbard &h)
+{
+    const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
+    const int64_t cpb = 256 * (int64_t)cpt;
+    return ((h.args.dim_dn + cpb - 1) / cpb) * h.args.dim_up;
+}
+
+int hv_plain(const Hubbard &h, bool cvec, const void *x, void *y, hipStream_t st)
+{
+    return hv_dispatch<0>(h, cvec, x, y, nullptr, nullptr, nullptr, 0, nullptr, st, nullptr);
+}
+
+// Lanczos step, first half: v[i] = x[i]/b[step]; y = H v - b[step] w (step > 0); partial Re<v,y>
+int hv_lanczos(const Hubbard &h, bool cvec, const void *f, void *y, void *v, const void *w, const double *bdev,
+               int step, double *partial, hipStream_t st, int64_t *nblocks)
+{
+    if (step == 0)
+        return hv_dispatch<HV_SCALE | HV_WRITEV | HV_DOT>(h, cvec, f, y, v, nullptr, bdev, step, partial, st,
+                                                          nblocks);
+    return hv_dispatch<HV_SCALE | HV_WRITEV | HV_AXPY | HV_DOT>(h, cvec, f, y, v, w, bdev, step, partial, st,
+                                                                 nblocks);
+}
+
+// Ritz rebuild: same recurrence without the dot
+int hv_lanczos_nodot(const Hubbard &h, bool cvec, const void *f, void *y, void *v, const void *w,
+                     const double *bdev, int step, hipStream_t st)
+{
+    if (step == 0)
+        return hv_dispatch<HV_SCALE | HV_WRITEV>(h, cvec, f, y, v, nullptr, bdev, step, nullptr, st, nullptr);
+    return hv_dispatch<HV_SCALE | HV_WRITEV | HV_AXPY>(h, cvec, f, y, v, w, bdev, step, nullptr, st, nullptr);
+}
+
+void Hubbard::release()
+{
+    for (Species *s : {&up, &dn}) {
+        dfree(s->ptr);
+        dfree(s->col);
+        dfree(s->val);
+        s->ptr = nullptr;
+        s->col = nullptr;
+        s->val = nullptr;
+    }
+    dfree(d_state_up);
+    dfree(d_state_dn);
+    dfree(d_uvec);
+    d_state_up = d_state_dn = nullptr;
+    d_uvec = nullptr;
+}
+
+} // namespace ed
+
+using namespace ed;
+
+extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *d_AxUP, const int32_t *d_AjUP,
+                                 const int32_t *d_stateUP, int64_t dim_dn, int S_dn, const double *d_AxDN,
+                                 const int32_t *d_AjDN, const int32_t *d_stateDN, int complex_tables, double U,
+                                 int64_t mask, const double *Uvec, ed_hubbard *out)
+{
+    if (!out || N < 1 || N > 31 || dim_up < 1 || dim_dn < 1 || S_up < 1 || S_dn < 1 || !d_AxUP || !d_AjUP ||
+        !d_AxDN || !d_AjDN || !d_stateUP || !d_stateDN)
+        return fail(ED_EINVAL, "ed_hubbard_create: bad arguments");
+    if (dim_dn > 0x7fffffffLL || dim_up > 0x7fffffffLL)
+        return fail(ED_EUNSUPPORTED, "ed_hubbard_create: species dimension beyond int32");
+    *out = nullptr;
+    Hubbard *h = new Hubbard();
+    h->N = N;
+    h->cplx_tables = complex_tables != 0;
+    hipStream_t st = nullptr;
+    SpeciesIn iu{dim_up, S_up, d_AxUP, d_AjUP}, idn{dim_dn, S_dn, d_AxDN, d_AjDN};
+    int s = compact_species(iu, h->cplx_tables, h->up, true, st);
+    if (s == ED_OK) s = compact_species(idn, h->cplx_tables, h->dn, false, st);
+    if (s == ED_OK) s = dmalloc((void **)&h->d_state_up, sizeof(int32_t) * (size_t)dim_up);
+    if (s == ED_OK) s = dmalloc((void **)&h->d_state_dn, sizeof(int32_t) * (size_t)dim_dn);
+    if (s == ED_OK && Uvec) {
+        s = dmalloc((void **)&h->d_uvec, sizeof(double) * 32);
+        if (s == ED_OK) {
+            double tmp[32] = {0};
+            for (int i = 0; i < N; ++i) tmp[i] = Uvec[i];
+            if (hipMemcpy(h->d_uvec, tmp, sizeof tmp, hipMemcpyHostToDevice) != hipSuccess)
+                s = fail(ED_EHIP, "ed_hubbard_create: Uvec upload failed");
+        }
+    }
+    if (s == ED_OK) {
+        if (hipMemcpy(h->d_state_up, d_stateUP, sizeof(int32_t) * (size_t)dim_up, hipMemcpyDeviceToDevice) !=
+                hipSuccess ||
+            hipMemcpy(h->d_state_dn, d_stateDN, sizeof(int32_t) * (size_t)dim_dn, hipMemcpyDeviceToDevice) !=
+                hipSuccess)
+            s = fail(ED_EHIP, "ed_hubbard_create: state copy failed");
+    }
+    if (s != ED_OK) {
+        h->release();
+        delete h;
+        return s;
+    }
+    h->U = U;
+    h->mask = (mask < 0) ? (int64_t)(((uint64_t)1 << N) - 1) : mask;
+    HvArgs &a = h->args;
+    a.dim_up = dim_up;
+    a.dim_dn = dim_dn;
+    a.up_ptr = h->up.ptr;
+    a.up_col = h->up.col;
+    a.up_val = h->up.val;
+    a.dn_sptr = h->dn.ptr;
+    a.dn_col = h->dn.col;
+    a.dn_val = h->dn.val;
+    a.state_up = h->d_state_up;
+    a.state_dn = h->d_state_dn;
+    a.U = U;
+    a.mask = h->mask;
+    a.Uvec = h->d_uvec;
+    a.has_diag = (U != 0.0 || h->d_uvec) ? 1 : 0;
+    a.blocks_per_row = 1;
+    if (h->up.max_nnz > 512) {
+        h->release();
+        delete h;
+        return fail(ED_EUNSUPPORTED, "ed_hubbard_create: a row has %d > 512 nonzero slots", h->up.max_nnz);
+    }
+    *out = reinterpret_cast<ed_hubbard>(h);
+    return ED_OK;
+}
+
+extern "C" int ed_hubbard_destroy(ed_hubbard hh)
+{
+    Hubbard *h = reinterpret_cast<Hubbard *>(hh);
+    if (!h) return ED_OK;
+    h->release();
+    delete h;
+    return ED_OK;
+}
+
+extern "C" int ed_hubbard_info(ed_hubbard hh, int64_t *dim_up, int64_t *dim_dn, int *complex_tables)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h) return fail(ED_EINVAL, "ed_hubbard_info: null handle");
+    if (dim_up) *dim_up = h->args.dim_up;
+    if (dim_dn) *dim_dn = h->args.dim_dn;
+    if (complex_tables) *complex_tables = h->cplx_tables ? 1 : 0;
+    return ED_OK;
+}
+
+extern "C" int ed_hubbard_apply(ed_hubbard hh, int complex_vectors, const void *d_x, void *d_y, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_x || !d_y) return fail(ED_EINVAL, "ed_hubbard_apply: null argument");
+    if (d_x == d_y) return fail(ED_EINVAL, "ed_hubbard_apply: x and y must differ");
+    return hv_plain(*h, complex_vectors != 0, d_x, d_y, stream_of(stream));
+}

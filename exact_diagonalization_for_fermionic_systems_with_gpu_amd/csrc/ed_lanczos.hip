@@ -1,0 +1,414 @@
+// ed_lanczos.hip -- host driver of the FP64 Lanczos ground-state solver and the
+// fixed-length tridiagonalisation used by the continued fraction.
+//
+// Reference: Lanczos<T>::eigen / tridiagonal (MBL_test/include/HubMatrix_GPU.cuh:541-613;
+// complex twin in TotalSpin/include/LanczosSolver.cuh:111-238).  The recurrence, the
+// anorm update, the N%10==9 convergence test |b_{N+1} z_N| < tol*anorm && |z_0| > tol*anorm
+// and the Ritz vector rebuilt by re-running the recurrence from the saved start are the
+// reference's; what changes is where the work runs:
+//   * one Lanczos step is two kernels: the H.v with the 1/b scaling, the -b v_{N-1}
+//     term and the <v_N, w> partials fused in (ed_hubbard.hip), then f = w - a v_N fused
+//     with the |f|^2 partials (ed_blas.hip).  Three resident vectors rotate roles.
+//   * a and b live on the device; the host reads them only at the convergence checks,
+//     so steps between checks are enqueued without a host round trip.
+//   * the tridiagonal eigenproblem (LAPACK dstedc in the reference) is solved on the
+//     host by Sturm bisection for the lowest eigenvalue and inverse iteration for its
+//     vector; all eigenvalues by the implicit QL method.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "ed_common.hpp"
+#include "ed_hubbard.hpp"
+
+namespace ed {
+
+// ---- symmetric tridiagonal eigen-solvers (host) ------------------------------------
+
+// number of eigenvalues below x (Sturm sequence of the LDL^T pivots)
+static int sturm_below(int n, const double *d, const double *e, double x)
+{
+    int c = 0;
+    double q = d[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < n; ++i) {
+        if (q == 0.0) q = std::numeric_limits<double>::min();
+        q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+        if (q < 0) ++c;
+    }
+    return c;
+}
+
+static double bisect_kth(int n, const double *d, const double *e, int k)
+{
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+        lo = std::fmin(lo, d[i] - r);
+        hi = std::fmax(hi, d[i] + r);
+    }
+    const double pad = 1e-12 * (hi - lo + 1.0);
+    lo -= pad;
+    hi += pad;
+    for (int it = 0; it < 256; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;
+        if (sturm_below(n, d, e, mid) > k) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// inverse iteration on T - lam I, elimination with partial pivoting; z normalised, sign
+// such that the largest component is positive
+static void inverse_iteration(int n, const double *d, const double *e, double lam, double *z)
+{
+    if (n == 1) {
+        z[0] = 1.0;
+        return;
+    }
+    std::vector<double> lo(n), dg(n), up(n), up2(n, 0.0), x(n, 1.0);
+    std::vector<char> sw(n, 0);
+    double scale = std::fabs(lam);
+    for (int i = 0; i < n; ++i) scale = std::fmax(scale, std::fabs(d[i]));
+    const double tiny = std::numeric_limits<double>::epsilon() * (scale + 1e-300);
+    for (int i = 0; i < n; ++i) {
+        dg[i] = d[i] - lam;
+        lo[i] = i < n - 1 ? e[i] : 0.0;
+        up[i] = i < n - 1 ? e[i] : 0.0;
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        if (std::fabs(dg[i]) >= std::fabs(lo[i])) {
+            if (dg[i] == 0.0) dg[i] = tiny;
+            const double m = lo[i] / dg[i];
+            lo[i] = m;
+            dg[i + 1] -= m * up[i];
+        } else {
+            const double m = dg[i] / lo[i];
+            dg[i] = lo[i];
+            lo[i] = m;
+            const double t = up[i];
+            up[i] = dg[i + 1];
+            dg[i + 1] = t - m * dg[i + 1];
+            if (i < n - 2) {
+                up2[i] = up[i + 1];
+                up[i + 1] = -m * up[i + 1];
+            }
+            sw[i] = 1;
+        }
+    }
+    if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
+    for (int sweep = 0; sweep < 3; ++sweep) {
+        for (int i = 0; i < n - 1; ++i) {
+            if (!sw[i]) {
+                x[i + 1] -= lo[i] * x[i];
+            } else {
+                const double t = x[i];
+                x[i] = x[i + 1];
+                x[i + 1] = t - lo[i] * x[i];
+            }
+        }
+        x[n - 1] /= dg[n - 1];
+        x[n - 2] = (x[n - 2] - up[n - 2] * x[n - 1]) / dg[n - 2];
+        for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - up[i] * x[i + 1] - up2[i] * x[i + 2]) / dg[i];
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += x[i] * x[i];
+        s = 1.0 / std::sqrt(s);
+        for (int i = 0; i < n; ++i) x[i] *= s;
+    }
+    int imax = 0;
+    for (int i = 1; i < n; ++i)
+        if (std::fabs(x[i]) > std::fabs(x[imax])) imax = i;
+    const double sg = x[imax] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) z[i] = sg * x[i];
+}
+
+// implicit QL with Wilkinson-type shifts (eigenvalues only), Golub & Van Loan sec. 8.3
+static void ql_eigvals(int n, double *d, double *e /* n entries, e[n-1] scratch */)
+{
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+            }
+            if (m != l) {
+                if (++iter > 60) break;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+}
+
+// ---- Lanczos workspace -------------------------------------------------------------
+struct Work {
+    double *buf[2] = {nullptr, nullptr};
+    double *partial = nullptr;
+    double *adev = nullptr, *bdev = nullptr, *ddev = nullptr;
+    ~Work()
+    {
+        dfree(buf[0]);
+        dfree(buf[1]);
+        dfree(partial);
+        dfree(adev);
+        dfree(bdev);
+        dfree(ddev);
+    }
+};
+
+static int alloc_work(Work &w, const Hubbard &h, int64_t nreal, int maxN, bool vecs)
+{
+    if (vecs) {
+        ED_TRY(dmalloc((void **)&w.buf[0], sizeof(double) * (size_t)nreal));
+        ED_TRY(dmalloc((void **)&w.buf[1], sizeof(double) * (size_t)nreal));
+    }
+    const int64_t np = std::max<int64_t>(hv_num_blocks(h), blas_num_blocks(nreal));
+    ED_TRY(dmalloc((void **)&w.partial, sizeof(double) * (size_t)np));
+    ED_TRY(dmalloc((void **)&w.adev, sizeof(double) * (size_t)(maxN + 2)));
+    ED_TRY(dmalloc((void **)&w.bdev, sizeof(double) * (size_t)(maxN + 3)));
+    ED_TRY(dmalloc((void **)&w.ddev, sizeof(double) * (size_t)(maxN + 2)));
+    return ED_OK;
+}
+
+// One step N: (A = f, B = v_{N-1}) -> (C = v_N, B = f_{N+1}); a[N], b[N+1] on the device.
+static int lanczos_step(const Hubbard &h, bool cvec, int64_t nreal, double *A, double *B, double *C, Work &w,
+                        int N, bool reorth, hipStream_t st)
+{
+    int64_t nb;
+    ED_TRY(hv_lanczos(h, cvec, A, B, C, B, w.bdev, N, w.partial, st, &nb));
+    ED_TRY(finalize_sum(w.partial, nb, w.adev + N, 0, st));
+    ED_TRY(lanczos_update(nreal, C, B, w.adev, N, w.partial, &nb, st));
+    if (reorth) {
+        // second Gram-Schmidt pass against v_N (Daniel-Gragg-Kaufman-Stewart correction):
+        // delta = <v_N, f>; f -= delta v_N; the tridiagonal entry is a[N] + delta
+        ED_TRY(dot_partials(nreal, C, B, w.partial, &nb, st));
+        ED_TRY(finalize_sum(w.partial, nb, w.ddev + N, 0, st));
+        ED_TRY(lanczos_update(nreal, C, B, w.ddev, N, w.partial, &nb, st));
+    }
+    ED_TRY(finalize_sum(w.partial, nb, w.bdev + N + 1, 1, st));
+    return ED_OK;
+}
+
+} // namespace ed
+
+using namespace ed;
+
+extern "C" int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z)
+{
+    if (n < 1 || !d || (n > 1 && !e) || !lambda) return fail(ED_EINVAL, "ed_tridiag_lowest: bad arguments");
+    const double lam = bisect_kth(n, d, e, 0);
+    *lambda = lam;
+    if (z) inverse_iteration(n, d, e, lam, z);
+    return ED_OK;
+}
+
+extern "C" int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w)
+{
+    if (n < 1 || !d || (n > 1 && !e) || !w) return fail(ED_EINVAL, "ed_tridiag_eigvals: bad arguments");
+    std::vector<double> dd(d, d + n), ee(n, 0.0);
+    for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
+    ql_eigvals(n, dd.data(), ee.data());
+    std::sort(dd.begin(), dd.end());
+    for (int i = 0; i < n; ++i) w[i] = dd[i];
+    return ED_OK;
+}
+
+extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f,
+                                double *eval, void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_f || !d_evec || !eval || d_f == d_evec) return fail(ED_EINVAL, "ed_lanczos_eigen: bad arguments");
+    const bool cvec = complex_vectors != 0;
+    if (h->cplx_tables && !cvec) return fail(ED_EINVAL, "ed_lanczos_eigen: complex tables need complex vectors");
+    const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
+    const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
+    const int every = cfg && cfg->check_every > 0 ? cfg->check_every : 10;
+    const bool reorth = cfg && cfg->reorth;
+    hipStream_t st = stream_of(stream);
+    const int64_t n = h->args.dim_up * h->args.dim_dn, nreal = cvec ? 2 * n : n;
+
+    Work w;
+    ED_TRY(alloc_work(w, *h, nreal, maxN, true));
+    double *ev = (double *)d_evec;
+    // evec = f keeps the start vector for the rebuild (HubMatrix_GPU.cuh:566)
+    ED_HIP(hipMemcpyAsync(ev, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    int64_t nb;
+    ED_TRY(dot_partials(nreal, (const double *)d_f, (const double *)d_f, w.partial, &nb, st));
+    ED_TRY(finalize_sum(w.partial, nb, w.bdev, 1, st));
+
+    std::vector<double> a(maxN + 2, 0.0), araw(maxN + 2, 0.0), dl(maxN + 2, 0.0), b(maxN + 3, 0.0), z;
+    double anorm = 0.0;
+    int lastN = -1, status = ED_ENOCONV;
+    int processed = 0; // a[0..processed-1] folded into anorm
+    double *A = (double *)d_f, *B = w.buf[0], *C = w.buf[1];
+    int N;
+    bool stop = false;
+    for (N = 0; N < maxN && !stop; ++N) {
+        ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, st));
+        double *t = A;
+        A = B;
+        B = C;
+        C = t;
+        if (N % every == every - 1 || N == maxN - 1) {
+            ED_HIP(hipMemcpyAsync(araw.data(), w.adev, sizeof(double) * (size_t)(N + 1), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipMemcpyAsync(b.data(), w.bdev, sizeof(double) * (size_t)(N + 2), hipMemcpyDeviceToHost, st));
+            if (reorth)
+                ED_HIP(hipMemcpyAsync(dl.data(), w.ddev, sizeof(double) * (size_t)(N + 1), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipStreamSynchronize(st));
+            if (reorth)
+                for (int k = 0; k <= N; ++k) a[k] = araw[k] + dl[k];
+            else
+                for (int k = 0; k <= N; ++k) a[k] = araw[k];
+            int M = N; // last usable step
+            for (int k = processed; k <= N; ++k) {
+                if (!(b[k + 1] > 0.0) || !std::isfinite(a[k])) { // breakdown / invariant subspace
+                    M = k;
+                    break;
+                }
+            }
+            for (int k = processed; k <= M; ++k) {
+                if (k == 0) anorm = std::fabs(a[0] + b[1]);
+                else anorm = std::fmax(anorm, b[k] + std::fabs(a[k] + b[k + 1]));
+            }
+            processed = M + 1;
+            z.assign(M + 1, 0.0);
+            double lam;
+            ed_tridiag_lowest(M + 1, a.data(), b.data() + 1, &lam, z.data());
+            *eval = lam;
+            lastN = M;
+            if (M < N || !(b[M + 1] > 0.0)) {
+                status = ED_OK; // exact invariant subspace: T_{M+1} eigenvalues are exact
+                stop = true;
+            } else if (std::fabs(b[M + 1] * z[M]) < tol * anorm && std::fabs(z[0]) > tol * anorm) {
+                status = ED_OK;
+                stop = true;
+            }
+        }
+    }
+    if (lastN < 0) return fail(ED_EINVAL, "ed_lanczos_eigen: no steps run");
+    const int Nk = lastN;
+
+    // Ritz vector: rerun the recurrence from the saved start (HubMatrix_GPU.cuh:600-612)
+    A = (double *)d_f;
+    B = w.buf[0];
+    C = w.buf[1];
+    ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
+    for (int i = 0; i < Nk + 1; ++i) {
+        ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
+        ED_TRY(ritz_update(nreal, C, B, ev, w.adev, z[i], i, st));
+        if (reorth) {
+            int64_t nbk;
+            ED_TRY(lanczos_update(nreal, C, B, w.ddev, i, w.partial, &nbk, st));
+        }
+        double *t = A;
+        A = B;
+        B = C;
+        C = t;
+    }
+    ED_TRY(ed_normalize(n, complex_vectors, ev, nullptr, stream));
+    ED_HIP(hipStreamSynchronize(st));
+    if (a_out) memcpy(a_out, a.data(), sizeof(double) * (size_t)(Nk + 1));
+    if (b_out) memcpy(b_out, b.data(), sizeof(double) * (size_t)(Nk + 2));
+    if (nsteps) *nsteps = Nk + 1;
+    if (status != ED_OK) return fail(status, "ed_lanczos_eigen: convergence test not met in %d steps", maxN);
+    return ED_OK;
+}
+
+extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int nsteps, const void *d_f,
+                                      double *a_out, double *b_out, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_f || nsteps < 1 || !a_out || !b_out) return fail(ED_EINVAL, "ed_lanczos_tridiagonal: bad arguments");
+    const bool cvec = complex_vectors != 0;
+    if (h->cplx_tables && !cvec) return fail(ED_EINVAL, "ed_lanczos_tridiagonal: complex tables need complex vectors");
+    hipStream_t st = stream_of(stream);
+    const int64_t n = h->args.dim_up * h->args.dim_dn, nreal = cvec ? 2 * n : n;
+    Work w;
+    ED_TRY(alloc_work(w, *h, nreal, nsteps, true));
+    double *f = nullptr;
+    ED_TRY(dmalloc((void **)&f, sizeof(double) * (size_t)nreal));
+    int s = ED_OK;
+    do {
+        if (hipMemcpyAsync(f, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_lanczos_tridiagonal: copy failed");
+            break;
+        }
+        int64_t nb;
+        if ((s = dot_partials(nreal, f, f, w.partial, &nb, st))) break;
+        if ((s = finalize_sum(w.partial, nb, w.bdev, 1, st))) break;
+        double *A = f, *B = w.buf[0], *C = w.buf[1];
+        for (int N = 0; N < nsteps && s == ED_OK; ++N) {
+            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, st);
+            double *t = A;
+            A = B;
+            B = C;
+            C = t;
+        }
+        if (s) break;
+        if (hipMemcpyAsync(a_out, w.adev, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(b_out, w.bdev, sizeof(double) * (size_t)(nsteps + 1), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            s = fail(ED_EHIP, "ed_lanczos_tridiagonal: readback failed");
+    } while (0);
+    dfree(f);
+    return s;
+}
+
+extern "C" int ed_continued_fraction(int n, const double *a, const double *b, double energy, double norm, int nw,
+                                     const double *w_ri, double *g_ri)
+{
+    if (n < 1 || !a || !b || nw < 0 || (nw > 0 && (!w_ri || !g_ri)))
+        return fail(ED_EINVAL, "ed_continued_fraction: bad arguments");
+    for (int k = 0; k < nw; ++k) {
+        const double wr = w_ri[2 * k] + energy, wi = w_ri[2 * k + 1];
+        // g = 1/(w + E - a[n-1]) then g = 1/(w + E - a[i] - g b[i+1]^2) for i = n-2..0
+        double gr, gi;
+        {
+            const double dr = wr - a[n - 1], di = wi;
+            const double den = dr * dr + di * di;
+            gr = dr / den;
+            gi = -di / den;
+        }
+        for (int i = n - 2; i >= 0; --i) {
+            const double bb = b[i + 1] * b[i + 1];
+            const double dr = wr - a[i] - gr * bb, di = wi - gi * bb;
+            const double den = dr * dr + di * di;
+            gr = dr / den;
+            gi = -di / den;
+        }
+        g_ri[2 * k] = norm * gr;
+        g_ri[2 * k + 1] = norm * gi;
+    }
+    return ED_OK;
+}

@@ -1,0 +1,134 @@
+"""H_TBq -- single-species basis and ELL operator tables, on the GPU.
+
+Mirror of class H_TBq (MBL_test/include/fermi_operator.h:42-242) with the same
+method names and argument meaning.  The reference fills host tables inside each
+Make_* call; here the calls are recorded in order and the tables are built by
+one GPU kernel (ed_tables_build) the first time they are needed -- replaying the
+calls in order gives the reference's slot placement exactly.
+
+Public fields as in the reference: N, Q, dim (=2^N), count, state, address, Ax, Aj
+(device tensors; .cpu() gives the host view the drivers index, e.g.
+Hq.address[...] in Hubbard_EE_test.cu:49).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import call, dptr, stream_ptr
+
+Z_SLOTS = 10  # fermi_operator.h:11
+
+
+def count1(state: int) -> int:
+    """fermi_operator.h:19-24"""
+    return bin(int(state)).count("1")
+
+
+def count1_bt_sites(basis: int, i_site: int, j_site: int) -> int:
+    """fermi_operator.h:26-40: occupied sites strictly between i and j"""
+    lo, hi = min(i_site, j_site), max(i_site, j_site)
+    return count1((int(basis) >> (lo + 1)) & ((1 << (hi - lo - 1)) - 1))
+
+
+class H_TBq:
+    """H_TBq(N, Q) (fermi_operator.h:80-88); H_TBq(N) is the full Fock space (90-98).
+
+    complex_tables=True stores complex matrix elements (needed for Make_H_phase,
+    the Peierls-twisted hop that has no counterpart in the reference)."""
+
+    def __init__(self, N: int, Q: int | None = None, complex_tables: bool = False, device="cuda"):
+        self.N = int(N)
+        self.full = Q is None
+        self.Q = self.N if Q is None else int(Q)
+        self.dim = 1 << self.N
+        self.complex_tables = bool(complex_tables)
+        self.device = torch.device(device)
+        cnt = C.c_int64()
+        call("ed_basis_count", self.N, -1 if self.full else self.Q, C.byref(cnt))
+        self.count = cnt.value
+        self.slots = Z_SLOTS * self.Q + 1
+        self.state = torch.empty(self.count, dtype=torch.int32, device=self.device)
+        self.address = torch.empty(self.dim, dtype=torch.int32, device=self.device)
+        call("ed_basis_build", self.N, -1 if self.full else self.Q, dptr(self.state), dptr(self.address),
+             stream_ptr())
+        self.ops: list[tuple] = []
+        self._Ax = None
+        self._Aj = None
+        self._built = -1
+
+    # --- the operator-insert methods (call order is kept) ------------------------------
+    def Make_H(self, i_site, j_site, t):
+        """fermi_operator.h:100-115"""
+        self.ops.append((_lib.ED_OP_HOP, int(i_site), int(j_site), float(t), 0.0))
+
+    def Make_J(self, i_site, j_site, t):
+        """fermi_operator.h:117-138"""
+        self.ops.append((_lib.ED_OP_CURRENT, int(i_site), int(j_site), float(t), 0.0))
+
+    def Make_N(self, i_site, v):
+        """fermi_operator.h:140-149"""
+        self.ops.append((_lib.ED_OP_DENSITY, int(i_site), int(i_site), float(v), 0.0))
+
+    def Insert_2Nop(self, i_site, j_site, v):
+        """fermi_operator.h:216-225"""
+        self.ops.append((_lib.ED_OP_DENSITY2, int(i_site), int(j_site), float(v), 0.0))
+
+    def Make_H_phase(self, i_site, j_site, t, phase):
+        """t e^{i phase} c+_i c_j + h.c. (complex tables only)."""
+        if not self.complex_tables:
+            raise ValueError("Make_H_phase needs H_TBq(..., complex_tables=True)")
+        self.ops.append((_lib.ED_OP_HOP_PHASE, int(i_site), int(j_site), float(t), float(phase)))
+
+    def add_ops(self, ops):
+        for o in ops:
+            self.ops.append((int(o[0]), int(o[1]), int(o[2]), float(o[3]), float(o[4]) if len(o) > 4 else 0.0))
+
+    def clear(self):
+        """fermi_operator.h:237-241 (tables reset, basis kept)"""
+        self.ops = []
+        self._built = -1
+
+    # --- tables ---------------------------------------------------------------------------
+    def _build(self):
+        if self._built == len(self.ops) and self._Ax is not None:
+            return
+        n = self.count * self.slots
+        if self._Ax is None:
+            self._Ax = torch.empty(n * (2 if self.complex_tables else 1), dtype=torch.float64, device=self.device)
+            self._Aj = torch.empty(n, dtype=torch.int32, device=self.device)
+        arr = (_lib.EdOp * max(1, len(self.ops)))()
+        for k, (kind, i, j, v, ph) in enumerate(self.ops):
+            arr[k] = _lib.EdOp(kind, i, j, 0, v, ph)
+        call("ed_tables_build", self.N, -1 if self.full else self.Q, dptr(self.state), dptr(self.address),
+             self.count, arr, len(self.ops), int(self.complex_tables), dptr(self._Ax), dptr(self._Aj),
+             stream_ptr())
+        self._built = len(self.ops)
+
+    @property
+    def Ax(self):
+        self._build()
+        if self.complex_tables:
+            return torch.view_as_complex(self._Ax.view(-1, 2))
+        return self._Ax
+
+    @property
+    def Aj(self):
+        self._build()
+        return self._Aj
+
+    def raw_tables(self):
+        self._build()
+        return self._Ax, self._Aj
+
+    def Write(self):
+        """Dense count x count matrix of the operator (fermi_operator.h:227-235), on the device."""
+        Ax, Aj = self.Ax, self.Aj
+        dt = torch.complex128 if self.complex_tables else torch.float64
+        M = torch.zeros(self.count * self.count, dtype=dt, device=self.device)
+        rows = torch.arange(self.count, device=self.device, dtype=torch.int64).repeat(self.slots)
+        nz = Ax != 0
+        M.index_add_(0, (rows * self.count + Aj.to(torch.int64))[nz], Ax[nz])
+        return M.view(self.count, self.count)

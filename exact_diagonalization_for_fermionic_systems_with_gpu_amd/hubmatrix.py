@@ -1,0 +1,242 @@
+"""Hubbard operator, Thrust_Lin BLAS-1, Lanczos and the continued-fraction correlator.
+
+Mirror of MBL_test/include/HubMatrix_GPU.cuh with the reference's class and method
+names (struct Hubbard :30-182, Thrust_Lin :291-471, Lanczos :477-613,
+correlator_continued_fraction / compute_sigma_omega :820-849).  Vectors are torch
+tensors on the GPU (float64 or complex128, length dimUP*dimDN, row-major
+[i_up*dimDN + i_dn]) standing in for thrust::device_vector; all arithmetic runs in
+libedhip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from ._lib import LanczosCfg, call, dptr, stream_ptr, EdError, ED_ENOCONV
+from .fermi_operator import H_TBq
+
+
+def _is_complex(t: torch.Tensor) -> int:
+    return 1 if t.is_complex() else 0
+
+
+def _raw(t: torch.Tensor):
+    return dptr(t)
+
+
+class Hubbard:
+    """Hubbard(HqUP, HqDN, U=0.0[, dilution]) (HubMatrix_GPU.cuh:46-95).
+
+    Uvec (extension): per-site interaction U_i replacing U*popcount(up&dn&dilution)."""
+
+    def __init__(self, HqUP: H_TBq, HqDN: H_TBq, U: float = 0.0, dilution: int | None = None, Uvec=None):
+        if HqUP.N != HqDN.N:
+            raise ValueError("both species must live on the same N sites")
+        self.N = HqUP.N
+        self.dimUP, self.dimDN = HqUP.count, HqDN.count
+        self.numcolsUP, self.numcolsDN = HqUP.slots, HqDN.slots
+        self.U = float(U)
+        self.is_diluted = dilution is not None
+        self.dilution = ((1 << self.N) - 1) if dilution is None else int(dilution)
+        self.complex_tables = bool(HqUP.complex_tables or HqDN.complex_tables)
+        if HqUP.complex_tables != HqDN.complex_tables:
+            raise ValueError("both species must use the same table type")
+        AxU, AjU = HqUP.raw_tables()
+        AxD, AjD = HqDN.raw_tables()
+        uv = None
+        if Uvec is not None:
+            uv = np.ascontiguousarray(Uvec, dtype=np.float64)
+            if uv.shape != (self.N,):
+                raise ValueError("Uvec needs N entries")
+        self._uv = uv
+        h = C.c_void_p()
+        call("ed_hubbard_create", self.N, self.dimUP, HqUP.slots, dptr(AxU), dptr(AjU), dptr(HqUP.state),
+             self.dimDN, HqDN.slots, dptr(AxD), dptr(AjD), dptr(HqDN.state), int(self.complex_tables), self.U,
+             self.dilution, None if uv is None else uv.ctypes.data_as(C.c_void_p), C.byref(h))
+        self._h = h
+        self.dim = self.dimUP * self.dimDN
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                call("ed_hubbard_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def Hubbard_mv(self, x: torch.Tensor, y: torch.Tensor):
+        """y = H x (HubMatrix_GPU.cuh:163-182); y is overwritten.  Asynchronous."""
+        self._check(x, y)
+        call("ed_hubbard_apply", self._h, _is_complex(x), _raw(x), _raw(y), stream_ptr())
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        y = torch.empty_like(x)
+        self.Hubbard_mv(x, y)
+        return y
+
+    def _check(self, *ts):
+        for t in ts:
+            if t.numel() != self.dim or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"vector must be a contiguous CUDA tensor of {self.dim} elements")
+            if t.dtype not in (torch.float64, torch.complex128):
+                raise ValueError("vectors are float64 or complex128")
+        if self.complex_tables and not ts[0].is_complex():
+            raise ValueError("complex tables need complex128 vectors")
+
+
+class Thrust_Lin:
+    """Vector BLAS-1 of HubMatrix_GPU.cuh:291-471.  norm() returns the SQUARED norm as in
+    the reference (:377-386); ax/axpy return new vectors (:388-449)."""
+
+    def __init__(self, dimUP: int, dimDN: int):
+        self.dimUP, self.dimDN = int(dimUP), int(dimDN)
+
+    @staticmethod
+    def thrust_Dotprod(v1: torch.Tensor, v2: torch.Tensor):
+        out = (C.c_double * 2)()
+        call("ed_dot", v1.numel(), _is_complex(v1), _raw(v1), _raw(v2), out, stream_ptr())
+        if v1.is_complex():
+            return complex(out[0], out[1])
+        return out[0]
+
+    @staticmethod
+    def norm(v: torch.Tensor) -> float:
+        out = (C.c_double * 2)()
+        call("ed_dot", v.numel(), _is_complex(v), _raw(v), _raw(v), out, stream_ptr())
+        return out[0]
+
+    @staticmethod
+    def normalize(v: torch.Tensor) -> float:
+        n2 = C.c_double()
+        call("ed_normalize", v.numel(), _is_complex(v), _raw(v), C.byref(n2), stream_ptr())
+        return n2.value
+
+    @staticmethod
+    def ax(s: float, v: torch.Tensor) -> torch.Tensor:
+        w = v.clone()
+        call("ed_scal", w.numel(), _is_complex(w), float(s), _raw(w), stream_ptr())
+        return w
+
+    @staticmethod
+    def axpy(s: float, v1: torch.Tensor, v2: torch.Tensor) -> torch.Tensor:
+        w = v2.clone()
+        call("ed_axpy", w.numel(), _is_complex(w), float(s), _raw(v1), _raw(w), stream_ptr())
+        return w
+
+
+class Lanczos(Thrust_Lin):
+    """Lanczos<T> (HubMatrix_GPU.cuh:477-613).  a, b are host lists; b starts as [0.0]."""
+
+    def __init__(self, dimUP: int, dimDN: int, max_steps: int = 1000, tol: float | None = None,
+                 check_every: int = 10, reorth: bool = False):
+        super().__init__(dimUP, dimDN)
+        self.a: list[float] = []
+        self.b: list[float] = [0.0]
+        self.max_steps = int(max_steps)
+        self.tol = 0.01 * math.sqrt(np.finfo(float).eps) if tol is None else float(tol)
+        self.check_every = int(check_every)
+        self.reorth = bool(reorth)
+        self.nsteps = 0
+        self.converged = False
+
+    def clear(self):
+        """HubMatrix_GPU.cuh:523-529"""
+        self.a = []
+        self.b = [0.0]
+
+    def random_vector(self, f: torch.Tensor, seed: int = 235):
+        """GenRand (seed 235) + normalize (HubMatrix_GPU.cuh:504-539)"""
+        call("ed_random_vector", f.numel(), seed, _is_complex(f), _raw(f), 1, stream_ptr())
+
+    def eigen(self, Hub: Hubbard, f: torch.Tensor, evec: torch.Tensor) -> float:
+        """Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613): returns eval, fills evec.  f is clobbered.
+        Raises EdError(ED_ENOCONV) if the convergence test is not met (evec/eval then hold the
+        last Ritz pair)."""
+        Hub._check(f, evec)
+        cfg = LanczosCfg(self.max_steps, self.tol, self.check_every, int(self.reorth))
+        ev = C.c_double()
+        a = (C.c_double * (self.max_steps + 2))()
+        b = (C.c_double * (self.max_steps + 3))()
+        ns = C.c_int()
+        from ._lib import lib
+        st = lib().ed_lanczos_eigen(Hub.handle, _is_complex(f), C.byref(cfg), _raw(f), C.byref(ev), _raw(evec),
+                                    a, b, C.byref(ns), stream_ptr())
+        n = ns.value
+        self.a = [a[k] for k in range(n)]
+        self.b = [b[k] for k in range(n + 1)]
+        self.nsteps = n
+        self.converged = st == 0
+        self.eval = ev.value
+        if st not in (0, ED_ENOCONV):
+            from ._lib import check
+            check(st)
+        if st == ED_ENOCONV:
+            raise EdError(st, f"Lanczos convergence test not met in {self.max_steps} steps (E0={ev.value})")
+        return ev.value
+
+    def tridiagonal(self, Hub: Hubbard, f: torch.Tensor, maxN: int = 200):
+        """Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): appends maxN a's and b's."""
+        Hub._check(f)
+        a = (C.c_double * maxN)()
+        b = (C.c_double * (maxN + 1))()
+        call("ed_lanczos_tridiagonal", Hub.handle, _is_complex(f), maxN, _raw(f), a, b, stream_ptr())
+        self.b[0] = b[0]
+        self.a += [a[k] for k in range(maxN)]
+        self.b += [b[k] for k in range(1, maxN + 1)]
+
+
+def correlator_continued_fraction(lanctri: Lanczos, H: Hubbard, nw: int, w, energy: float, phi: torch.Tensor,
+                                  maxN: int = 200):
+    """HubMatrix_GPU.cuh:820-835: g(w) = |phi|^2/(w+E-a0-b1^2/(w+E-a1-...)).  phi is
+    normalized in place (as in the reference).  Returns g as a complex numpy array."""
+    norm = lanctri.norm(phi)
+    lanctri.normalize(phi)
+    lanctri.tridiagonal(H, phi, maxN)
+    a = np.ascontiguousarray(lanctri.a, dtype=np.float64)
+    b = np.ascontiguousarray(lanctri.b, dtype=np.float64)
+    w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128)[:nw])
+    g = np.empty_like(w)
+    call("ed_continued_fraction", len(a), a.ctypes.data_as(C.POINTER(C.c_double)),
+         b.ctypes.data_as(C.POINTER(C.c_double)), float(energy), float(norm),
+         len(w), w.ctypes.data_as(C.POINTER(C.c_double)), g.ctypes.data_as(C.POINTER(C.c_double)))
+    return g
+
+
+def compute_sigma_omega(lanctri: Lanczos, H: Hubbard, O: Hubbard, E: float, evec: torch.Tensor, w,
+                        maxN: int = 200):
+    """HubMatrix_GPU.cuh:837-849: Lambda(w) for the correlator of O on the ground state."""
+    lanctri.clear()
+    phi = torch.empty_like(evec)
+    O.Hubbard_mv(evec, phi)
+    return correlator_continued_fraction(lanctri, H, len(w), w, E, phi, maxN)
+
+
+# constants of the Drude drivers (drude_v2_cal_Kagome_randomHop.cu:14-21)
+NOMEGA = 8000
+OMEGA_STEP = 0.004
+DELTA = 0.01
+OMEGA_CUTOFF = 0.01
+
+
+def drude_weight_Jx(lanc: Lanczos, Hmat: Hubbard, Jxmat: Hubbard, Kxmat: Hubbard, eval_: float,
+                    evec: torch.Tensor, vol: float, nomega: int = NOMEGA, omega_step: float = OMEGA_STEP,
+                    delta: float = DELTA, omega_cutoff: float = OMEGA_CUTOFF):
+    """drude_weight_Jx (drude_v2_cal_Kagome_randomHop.cu:183-220): D = (-<K_x> - 2 Vertex)/VOL with
+    sigma(w) = -Im Lambda(w)/(pi w) from the 200-step continued fraction and
+    Vertex = sum_{w > cutoff} sigma(w) * step.  Returns (D, <K_x>, Lambda)."""
+    omega = omega_step * (np.arange(nomega, dtype=np.float64) + 1.0) + 1j * delta
+    Lam = compute_sigma_omega(lanc, Hmat, Jxmat, eval_, evec, omega)
+    sigma = -1.0 / (math.pi * omega.real) * Lam.imag
+    vertex = float(np.sum(np.where(omega.real > omega_cutoff, sigma * omega_step, 0.0)))
+    KV = torch.empty_like(evec)
+    Kxmat.Hubbard_mv(evec, KV)
+    kinetic = lanc.thrust_Dotprod(evec, KV)
+    kinetic = kinetic.real if isinstance(kinetic, complex) else kinetic
+    return (-kinetic - 2.0 * vertex) / vol, kinetic, Lam

@@ -1,0 +1,174 @@
+/*
+ * edhip.h -- C ABI of the MI355X exact-diagonalization library (libedhip.so).
+ *
+ * Plain pointers and sizes only.  "d_" arguments are device pointers (HIP device
+ * memory on the current device); everything else is host memory.  Streams are
+ * hipStream_t passed as void* (NULL = the default stream).  Every entry point
+ * returns an ed_status; ed_last_error() gives a thread-local message for the
+ * last failure on the calling thread.
+ *
+ * The reference has no FFI: its boundary is header-only C++ templates over
+ * thrust::device_vector.  Each declaration below cites the reference interface
+ * it replaces; the headers under include/ed/ re-create those C++ classes on top of
+ * this ABI.
+ *
+ * Layout conventions:
+ *   basis:   ascending integers with popcount Q (colex order); address[] has 2^N
+ *            entries, the rank inside the sector and 0 outside it.
+ *   tables:  column-major ELL, slot k of row a at k*count + a, S = 10*Q + 1 slots
+ *            (10*N + 1 for the full Fock space); complex tables interleave (re, im).
+ *   vectors: psi[i_up*dim_dn + i_dn], FP64 or complex128 (interleaved re, im).
+ */
+#ifndef EDHIP_H
+#define EDHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ED_OK = 0,
+    ED_EINVAL = 1,      /* bad argument */
+    ED_ENOMEM = 2,      /* device or host allocation failed */
+    ED_EBREAKDOWN = 3,  /* Lanczos breakdown that could not be handled */
+    ED_ENOCONV = 4,     /* Lanczos did not meet its convergence test within max_steps */
+    ED_EHIP = 5,        /* a HIP runtime call failed */
+    ED_EUNSUPPORTED = 6,
+    ED_ESLOT = 7        /* an ELL row ran out of slots (the reference writes past the table) */
+} ed_status;
+
+/* operator kinds for ed_tables_build, one per H_TBq method */
+typedef enum {
+    ED_OP_HOP = 0,       /* H_TBq::Make_H(i,j,t)        fermi_operator.h:100-115 */
+    ED_OP_CURRENT = 1,   /* H_TBq::Make_J(i,j,t)        fermi_operator.h:117-138 */
+    ED_OP_DENSITY = 2,   /* H_TBq::Make_N(i,v)          fermi_operator.h:140-149 */
+    ED_OP_DENSITY2 = 3,  /* H_TBq::Insert_2Nop(i,j,v)   fermi_operator.h:216-225 */
+    ED_OP_HOP_PHASE = 4  /* t e^{i phase} c+_i c_j + h.c. (complex tables; a Peierls twist --
+                            no counterpart in the reference, whose tables are real) */
+} ed_op_kind;
+
+typedef struct {
+    int32_t kind; /* ed_op_kind */
+    int32_t i, j; /* sites (j unused for ED_OP_DENSITY) */
+    int32_t pad;
+    double value; /* t or v */
+    double phase; /* ED_OP_HOP_PHASE only */
+} ed_op;
+
+const char *ed_last_error(void);
+int ed_version(void);
+int ed_device_sync(void);
+
+/* ---------------- basis + tables (fermi_operator.h) ---------------------------- */
+
+/* count = C(N,Q) (2^N when Q < 0: the full-Fock H_TBq(N), fermi_operator.h:90-98) */
+int ed_basis_count(int N, int Q, int64_t *count);
+
+/* H_TBq::Labeling / Full_Labeling (fermi_operator.h:44-62, ctor 80-98):
+ * d_state[count], d_address[2^N] (rank inside the sector, 0 elsewhere). */
+int ed_basis_build(int N, int Q, int32_t *d_state, int32_t *d_address, void *stream);
+
+/* H_TBq::Make_H / Make_J / Make_N / Insert_2Nop applied in call order with the
+ * stacking rule (fermi_operator.h:64-149,216-225).  ops is a host array.  The
+ * tables are cleared first (H_TBq::clear, fermi_operator.h:237-241), so one call
+ * builds a whole operator; d_Ax holds count*S doubles (2x for complex tables).
+ * Returns ED_ESLOT if a row overflows its S slots. */
+int ed_tables_build(int N, int Q, const int32_t *d_state, const int32_t *d_address, int64_t count,
+                    const ed_op *ops, int nops, int complex_tables, double *d_Ax, int32_t *d_Aj, void *stream);
+
+/* ---------------- Hubbard operator (HubMatrix_GPU.cuh:30-182) ---------------- */
+
+typedef struct ed_hubbard_s *ed_hubbard;
+
+/* Hubbard(HqUP, HqDN, U [, dilution]) (HubMatrix_GPU.cuh:46-95).  Reads the ELL
+ * tables (device) once and keeps a compact copy; the caller's tables may be freed
+ * afterwards.  mask = dilution (-1: all sites).  Uvec: optional host array of N
+ * per-site interactions that replaces U*popcount(up&dn&mask). */
+int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *d_AxUP, const int32_t *d_AjUP,
+                      const int32_t *d_stateUP, int64_t dim_dn, int S_dn, const double *d_AxDN,
+                      const int32_t *d_AjDN, const int32_t *d_stateDN, int complex_tables, double U, int64_t mask,
+                      const double *Uvec, ed_hubbard *out);
+int ed_hubbard_destroy(ed_hubbard h);
+int ed_hubbard_info(ed_hubbard h, int64_t *dim_up, int64_t *dim_dn, int *complex_tables);
+
+/* Hubbard::Hubbard_mv<M>(x, y) (HubMatrix_GPU.cuh:163-182): y = H x, y overwritten,
+ * x != y.  complex_vectors selects M = complex<double>.  Asynchronous on stream. */
+int ed_hubbard_apply(ed_hubbard h, int complex_vectors, const void *d_x, void *d_y, void *stream);
+
+/* ---------------- vector BLAS-1 (Thrust_Lin, HubMatrix_GPU.cuh:291-471) -------- */
+
+/* GenRand + random_vector (HubMatrix_GPU.cuh:504-539): minstd_rand seeded with
+ * `seed`, element idx = draw idx+1, U(-1,1); complex: two consecutive draws
+ * (LanczosSolver.cuh:38-55).  normalize != 0 scales to unit norm. */
+int ed_random_vector(int64_t n, uint32_t seed, int complex_vectors, void *d_v, int normalize, void *stream);
+/* thrust_Dotprod (HubMatrix_GPU.cuh:451-471): out[0] = Re<a,b>, out[1] = Im<a,b> (0 for
+ * real); deterministic reduction; synchronises the stream. */
+int ed_dot(int64_t n, int complex_vectors, const void *d_a, const void *d_b, double *out, void *stream);
+/* y = alpha*x + y (real alpha) */
+int ed_axpy(int64_t n, int complex_vectors, double alpha, const void *d_x, void *d_y, void *stream);
+/* x = alpha*x */
+int ed_scal(int64_t n, int complex_vectors, double alpha, void *d_x, void *stream);
+/* normalize (HubMatrix_GPU.cuh:366-375); returns the squared norm before scaling */
+int ed_normalize(int64_t n, int complex_vectors, void *d_x, double *norm2_out, void *stream);
+
+/* ---------------- Lanczos (HubMatrix_GPU.cuh:477-613) ------------------------- */
+
+typedef struct {
+    int max_steps;   /* maxN (reference 1000) */
+    double tol;      /* tolconv (reference 0.01*sqrt(eps)) */
+    int check_every; /* convergence test cadence (reference 10) */
+    int reorth;      /* 1: second Gram-Schmidt pass of f against v_N (DGKS correction) */
+} ed_lanczos_cfg;
+
+/* Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613).  d_f: start vector in (any norm),
+ * clobbered.  d_evec: normalized Ritz vector out, rebuilt by re-running the
+ * recurrence.  a_out/b_out (host, max_steps+2 entries, may be NULL) receive the
+ * tridiagonal; *nsteps = N+1 steps kept.  Returns ED_ENOCONV (with the last Ritz
+ * pair) when the test is not met by max_steps.  Synchronises the stream. */
+int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, double *eval,
+                     void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream);
+
+/* Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): nsteps fixed steps from d_f
+ * (not modified); a_out[nsteps], b_out[nsteps+1] with b_out[0] = |f|. */
+int ed_lanczos_tridiagonal(ed_hubbard h, int complex_vectors, int nsteps, const void *d_f, double *a_out,
+                           double *b_out, void *stream);
+
+/* lowest eigenpair of a symmetric tridiagonal (replaces LAPACKf77::dsted, MAGMA_LAPACK.h:296-326,
+ * for the Lanczos test): d[n], e[n-1]; z[n] (may be NULL) gets the unit eigenvector. */
+int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
+/* all eigenvalues ascending */
+int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
+
+/* ---------------- observables -------------------------------------------------- */
+
+/* TotalSpin::Hubbard (HubMatrix_TotalSpin.cuh:126-173): y = (S+S- + S-S+)/2 x, real or complex. */
+typedef struct ed_s2_s *ed_s2;
+int ed_s2_create(int N, int64_t dim_up, const int32_t *d_stateUP, const int32_t *d_addressUP, int64_t dim_dn,
+                 const int32_t *d_stateDN, const int32_t *d_addressDN, ed_s2 *out);
+int ed_s2_destroy(ed_s2 s);
+int ed_s2_apply(ed_s2 s, int complex_vectors, const void *d_x, void *d_y, void *stream);
+
+/* correlator_continued_fraction (HubMatrix_GPU.cuh:829-834), host: g[k] = norm / (w_k + E - a_0 - b_1^2/(...)) */
+int ed_continued_fraction(int n, const double *a, const double *b, double energy, double norm, int nw,
+                          const double *w_ri, double *g_ri);
+
+/* Entanglement entropy of psi for the bipartition A = sites in maskA (EEmatHubAB_GPU +
+ * cusolverDsvd/Zsvd + -sum s ln s, entangle_entropy_GPU.cuh:27-144, entangle_entropy.h:231-353,
+ * Hubbard_EE_test.cu:77-83).  The (A,B) reshaped psi is block-diagonal in (N_A^up, N_A^dn);
+ * each block rho_A = Psi Psi^T (Psi^H) is formed with FP64 MFMA and diagonalised
+ * (values only).  out[0] = S_vN = -sum lambda ln lambda; out[1] = S_ref = -sum sigma ln sigma
+ * with sigma from an SVD of each block (the reference's formula).  nsv_out (may be NULL):
+ * number of nonzero Schmidt values.  Synchronises the stream. */
+int ed_entanglement_entropy(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
+                            double *out, int64_t *nsv_out, void *stream);
+/* all eigenvalues of rho_A, descending, into lambda_out (capacity cap); *nlambda = total count */
+int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
+                             double *lambda_out, int64_t cap, int64_t *nlambda, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDHIP_H */
