@@ -1,0 +1,48 @@
+"""Entanglement entropy of a Hubbard state for a bipartition A|B of the sites.
+
+Mirror of EEmatHubAB_GPU (MBL_test/include/entangle_entropy_GPU.cuh:27-144) plus the
+singular-value step the drivers run on it (cusolverDsvd/Zsvd, entangle_entropy.h:231-353;
+S = -sum s ln s, Hubbard_EE_test.cu:77-83).  The reference materialises the whole
+rowD x colD reshaped matrix; libedhip works block by block in (N_A^up, N_A^dn), which has
+the same singular values (see ed_entangle.hip).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import call, dptr, stream_ptr
+from .fermi_operator import H_TBq
+
+
+class EEmatHubAB_GPU:
+    """EEmatHubAB_GPU(Asite, Asize, HqUP, HqDN) (entangle_entropy_GPU.cuh:34-51)."""
+
+    def __init__(self, Asite, Asize, HqUP: H_TBq, HqDN: H_TBq):
+        self.Asite = [int(a) for a in list(Asite)[:Asize]]
+        self.N, self.Qup, self.Qdn = HqUP.N, HqUP.Q, HqDN.Q
+        self.dimUP, self.dimDN = HqUP.count, HqDN.count
+        self.maskA = sum(1 << a for a in self.Asite)
+
+    def entropy(self, psi: torch.Tensor):
+        """(S_vN, S_ref, n_schmidt): -sum lambda ln lambda of rho_A, the reference's
+        -sum sigma ln sigma, and the number of nonzero Schmidt values."""
+        if psi.numel() != self.dimUP * self.dimDN:
+            raise ValueError("psi has the wrong length")
+        out = (C.c_double * 2)()
+        nsv = C.c_int64()
+        call("ed_entanglement_entropy", self.N, self.Qup, self.Qdn, self.maskA, 1 if psi.is_complex() else 0,
+             dptr(psi), out, C.byref(nsv), stream_ptr())
+        return out[0], out[1], nsv.value
+
+    def spectrum(self, psi: torch.Tensor) -> np.ndarray:
+        """all eigenvalues of rho_A, descending"""
+        n = C.c_int64()
+        call("ed_entanglement_spectrum", self.N, self.Qup, self.Qdn, self.maskA, 1 if psi.is_complex() else 0,
+             dptr(psi), None, 0, C.byref(n), stream_ptr())
+        lam = np.zeros(max(1, n.value))
+        call("ed_entanglement_spectrum", self.N, self.Qup, self.Qdn, self.maskA, 1 if psi.is_complex() else 0,
+             dptr(psi), lam.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n), stream_ptr())
+        return lam[: n.value]
