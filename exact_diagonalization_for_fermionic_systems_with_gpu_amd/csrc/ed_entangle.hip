@@ -74,7 +74,8 @@ __global__ __launch_bounds__(256) void k_gather_blocks(int64_t dim_up, int64_t d
 //    upper-triangle tiles only.  256 threads = 4 waves, 64x64 tile per workgroup, each
 //    wave a 32x32 quadrant = 2x2 MFMA 16x16 tiles; K staged through LDS 16 at a time.
 //    MFMA f64 16x16x4 operand layout: A lane l -> (row l%16, k l/16); B lane l ->
-//    (col l%16, k l/16); D lane l -> (rows 4*(l/16)+r, col l%16), r = 0..3.
+//    (col l%16, k l/16); D lane l -> (rows l/16 + 4r, col l%16), r = 0..3 (checked on
+//    gfx950 with a one-wave decode of (i+1)(j+1) products).
 // ---------------------------------------------------------------------------------
 constexpr int TT = 64, TK = 16;
 
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t i = i0 + wi + p * 16 + 4 * (lane >> 4) + r;
+                const int64_t i = i0 + wi + p * 16 + (lane >> 4) + 4 * r;
                 const int64_t j = j0 + wj + q * 16 + (lane & 15);
                 if (i < k && j < k) {
                     if (CPLX) {

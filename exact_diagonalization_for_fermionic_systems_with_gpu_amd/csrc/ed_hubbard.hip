@@ -20,6 +20,7 @@
 //   diag     -- computed from the two state words (popcount, or a per-site U vector).
 // All element offsets are 64-bit (D = 2.36e9 at 18 sites).
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -371,28 +372,250 @@ int choose_cpt(int64_t dim_up, int64_t dim_dn)
     return 1;
 }
 
+// ------------------------------------------------------------------------------------
+// Two-pass H.v (default).  Workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8), so a physical id is remapped to a logical tile
+// t = (b % 8) * per_xcd + b / 8: every XCD walks one contiguous range of tiles and its
+// private 4 MB L2 holds that range's working set.
+//   pass 1 (UP term): tiles = column slabs of W doubles x RB = 256/W rows, slab-major.
+//            An XCD sweeps all rows of its slabs; the slab X[:, slab] (dim_up*W*8 bytes,
+//            3.1 MB at W = 8 for 18 sites) stays in its L2, so the ~19 neighbour-row
+//            reads per element hit L2 instead of HBM.  Writes the UP partial sum.
+//   pass 2 (DN term + diagonal + Lanczos epilogue): tiles = R rows x 256 columns,
+//            row-panel-major, so the rows being gathered stay in the XCD's L2 and each
+//            DN table entry is loaded once for R rows.  Starts from the pass-1 partial,
+//            so each element is summed in the reference order (UP, DN, then U).
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int64_t xcd_tile(int64_t per_xcd)
+{
+    const int64_t b = blockIdx.x;
+    return (b & 7) * per_xcd + (b >> 3);
+}
+
+template <typename V, typename TV, int W, bool SCALE>
+__global__ __launch_bounds__(256) void k_hv_up(HvArgs p, int64_t ntiles, int64_t per_xcd, int64_t nrp, int maxnnz,
+                                               const V *__restrict__ x, V *out, const double *__restrict__ bdev,
+                                               int step)
+{
+    constexpr int RB = 256 / W;
+    extern __shared__ double smem_d[];
+    TV *s_val = reinterpret_cast<TV *>(smem_d);
+    int32_t *s_col = reinterpret_cast<int32_t *>(s_val + RB * maxnnz);
+    int32_t *s_n = s_col + RB * maxnnz;
+    const int64_t t = xcd_tile(per_xcd);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const int64_t slab = t / nrp, rp = t - slab * nrp;
+    const int64_t u0 = rp * RB;
+    const int tid = threadIdx.x;
+    if (tid < RB) {
+        const int64_t u = u0 + tid;
+        s_n[tid] = (u < p.dim_up) ? (int)(p.up_ptr[u + 1] - p.up_ptr[u]) : 0;
+    }
+    for (int e = tid; e < RB * maxnnz; e += 256) {
+        const int r = e / maxnnz, j = e - r * maxnnz;
+        const int64_t u = u0 + r;
+        if (u < p.dim_up) {
+            const int64_t b0 = p.up_ptr[u];
+            if (j < (int)(p.up_ptr[u + 1] - b0)) {
+                s_col[e] = p.up_col[b0 + j];
+                s_val[e] = reinterpret_cast<const TV *>(p.up_val)[b0 + j];
+            }
+        }
+    }
+    __syncthreads();
+    const int r = tid / W, c = tid - r * W;
+    const int64_t u = u0 + r, d = slab * W + c;
+    if (u >= p.dim_up || d >= p.dim_dn) return;
+    double s = 1.0;
+    if (SCALE) s = 1.0 / bdev[step];
+    V acc = VT<V>::zero();
+    const int n = s_n[r];
+    const int32_t *cl = s_col + r * maxnnz;
+    const TV *vl = s_val + r * maxnnz;
+    for (int j = 0; j < n; ++j) {
+        V xv = x[(int64_t)cl[j] * p.dim_dn + d];
+        if (SCALE) xv = vscale(s, xv);
+        acc = mac(vl[j], xv, acc);
+    }
+    out[u * p.dim_dn + d] = acc;
+}
+
+template <typename V, typename TV, int R, int FLAGS>
+__global__ __launch_bounds__(256) void k_hv_dn(HvArgs p, int64_t ntiles, int64_t per_xcd, int64_t ncb, bool has_up,
+                                               const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
+                                               const double *__restrict__ bdev, int step,
+                                               double *__restrict__ partial)
+{
+    __shared__ double s_red[4];
+    const int64_t t = xcd_tile(per_xcd);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const int64_t rp = t / ncb, cb = t - rp * ncb;
+    const int64_t u0 = rp * R, d = cb * 256 + threadIdx.x;
+    const int64_t dim_dn = p.dim_dn;
+    double s = 1.0, c = 0.0;
+    if (FLAGS & HV_SCALE) s = 1.0 / bdev[step];
+    if (FLAGS & HV_AXPY) c = -bdev[step];
+    double dot = 0.0;
+    if (d < dim_dn) {
+        V acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            acc[r] = (has_up && u < p.dim_up) ? ypart[u * dim_dn + d] : VT<V>::zero();
+        }
+        const int64_t sl = d >> 6;
+        const int lane = (int)(d & 63);
+        const int64_t base = p.dn_sptr[sl];
+        const int wdt = (int)((p.dn_sptr[sl + 1] - base) >> 6);
+        const TV *dv = reinterpret_cast<const TV *>(p.dn_val);
+        for (int e = 0; e < wdt; ++e) {
+            const int64_t ei = base + (int64_t)e * 64 + lane;
+            const int64_t col = p.dn_col[ei];
+            const TV a = dv[ei];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t u = u0 + r;
+                if (u < p.dim_up) {
+                    V xv = x[u * dim_dn + col];
+                    if (FLAGS & HV_SCALE) xv = vscale(s, xv);
+                    acc[r] = mac(a, xv, acc[r]);
+                }
+            }
+        }
+        const int32_t sd = p.state_dn[d];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            if (u >= p.dim_up) continue;
+            const int64_t i = u * dim_dn + d;
+            V xd = x[i];
+            if (FLAGS & HV_SCALE) xd = vscale(s, xd);
+            if (p.has_diag) acc[r] = VT<V>::fma_rr(diag_value(p.state_up[u], sd, p.U, p.mask, p.Uvec), xd, acc[r]);
+            V o = acc[r];
+            if (FLAGS & HV_AXPY) o = VT<V>::fma_rr(c, w[i], o);
+            y[i] = o;
+            if (FLAGS & HV_WRITEV) vout[i] = xd;
+            if (FLAGS & HV_DOT) dot += VT<V>::dotre(xd, o);
+        }
+    }
+    if (FLAGS & HV_DOT) {
+        const double rsum = block_sum<256>(dot, s_red);
+        if (threadIdx.x == 0) partial[t] = rsum;
+    }
+}
+
+struct Plan2 {
+    int W, R;
+    int64_t nslab, nrp1, tiles1, per1;
+    int64_t ncb, nrp2, tiles2, per2;
+};
+
+static Plan2 plan2(const Hubbard &h)
+{
+    Plan2 P;
+    const int64_t du = h.args.dim_up, dd = h.args.dim_dn;
+    P.W = (h.hv_w == 16) ? 16 : 8;
+    const int RB = 256 / P.W;
+    P.nslab = (dd + P.W - 1) / P.W;
+    P.nrp1 = (du + RB - 1) / RB;
+    P.tiles1 = P.nslab * P.nrp1;
+    P.per1 = (P.tiles1 + 7) / 8;
+    P.ncb = (dd + 255) / 256;
+    P.R = 1;
+    for (int R : {8, 4, 2}) {
+        if (R <= h.hv_rmax && ((du + R - 1) / R) * P.ncb >= 4096) {
+            P.R = R;
+            break;
+        }
+    }
+    P.nrp2 = (du + P.R - 1) / P.R;
+    P.tiles2 = P.nrp2 * P.ncb;
+    P.per2 = (P.tiles2 + 7) / 8;
+    return P;
+}
+
+template <typename V, typename TV, int FLAGS>
+static int launch_hv2(const Hubbard &h, const V *x, V *y, V *vout, const V *w, const double *bdev, int step,
+                      double *partial, hipStream_t st, int64_t *nblocks_out)
+{
+    const Plan2 P = plan2(h);
+    if (P.per1 * 8 > 0x7fffffffLL || P.per2 * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "H.v grid too large");
+    const int maxnnz = std::max(1, h.up.max_nnz);
+    const bool has_up = h.up.nent > 0;
+    // Lanczos: the UP partial goes into vout (pass 2 reads it back before writing v there),
+    // because y aliases w (v_{N-1}) which pass 2 still needs.
+    V *upbuf = (FLAGS & HV_WRITEV) ? vout : y;
+    constexpr bool SC = (FLAGS & HV_SCALE) != 0;
+    if (has_up) {
+        const int RB = 256 / P.W;
+        const size_t sh = (size_t)RB * maxnnz * (sizeof(TV) + sizeof(int32_t)) + RB * sizeof(int32_t);
+        if (P.W == 16)
+            hipLaunchKernelGGL((k_hv_up<V, TV, 16, SC>), dim3((unsigned)(P.per1 * 8)), dim3(256), sh, st, h.args,
+                               P.tiles1, P.per1, P.nrp1, maxnnz, x, upbuf, bdev, step);
+        else
+            hipLaunchKernelGGL((k_hv_up<V, TV, 8, SC>), dim3((unsigned)(P.per1 * 8)), dim3(256), sh, st, h.args,
+                               P.tiles1, P.per1, P.nrp1, maxnnz, x, upbuf, bdev, step);
+        ED_CHECK_LAUNCH();
+    }
+    dim3 g2((unsigned)(P.per2 * 8)), b2(256);
+    switch (P.R) {
+    case 8:
+        hipLaunchKernelGGL((k_hv_dn<V, TV, 8, FLAGS>), g2, b2, 0, st, h.args, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_hv_dn<V, TV, 4, FLAGS>), g2, b2, 0, st, h.args, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_hv_dn<V, TV, 2, FLAGS>), g2, b2, 0, st, h.args, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    default:
+        hipLaunchKernelGGL((k_hv_dn<V, TV, 1, FLAGS>), g2, b2, 0, st, h.args, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    }
+    ED_CHECK_LAUNCH();
+    if (nblocks_out) *nblocks_out = P.tiles2;
+    return ED_OK;
+}
+
 template <int FLAGS>
 int hv_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout, const void *w, const double *bdev,
                 int step, double *partial, hipStream_t st, int64_t *nblocks)
 {
-    const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
-    if (!cvec) {
-        if (h.cplx_tables) return fail(ED_EINVAL, "complex tables need complex vectors");
-        return launch_hv_cpt<double, double, FLAGS>(h.args, cpt, (const double *)x, (double *)y, (double *)vout,
-                                                    (const double *)w, bdev, step, partial, st, nblocks);
+    if (!cvec && h.cplx_tables) return fail(ED_EINVAL, "complex tables need complex vectors");
+    if (h.algo == 1) {
+        const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
+        if (!cvec)
+            return launch_hv_cpt<double, double, FLAGS>(h.args, cpt, (const double *)x, (double *)y,
+                                                        (double *)vout, (const double *)w, bdev, step, partial, st,
+                                                        nblocks);
+        if (h.cplx_tables)
+            return launch_hv_cpt<cplx, cplx, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout,
+                                                    (const cplx *)w, bdev, step, partial, st, nblocks);
+        return launch_hv_cpt<cplx, double, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout,
+                                                  (const cplx *)w, bdev, step, partial, st, nblocks);
     }
+    if (!cvec)
+        return launch_hv2<double, double, FLAGS>(h, (const double *)x, (double *)y, (double *)vout,
+                                                 (const double *)w, bdev, step, partial, st, nblocks);
     if (h.cplx_tables)
-        return launch_hv_cpt<cplx, cplx, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout,
-                                                (const cplx *)w, bdev, step, partial, st, nblocks);
-    return launch_hv_cpt<cplx, double, FLAGS>(h.args, cpt, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w,
-                                              bdev, step, partial, st, nblocks);
+        return launch_hv2<cplx, cplx, FLAGS>(h, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w, bdev,
+                                             step, partial, st, nblocks);
+    return launch_hv2<cplx, double, FLAGS>(h, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w, bdev, step,
+                                           partial, st, nblocks);
 }
 
 int64_t hv_num_blocks(const Hubbard &h)
 {
-    const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
-    const int64_t cpb = 256 * (int64_t)cpt;
-    return ((h.args.dim_dn + cpb - 1) / cpb) * h.args.dim_up;
+    if (h.algo == 1) {
+        const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
+        const int64_t cpb = 256 * (int64_t)cpt;
+        return ((h.args.dim_dn + cpb - 1) / cpb) * h.args.dim_up;
+    }
+    return plan2(h).tiles2;
 }
 
 int hv_plain(const Hubbard &h, bool cvec, const void *x, void *y, hipStream_t st)
@@ -500,6 +723,14 @@ extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *
     a.Uvec = h->d_uvec;
     a.has_diag = (U != 0.0 || h->d_uvec) ? 1 : 0;
     a.blocks_per_row = 1;
+    // kernel variant: ED_HV_ALGO=1 selects the single-pass row kernel (for A/B timing),
+    // ED_HV_W (8|16) the pass-1 slab width, ED_HV_RMAX the largest pass-2 row panel
+    if (const char *e = getenv("ED_HV_ALGO")) h->algo = atoi(e) == 1 ? 1 : 2;
+    if (const char *e = getenv("ED_HV_W")) h->hv_w = atoi(e);
+    if (const char *e = getenv("ED_HV_RMAX")) h->hv_rmax = atoi(e);
+    // pass 1 keeps 256/W rows of UP lists in LDS: rows with very many slots use the
+    // single-pass kernel instead (its LDS holds one row's list)
+    if ((size_t)32 * h->up.max_nnz * (2 * sizeof(double) + sizeof(int32_t)) + 256 > 65536) h->algo = 1;
     if (h->up.max_nnz > 512) {
         h->release();
         delete h;

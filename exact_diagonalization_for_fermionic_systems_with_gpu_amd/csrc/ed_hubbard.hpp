@@ -49,6 +49,9 @@ struct Hubbard {
     int32_t *d_state_up = nullptr, *d_state_dn = nullptr;
     double *d_uvec = nullptr;
     HvArgs args{};
+    int algo = 2;    // 2: two-pass XCD-aware kernels; 1: single-pass row kernel
+    int hv_w = 8;    // pass-1 slab width (doubles)
+    int hv_rmax = 8; // largest pass-2 row panel
     void release();
 };
 
