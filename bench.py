@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sites", type=int, default=18, choices=[12, 18])
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run one device copy of a vector (16 D bytes) as the byte-count reference for "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the H.v path here is single-GPU (multi-GPU sharding is a design "
@@ -122,6 +125,10 @@ def main():
     gbs = alg_bytes / (mean_ms * 1e-3) / 1e9
     log(f"[bench] H.v mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, wall/step {wall / args.steps * 1e3:.3f} ms")
 
+    if args.copy_ref:
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copy reference: {16 * D} bytes moved by one copy kernel")
     traffic = read_traffic()
     out = {
         "metric": METRIC,
