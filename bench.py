@@ -126,7 +126,7 @@ def main():
     log(f"[bench] H.v mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, wall/step {wall / args.steps * 1e3:.3f} ms")
 
     if args.copy_ref:
-        y.copy_(x)
+        torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         log(f"[bench] copy reference: {16 * D} bytes moved by one copy kernel")
     traffic = read_traffic()

@@ -48,6 +48,7 @@ SIGNATURES = {
     "ed_tables_build": (_i32, [_i32, _i32, _p, _p, _i64, _p, _i32, _i32, _p, _p, _p]),
     "ed_hubbard_create": (_i32, [_i32, _i64, _i32, _p, _p, _p, _i64, _i32, _p, _p, _p, _i32, _d, _i64, _p,
                                  C.POINTER(_p)]),
+    "ed_hubbard_create_ops": (_i32, [_i32, _i32, _p, _i32, _i32, _p, _i32, _i32, _d, _i64, _p, C.POINTER(_p)]),
     "ed_hubbard_destroy": (_i32, [_p]),
     "ed_hubbard_info": (_i32, [_p, _pi64, _pi64, _pi]),
     "ed_hubbard_apply": (_i32, [_p, _i32, _p, _p, _p]),

@@ -252,7 +252,6 @@ __device__ __forceinline__ double diag_value(int32_t su, int32_t sd, double U, i
 // FLAGS: HV_SCALE (x gathered as s*x, s = 1/b_dev[step]), HV_WRITEV (v[i] = s*x[i]),
 //        HV_AXPY (y = Hx + c*w, c = -b_dev[step]), HV_DOT (block partial Re<v,y>).
 // ------------------------------------------------------------------------------------
-enum { HV_SCALE = 1, HV_WRITEV = 2, HV_AXPY = 4, HV_DOT = 8 };
 
 template <typename V, typename TV, int CPT, int FLAGS>
 __global__ __launch_bounds__(256) void k_hv(HvArgs p, const V *__restrict__ x, V *y,
@@ -586,6 +585,7 @@ int hv_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout,
                 int step, double *partial, hipStream_t st, int64_t *nblocks)
 {
     if (!cvec && h.cplx_tables) return fail(ED_EINVAL, "complex tables need complex vectors");
+    if (h.algo == 3) return otf_dispatch<FLAGS>(h, cvec, x, y, vout, w, bdev, step, partial, st, nblocks);
     if (h.algo == 1) {
         const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
         if (!cvec)
@@ -610,6 +610,7 @@ int hv_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout,
 
 int64_t hv_num_blocks(const Hubbard &h)
 {
+    if (h.algo == 3) return otf_num_blocks(h);
     if (h.algo == 1) {
         const int cpt = choose_cpt(h.args.dim_up, h.args.dim_dn);
         const int64_t cpb = 256 * (int64_t)cpt;
@@ -656,8 +657,14 @@ void Hubbard::release()
     dfree(d_state_up);
     dfree(d_state_dn);
     dfree(d_uvec);
+    dfree(d_addr_up);
+    dfree(d_addr_dn);
+    dfree(otf.up.ops_own);
+    dfree(otf.dn.ops_own);
     d_state_up = d_state_dn = nullptr;
     d_uvec = nullptr;
+    d_addr_up = d_addr_dn = nullptr;
+    otf.up.ops_own = otf.dn.ops_own = nullptr;
 }
 
 } // namespace ed
@@ -765,4 +772,78 @@ extern "C" int ed_hubbard_apply(ed_hubbard hh, int complex_vectors, const void *
     if (!h || !d_x || !d_y) return fail(ED_EINVAL, "ed_hubbard_apply: null argument");
     if (d_x == d_y) return fail(ED_EINVAL, "ed_hubbard_apply: x and y must differ");
     return hv_plain(*h, complex_vectors != 0, d_x, d_y, stream_of(stream));
+}
+
+extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int nops_up, int Q_dn, const ed_op *ops_dn,
+                                     int nops_dn, int complex_tables, double U, int64_t mask, const double *Uvec,
+                                     ed_hubbard *out)
+{
+    if (!out || N < 1 || N > 31 || Q_up < 0 || Q_dn < 0 || Q_up > N || Q_dn > N || nops_up < 0 || nops_dn < 0 ||
+        (nops_up && !ops_up) || (nops_dn && !ops_dn))
+        return fail(ED_EINVAL, "ed_hubbard_create_ops: bad arguments");
+    *out = nullptr;
+    int64_t cu, cd;
+    ED_TRY(ed_basis_count(N, Q_up, &cu));
+    ED_TRY(ed_basis_count(N, Q_dn, &cd));
+    const int Su = kZSlots * Q_up + 1, Sd = kZSlots * Q_dn + 1;
+    const size_t full = (size_t)1 << N;
+    const size_t vs = complex_tables ? 2 : 1;
+    int32_t *su = nullptr, *au = nullptr, *sd = nullptr, *ad = nullptr, *ju = nullptr, *jd = nullptr;
+    double *xu = nullptr, *xd = nullptr;
+    ed_hubbard hh = nullptr;
+    int s = ED_OK;
+    auto cleanup = [&] {
+        dfree(su);
+        dfree(sd);
+        dfree(ju);
+        dfree(jd);
+        dfree(xu);
+        dfree(xd);
+        dfree(au);
+        dfree(ad);
+    };
+    do {
+        if ((s = dmalloc((void **)&su, sizeof(int32_t) * cu)) || (s = dmalloc((void **)&sd, sizeof(int32_t) * cd)) ||
+            (s = dmalloc((void **)&au, sizeof(int32_t) * full)) ||
+            (s = dmalloc((void **)&ad, sizeof(int32_t) * full)) ||
+            (s = dmalloc((void **)&xu, sizeof(double) * vs * cu * Su)) ||
+            (s = dmalloc((void **)&xd, sizeof(double) * vs * cd * Sd)) ||
+            (s = dmalloc((void **)&ju, sizeof(int32_t) * cu * Su)) ||
+            (s = dmalloc((void **)&jd, sizeof(int32_t) * cd * Sd)))
+            break;
+        if ((s = ed_basis_build(N, Q_up, su, au, nullptr)) || (s = ed_basis_build(N, Q_dn, sd, ad, nullptr))) break;
+        if ((s = ed_tables_build(N, Q_up, su, au, cu, ops_up, nops_up, complex_tables, xu, ju, nullptr)) ||
+            (s = ed_tables_build(N, Q_dn, sd, ad, cd, ops_dn, nops_dn, complex_tables, xd, jd, nullptr)))
+            break;
+        if ((s = ed_hubbard_create(N, cu, Su, xu, ju, su, cd, Sd, xd, jd, sd, complex_tables, U, mask, Uvec, &hh)))
+            break;
+        Hubbard *h = reinterpret_cast<Hubbard *>(hh);
+        h->d_addr_up = au;
+        h->d_addr_dn = ad;
+        au = ad = nullptr; // owned by h now
+        OtfArgs &o = h->otf;
+        o.dim_up = cu;
+        o.dim_dn = cd;
+        o.state_up = h->d_state_up;
+        o.state_dn = h->d_state_dn;
+        o.U = h->U;
+        o.mask = h->mask;
+        o.Uvec = h->d_uvec;
+        o.has_diag = h->args.has_diag;
+        if ((s = otf_prepare(o.up, ops_up, nops_up, h->d_addr_up, nullptr)) ||
+            (s = otf_prepare(o.dn, ops_dn, nops_dn, h->d_addr_dn, nullptr)))
+            break;
+        int algo = (nops_up <= 64 && nops_dn <= 64) ? 3 : h->algo;
+        if (const char *e = getenv("ED_HV_ALGO")) algo = atoi(e);
+        if (algo == 3 && (nops_up > 64 || nops_dn > 64)) algo = 2;
+        if (algo < 1 || algo > 3) algo = 2;
+        h->algo = algo;
+    } while (0);
+    cleanup();
+    if (s) {
+        if (hh) ed_hubbard_destroy(hh);
+        return s;
+    }
+    *out = hh;
+    return ED_OK;
 }

@@ -5,6 +5,10 @@
 
 namespace ed {
 
+// fused Lanczos options of the H.v kernels: HV_SCALE (x gathered as x/b[step]), HV_WRITEV
+// (v = x/b[step] written), HV_AXPY (y = Hv - b[step] w), HV_DOT (block partials of Re<v,y>)
+enum { HV_SCALE = 1, HV_WRITEV = 2, HV_AXPY = 4, HV_DOT = 8 };
+
 struct SpeciesIn {
     int64_t dim;
     int S;
@@ -40,6 +44,31 @@ struct HvArgs {
     int64_t blocks_per_row;
 };
 
+// operator call of H_TBq, prepared for on-the-fly hop generation (ed_hv_otf.hip)
+struct OtfOp {
+    int32_t kind, i, j, pad;
+    uint32_t m, between;          // 1<<i | 1<<j, sites strictly between i and j
+    double vf_re, vf_im;          // value for a row with site i occupied (j empty)
+    double vb_re, vb_im;          // value for a row with site j occupied (i empty)
+};
+
+struct OtfSpecies {
+    int nops = 0, nhop = 0;
+    const OtfOp *ops = nullptr;
+    OtfOp *ops_own = nullptr;
+    const int32_t *addr = nullptr; // 2^N rank table (H_TBq::address)
+};
+
+struct OtfArgs {
+    int64_t dim_up, dim_dn;
+    OtfSpecies up, dn;
+    const int32_t *state_up, *state_dn;
+    double U;
+    int64_t mask;
+    const double *Uvec;
+    int has_diag;
+};
+
 struct Hubbard {
     int N = 0;
     bool cplx_tables = false;
@@ -49,7 +78,9 @@ struct Hubbard {
     int32_t *d_state_up = nullptr, *d_state_dn = nullptr;
     double *d_uvec = nullptr;
     HvArgs args{};
-    int algo = 2;    // 2: two-pass XCD-aware kernels; 1: single-pass row kernel
+    int algo = 2;    // 3: on-the-fly two-pass; 2: two-pass table kernels; 1: single-pass row kernel
+    OtfArgs otf{};
+    int32_t *d_addr_up = nullptr, *d_addr_dn = nullptr;
     int hv_w = 8;    // pass-1 slab width (doubles)
     int hv_rmax = 8; // largest pass-2 row panel
     void release();
@@ -61,6 +92,11 @@ int hv_lanczos(const Hubbard &h, bool cvec, const void *f, void *y, void *v, con
 int hv_lanczos_nodot(const Hubbard &h, bool cvec, const void *f, void *y, void *v, const void *w,
                      const double *bdev, int step, hipStream_t st);
 int64_t hv_num_blocks(const Hubbard &h);
+template <int FLAGS>
+int otf_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout, const void *w, const double *bdev,
+                 int step, double *partial, hipStream_t st, int64_t *nblocks);
+int64_t otf_num_blocks(const Hubbard &h);
+int otf_prepare(OtfSpecies &sp, const ed_op *ops, int nops, const int32_t *d_address, hipStream_t st);
 
 // deterministic reductions and BLAS-1 (ed_blas.hip)
 int finalize_sum(const double *partial, int64_t n, double *dst, int mode, hipStream_t st);

@@ -1,0 +1,403 @@
+// ed_hv_otf.hip -- H.v with hop descriptors generated on the fly (default kernel).
+//
+// Same operator as Hubbard_mv_ (MBL_test/include/HubMatrix_GPU.cuh:114-161), same
+// per-element summation order (UP entries in H_TBq call order, DN entries in call order,
+// then U), but the hop lists are not read from ELL tables: for a species state s and an
+// operator (i, j) of the H_TBq call list, the neighbour is address[s ^ (1<<i | 1<<j)] and the
+// matrix element sign * value with sign = (-1)^popcount(s & between(i,j))
+// (fermi_operator.h:100-149).  The reference's ELL slots are filled in call order and each
+// nonzero call takes its own slot, so replaying the call list in order visits the nonzero
+// slots in slot order.
+//
+// Why: with tables, the descriptors (12 B per hop, ~19 hops per row/column) are re-read for
+// every column slab (UP) and row panel (DN) and cost several times the vector traffic
+// (rocprofv3: 145 + 164 GB fetched per 18-site H.v against 38 GB of vector data).
+// Generating them costs a few integer ops and one address-table lookup (the 2^N-entry
+// rank table, L2-resident) per hop, amortised over W columns (UP) or R rows (DN).
+//
+// Pass 1 (UP term) -- tiles of 32 rows x 8 columns, XCD-aware slab-major order.  Each
+// wavefront owns 8 rows; for each row the 64 lanes evaluate up to 64 operator calls in
+// parallel, a ballot orders the hopping ones, and the (column, value) list lands in LDS.
+// Then lane (row, col) streams x[neighbour row, col] over the list.
+// Pass 2 (DN term + diagonal + Lanczos epilogue) -- tiles of R rows x 256 columns,
+// XCD-aware row-panel order; each lane walks the call list for its column once and
+// applies every hop to its R rows, starting from the pass-1 partial.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ed_common.hpp"
+#include "ed_hubbard.hpp"
+
+namespace ed {
+
+template <typename V>
+struct VO;
+template <>
+struct VO<double> {
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double scale(double s, double x) { return s * x; }
+    static __device__ __forceinline__ double fmr(double a, double x, double acc) { return fma(a, x, acc); }
+    static __device__ __forceinline__ double dotre(double a, double b) { return a * b; }
+};
+template <>
+struct VO<cplx> {
+    static __device__ __forceinline__ cplx zero() { return cplx{0.0, 0.0}; }
+    static __device__ __forceinline__ cplx scale(double s, cplx x) { return cplx{s * x.re, s * x.im}; }
+    static __device__ __forceinline__ cplx fmr(double a, cplx x, cplx acc)
+    {
+        return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)};
+    }
+    static __device__ __forceinline__ double dotre(cplx a, cplx b) { return a.re * b.re + a.im * b.im; }
+};
+
+__device__ __forceinline__ double omac(double a, double x, double acc) { return fma(a, x, acc); }
+__device__ __forceinline__ cplx omac(double a, cplx x, cplx acc) { return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)}; }
+__device__ __forceinline__ cplx omac(cplx a, cplx x, cplx acc)
+{
+    return cplx{fma(a.re, x.re, fma(-a.im, x.im, acc.re)), fma(a.re, x.im, fma(a.im, x.re, acc.im))};
+}
+
+template <typename TV>
+__device__ __forceinline__ TV tv_make(double re, double im);
+template <>
+__device__ __forceinline__ double tv_make<double>(double re, double) { return re; }
+template <>
+__device__ __forceinline__ cplx tv_make<cplx>(double re, double im) { return cplx{re, im}; }
+
+template <typename TV>
+__device__ __forceinline__ TV tv_neg(TV a);
+template <>
+__device__ __forceinline__ double tv_neg<double>(double a) { return -a; }
+template <>
+__device__ __forceinline__ cplx tv_neg<cplx>(cplx a) { return cplx{-a.re, -a.im}; }
+
+// Contribution of call o to the row of state s (row index `self`): returns true and
+// (col, value) when the call has a matrix element in this row.  Values are the table
+// values of fermi_operator.h: sign * t for Make_H, sign * (+-t) for Make_J, t for Make_N /
+// Insert_2Nop, sign * t e^{+-i phi} for the phase hop.
+template <typename TV>
+__device__ __forceinline__ bool op_entry(const OtfOp &op, uint32_t s, int32_t self, const int32_t *__restrict__ addr,
+                                         int32_t &col, TV &val)
+{
+    const uint32_t oi = (s >> op.i) & 1u, oj = (s >> op.j) & 1u;
+    switch (op.kind) {
+    case ED_OP_DENSITY:
+        if (!oi) return false;
+        col = self;
+        val = tv_make<TV>(op.vf_re, op.vf_im);
+        return true;
+    case ED_OP_DENSITY2:
+        if (!(oi & oj)) return false;
+        col = self;
+        val = tv_make<TV>(op.vf_re, op.vf_im);
+        return true;
+    default: {
+        if (oi == oj) return false;
+        col = addr[s ^ op.m];
+        const TV v = oi ? tv_make<TV>(op.vf_re, op.vf_im) : tv_make<TV>(op.vb_re, op.vb_im);
+        val = (__popc(s & op.between) & 1) ? tv_neg<TV>(v) : v;
+        return true;
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// pass 1: UP term
+// ---------------------------------------------------------------------------------
+constexpr int OW = 8;        // slab width (doubles, 64 B)
+constexpr int ORB = 256 / OW; // rows per tile
+
+template <typename V, typename TV, bool SCALE>
+__global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t nrp,
+                                                const V *__restrict__ x, V *out, const double *__restrict__ bdev,
+                                                int step)
+{
+    extern __shared__ double smem_o[];
+    const int nops = p.up.nops;
+    TV *s_val = reinterpret_cast<TV *>(smem_o);                       // [ORB][nops]
+    int32_t *s_col = reinterpret_cast<int32_t *>(s_val + ORB * nops); // [ORB][nops]
+    __shared__ int s_cnt[ORB];
+    const int64_t b = blockIdx.x;
+    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const int64_t slab = t / nrp, rp = t - slab * nrp;
+    const int64_t u0 = rp * ORB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // descriptor phase: wave `wave` owns rows 8*wave .. 8*wave+7
+    for (int rr = 0; rr < 8; ++rr) {
+        const int r = wave * 8 + rr;
+        const int64_t u = u0 + r;
+        bool has = false;
+        int32_t col = 0;
+        TV val{};
+        if (u < p.dim_up && lane < nops) {
+            const OtfOp op = p.up.ops[lane];
+            has = op_entry<TV>(op, (uint32_t)p.state_up[u], (int32_t)u, p.up.addr, col, val);
+        }
+        const uint64_t bal = __ballot(has);
+        if (has) {
+            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+            s_col[r * nops + pos] = col;
+            s_val[r * nops + pos] = val;
+        }
+        if (lane == 0) s_cnt[r] = __popcll(bal);
+    }
+    __syncthreads();
+    const int r = wave * 8 + (lane >> 3), c = lane & 7;
+    const int64_t u = u0 + r, d = slab * OW + c;
+    if (u >= p.dim_up || d >= p.dim_dn) return;
+    double sc = 1.0;
+    if (SCALE) sc = 1.0 / bdev[step];
+    V acc = VO<V>::zero();
+    const int n = s_cnt[r];
+    const int32_t *cl = s_col + r * nops;
+    const TV *vl = s_val + r * nops;
+    for (int j = 0; j < n; ++j) {
+        V xv = x[(int64_t)cl[j] * p.dim_dn + d];
+        if (SCALE) xv = VO<V>::scale(sc, xv);
+        acc = omac(vl[j], xv, acc);
+    }
+    out[u * p.dim_dn + d] = acc;
+}
+
+// ---------------------------------------------------------------------------------
+// pass 2: DN term + diagonal + epilogue
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double diag_of(int32_t su, int32_t sd, const OtfArgs &p)
+{
+    const uint32_t dbl = (uint32_t)(su & sd) & (uint32_t)p.mask;
+    if (!p.Uvec) return p.U * (double)__popc(dbl);
+    double s = 0.0;
+    uint32_t m = dbl;
+    while (m) {
+        const int i = __ffs(m) - 1;
+        s += p.Uvec[i];
+        m &= m - 1;
+    }
+    return s;
+}
+
+template <typename V, typename TV, int R, int FLAGS>
+__global__ __launch_bounds__(256) void k_otf_dn(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t ncb, bool has_up,
+                                                const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
+                                                const double *__restrict__ bdev, int step,
+                                                double *__restrict__ partial)
+{
+    __shared__ double s_red[4];
+    const int64_t b = blockIdx.x;
+    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const int64_t rp = t / ncb, cb = t - rp * ncb;
+    const int64_t u0 = rp * R, d = cb * 256 + threadIdx.x;
+    const int64_t dim_dn = p.dim_dn;
+    double sc = 1.0, c = 0.0;
+    if (FLAGS & HV_SCALE) sc = 1.0 / bdev[step];
+    if (FLAGS & HV_AXPY) c = -bdev[step];
+    double dot = 0.0;
+    if (d < dim_dn) {
+        V acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            acc[r] = (has_up && u < p.dim_up) ? ypart[u * dim_dn + d] : VO<V>::zero();
+        }
+        const uint32_t sd = (uint32_t)p.state_dn[d];
+        const int nops = p.dn.nops;
+        for (int o = 0; o < nops; ++o) {
+            const OtfOp op = p.dn.ops[o];
+            int32_t col;
+            TV a;
+            if (!op_entry<TV>(op, sd, (int32_t)d, p.dn.addr, col, a)) continue;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t u = u0 + r;
+                if (u < p.dim_up) {
+                    V xv = x[u * dim_dn + col];
+                    if (FLAGS & HV_SCALE) xv = VO<V>::scale(sc, xv);
+                    acc[r] = omac(a, xv, acc[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            if (u >= p.dim_up) continue;
+            const int64_t i = u * dim_dn + d;
+            V xd = x[i];
+            if (FLAGS & HV_SCALE) xd = VO<V>::scale(sc, xd);
+            if (p.has_diag) acc[r] = VO<V>::fmr(diag_of(p.state_up[u], (int32_t)sd, p), xd, acc[r]);
+            V o = acc[r];
+            if (FLAGS & HV_AXPY) o = VO<V>::fmr(c, w[i], o);
+            y[i] = o;
+            if (FLAGS & HV_WRITEV) vout[i] = xd;
+            if (FLAGS & HV_DOT) dot += VO<V>::dotre(xd, o);
+        }
+    }
+    if (FLAGS & HV_DOT) {
+        const double rsum = block_sum<256>(dot, s_red);
+        if (threadIdx.x == 0) partial[t] = rsum;
+    }
+}
+
+struct Plan3 {
+    int R;
+    int64_t nslab, nrp1, tiles1, per1, ncb, nrp2, tiles2, per2;
+};
+
+static Plan3 plan3(const Hubbard &h)
+{
+    Plan3 P;
+    const int64_t du = h.args.dim_up, dd = h.args.dim_dn;
+    P.nslab = (dd + OW - 1) / OW;
+    P.nrp1 = (du + ORB - 1) / ORB;
+    P.tiles1 = P.nslab * P.nrp1;
+    P.per1 = (P.tiles1 + 7) / 8;
+    P.ncb = (dd + 255) / 256;
+    P.R = 1;
+    for (int R : {8, 4, 2}) {
+        if (R <= h.hv_rmax && ((du + R - 1) / R) * P.ncb >= 4096) {
+            P.R = R;
+            break;
+        }
+    }
+    P.nrp2 = (du + P.R - 1) / P.R;
+    P.tiles2 = P.nrp2 * P.ncb;
+    P.per2 = (P.tiles2 + 7) / 8;
+    return P;
+}
+
+int64_t otf_num_blocks(const Hubbard &h) { return plan3(h).tiles2; }
+
+template <typename V, typename TV, int FLAGS>
+static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, const double *bdev, int step,
+                      double *partial, hipStream_t st, int64_t *nblocks_out)
+{
+    const Plan3 P = plan3(h);
+    if (P.per1 * 8 > 0x7fffffffLL || P.per2 * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "H.v grid too large");
+    const bool has_up = h.otf.up.nhop > 0;
+    V *upbuf = (FLAGS & HV_WRITEV) ? vout : y;
+    constexpr bool SC = (FLAGS & HV_SCALE) != 0;
+    if (has_up) {
+        const size_t sh = (size_t)ORB * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
+        hipLaunchKernelGGL((k_otf_up<V, TV, SC>), dim3((unsigned)(P.per1 * 8)), dim3(256), sh, st, h.otf, P.tiles1,
+                           P.per1, P.nrp1, x, upbuf, bdev, step);
+        ED_CHECK_LAUNCH();
+    }
+    dim3 g2((unsigned)(P.per2 * 8)), b2(256);
+    switch (P.R) {
+    case 8:
+        hipLaunchKernelGGL((k_otf_dn<V, TV, 8, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_otf_dn<V, TV, 4, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_otf_dn<V, TV, 2, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    default:
+        hipLaunchKernelGGL((k_otf_dn<V, TV, 1, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
+                           upbuf, y, vout, w, bdev, step, partial);
+        break;
+    }
+    ED_CHECK_LAUNCH();
+    if (nblocks_out) *nblocks_out = P.tiles2;
+    return ED_OK;
+}
+
+template <int FLAGS>
+int otf_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout, const void *w, const double *bdev,
+                 int step, double *partial, hipStream_t st, int64_t *nblocks)
+{
+    if (!cvec)
+        return launch_otf<double, double, FLAGS>(h, (const double *)x, (double *)y, (double *)vout,
+                                                 (const double *)w, bdev, step, partial, st, nblocks);
+    if (h.cplx_tables)
+        return launch_otf<cplx, cplx, FLAGS>(h, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w, bdev,
+                                             step, partial, st, nblocks);
+    return launch_otf<cplx, double, FLAGS>(h, (const cplx *)x, (cplx *)y, (cplx *)vout, (const cplx *)w, bdev, step,
+                                           partial, st, nblocks);
+}
+
+template int otf_dispatch<0>(const Hubbard &, bool, const void *, void *, void *, const void *, const double *, int,
+                             double *, hipStream_t, int64_t *);
+template int otf_dispatch<HV_SCALE | HV_WRITEV | HV_DOT>(const Hubbard &, bool, const void *, void *, void *,
+                                                         const void *, const double *, int, double *, hipStream_t,
+                                                         int64_t *);
+template int otf_dispatch<HV_SCALE | HV_WRITEV | HV_AXPY | HV_DOT>(const Hubbard &, bool, const void *, void *, void *,
+                                                                   const void *, const double *, int, double *,
+                                                                   hipStream_t, int64_t *);
+template int otf_dispatch<HV_SCALE | HV_WRITEV>(const Hubbard &, bool, const void *, void *, void *, const void *,
+                                                const double *, int, double *, hipStream_t, int64_t *);
+template int otf_dispatch<HV_SCALE | HV_WRITEV | HV_AXPY>(const Hubbard &, bool, const void *, void *, void *,
+                                                          const void *, const double *, int, double *, hipStream_t,
+                                                          int64_t *);
+
+// ---------------------------------------------------------------------------------
+// operator-list preparation (values computed on the device with the same sincos as the
+// table builder, so both paths use bit-identical matrix elements)
+// ---------------------------------------------------------------------------------
+__global__ void k_prep_ops(int n, const ed_op *__restrict__ in, OtfOp *__restrict__ out)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const ed_op a = in[o];
+    OtfOp r;
+    r.kind = a.kind;
+    r.i = a.i;
+    r.j = a.j;
+    r.m = (1u << a.i) | (1u << a.j);
+    const int lo = a.i < a.j ? a.i : a.j, hi = a.i < a.j ? a.j : a.i;
+    r.between = (hi - lo > 1) ? ((((1u << (hi - lo - 1)) - 1u)) << (lo + 1)) : 0u;
+    r.vf_im = r.vb_im = 0.0;
+    switch (a.kind) {
+    case ED_OP_HOP: r.vf_re = r.vb_re = a.value; break;
+    case ED_OP_CURRENT: // row with i empty, j occupied gets +t (c+_i c_j), the other -t
+        r.vb_re = a.value;
+        r.vf_re = -a.value;
+        break;
+    case ED_OP_HOP_PHASE: {
+        double s, c;
+        sincos(a.phase, &s, &c);
+        r.vf_re = a.value * c;
+        r.vf_im = a.value * s;
+        sincos(-a.phase, &s, &c);
+        r.vb_re = a.value * c;
+        r.vb_im = a.value * s;
+        break;
+    }
+    default: // DENSITY / DENSITY2
+        r.vf_re = r.vb_re = a.value;
+        break;
+    }
+    out[o] = r;
+}
+
+int otf_prepare(OtfSpecies &sp, const ed_op *ops, int nops, const int32_t *d_address, hipStream_t st)
+{
+    sp.nops = nops;
+    sp.addr = d_address;
+    int nhop = 0;
+    for (int o = 0; o < nops; ++o) nhop += 1;
+    sp.nhop = nhop;
+    ED_TRY(dmalloc((void **)&sp.ops_own, sizeof(OtfOp) * std::max(1, nops)));
+    sp.ops = sp.ops_own;
+    if (nops == 0) return ED_OK;
+    ed_op *din = nullptr;
+    ED_TRY(dmalloc((void **)&din, sizeof(ed_op) * nops));
+    int s = ED_OK;
+    if (hipMemcpyAsync(din, ops, sizeof(ed_op) * nops, hipMemcpyHostToDevice, st) != hipSuccess)
+        s = fail(ED_EHIP, "otf_prepare: upload failed");
+    if (!s) {
+        hipLaunchKernelGGL(k_prep_ops, dim3((nops + 63) / 64), dim3(64), 0, st, nops, din, sp.ops_own);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            s = fail(ED_EHIP, "otf_prepare: kernel failed");
+    }
+    dfree(din);
+    return s;
+}
+
+} // namespace ed

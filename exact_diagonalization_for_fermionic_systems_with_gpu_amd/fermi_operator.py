@@ -91,6 +91,13 @@ class H_TBq:
         self.ops = []
         self._built = -1
 
+    def ops_array(self):
+        """the recorded calls as a C array of ed_op (for ed_tables_build / ed_hubbard_create_ops)"""
+        arr = (_lib.EdOp * max(1, len(self.ops)))()
+        for k, (kind, i, j, v, ph) in enumerate(self.ops):
+            arr[k] = _lib.EdOp(kind, i, j, 0, v, ph)
+        return arr, len(self.ops)
+
     # --- tables ---------------------------------------------------------------------------
     def _build(self):
         if self._built == len(self.ops) and self._Ax is not None:
@@ -99,9 +106,7 @@ class H_TBq:
         if self._Ax is None:
             self._Ax = torch.empty(n * (2 if self.complex_tables else 1), dtype=torch.float64, device=self.device)
             self._Aj = torch.empty(n, dtype=torch.int32, device=self.device)
-        arr = (_lib.EdOp * max(1, len(self.ops)))()
-        for k, (kind, i, j, v, ph) in enumerate(self.ops):
-            arr[k] = _lib.EdOp(kind, i, j, 0, v, ph)
+        arr, _ = self.ops_array()
         call("ed_tables_build", self.N, -1 if self.full else self.Q, dptr(self.state), dptr(self.address),
              self.count, arr, len(self.ops), int(self.complex_tables), dptr(self._Ax), dptr(self._Aj),
              stream_ptr())

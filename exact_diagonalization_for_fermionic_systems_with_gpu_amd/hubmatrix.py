@@ -32,7 +32,11 @@ class Hubbard:
 
     Uvec (extension): per-site interaction U_i replacing U*popcount(up&dn&dilution)."""
 
-    def __init__(self, HqUP: H_TBq, HqDN: H_TBq, U: float = 0.0, dilution: int | None = None, Uvec=None):
+    def __init__(self, HqUP: H_TBq, HqDN: H_TBq, U: float = 0.0, dilution: int | None = None, Uvec=None,
+                 from_tables: bool = False):
+        """from_tables=True builds from the device ELL tables (the reference's uploaded
+        Ax/Aj, HubMatrix_GPU.cuh:63-69); otherwise from the two H_TBq call lists, which
+        lets the library generate hop descriptors on the fly."""
         if HqUP.N != HqDN.N:
             raise ValueError("both species must live on the same N sites")
         self.N = HqUP.N
@@ -44,18 +48,25 @@ class Hubbard:
         self.complex_tables = bool(HqUP.complex_tables or HqDN.complex_tables)
         if HqUP.complex_tables != HqDN.complex_tables:
             raise ValueError("both species must use the same table type")
-        AxU, AjU = HqUP.raw_tables()
-        AxD, AjD = HqDN.raw_tables()
         uv = None
         if Uvec is not None:
             uv = np.ascontiguousarray(Uvec, dtype=np.float64)
             if uv.shape != (self.N,):
                 raise ValueError("Uvec needs N entries")
         self._uv = uv
+        uvp = None if uv is None else uv.ctypes.data_as(C.c_void_p)
         h = C.c_void_p()
-        call("ed_hubbard_create", self.N, self.dimUP, HqUP.slots, dptr(AxU), dptr(AjU), dptr(HqUP.state),
-             self.dimDN, HqDN.slots, dptr(AxD), dptr(AjD), dptr(HqDN.state), int(self.complex_tables), self.U,
-             self.dilution, None if uv is None else uv.ctypes.data_as(C.c_void_p), C.byref(h))
+        if from_tables or HqUP.full or HqDN.full:
+            AxU, AjU = HqUP.raw_tables()
+            AxD, AjD = HqDN.raw_tables()
+            call("ed_hubbard_create", self.N, self.dimUP, HqUP.slots, dptr(AxU), dptr(AjU), dptr(HqUP.state),
+                 self.dimDN, HqDN.slots, dptr(AxD), dptr(AjD), dptr(HqDN.state), int(self.complex_tables), self.U,
+                 self.dilution, uvp, C.byref(h))
+        else:
+            au, nu = HqUP.ops_array()
+            ad, nd = HqDN.ops_array()
+            call("ed_hubbard_create_ops", self.N, HqUP.Q, au, nu, HqDN.Q, ad, nd, int(self.complex_tables), self.U,
+                 self.dilution, uvp, C.byref(h))
         self._h = h
         self.dim = self.dimUP * self.dimDN
 

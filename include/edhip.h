@@ -90,6 +90,14 @@ int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *d_AxUP, con
                       const int32_t *d_stateUP, int64_t dim_dn, int S_dn, const double *d_AxDN,
                       const int32_t *d_AjDN, const int32_t *d_stateDN, int complex_tables, double U, int64_t mask,
                       const double *Uvec, ed_hubbard *out);
+/* Hubbard(HqUP, HqDN, U [, dilution]) built straight from the two H_TBq call lists
+ * (the Make_H / Make_J / Make_N / Insert_2Nop calls of fermi_operator.h:100-225, in call
+ * order, as ed_tables_build takes them).  Builds the bases and tables itself and, when each
+ * list has at most 64 calls, applies H with hop descriptors generated on the fly from the
+ * call list (same matrix elements, same per-element summation order as the tables). */
+int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int nops_up, int Q_dn, const ed_op *ops_dn,
+                          int nops_dn, int complex_tables, double U, int64_t mask, const double *Uvec,
+                          ed_hubbard *out);
 int ed_hubbard_destroy(ed_hubbard h);
 int ed_hubbard_info(ed_hubbard h, int64_t *dim_up, int64_t *dim_dn, int *complex_tables);
 

@@ -113,12 +113,17 @@ def _hv_cases():
     }
 
 
+@pytest.mark.parametrize("path", ["otf", "tables", "rowkernel"])
 @pytest.mark.parametrize("name", list(_hv_cases().keys()))
-def test_hv_parity(name):
+def test_hv_parity(name, path, monkeypatch):
+    """otf: descriptors generated from the call list (default); tables: two-pass kernels on the
+    ELL tables; rowkernel: the single-pass row kernel (ED_HV_ALGO=1)."""
     N, Qu, Qd, ops, U, mask, Uvec, cplx = _hv_cases()[name]
-    print(f"hv {name}", flush=True)
+    print(f"hv {name} {path}", flush=True)
     gu, gd = gpu_species(N, Qu, ops, cplx), gpu_species(N, Qd, ops, cplx)
-    H = Hubbard(gu, gd, U, dilution=mask, Uvec=Uvec)
+    if path == "rowkernel":
+        monkeypatch.setenv("ED_HV_ALGO", "1")
+    H = Hubbard(gu, gd, U, dilution=mask, Uvec=Uvec, from_tables=(path != "otf"))
     ou, od = O.Species(N, Qu, ops, cplx), O.Species(N, Qd, ops, cplx)
     OH = O.Hubbard(ou, od, U, mask=mask, Uvec=Uvec)
     rng = np.random.default_rng(1)
