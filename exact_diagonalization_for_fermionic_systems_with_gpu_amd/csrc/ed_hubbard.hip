@@ -838,6 +838,10 @@ extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int n
         if (algo == 3 && (nops_up > 64 || nops_dn > 64)) algo = 2;
         if (algo < 1 || algo > 3) algo = 2;
         h->algo = algo;
+        // on-the-fly pass 1 reads 64-column slabs (one 512 B row segment per wavefront load);
+        // narrower slabs re-walk every row of x once per slab (rocprofv3, 18-site H.v:
+        // W=8 82 ms, W=16 ~55 ms, W=64 37 ms for the UP pass)
+        if (algo == 3 && !getenv("ED_HV_W")) h->hv_w = 64;
     } while (0);
     cleanup();
     if (s) {

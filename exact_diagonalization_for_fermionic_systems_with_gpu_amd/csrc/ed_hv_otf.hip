@@ -105,14 +105,14 @@ __device__ __forceinline__ bool op_entry(const OtfOp &op, uint32_t s, int32_t se
 // ---------------------------------------------------------------------------------
 // pass 1: UP term
 // ---------------------------------------------------------------------------------
-constexpr int OW = 8;        // slab width (doubles, 64 B)
-constexpr int ORB = 256 / OW; // rows per tile
-
-template <typename V, typename TV, bool SCALE>
+// slab width OW (doubles) is a template parameter: ORB = 256/OW rows per tile, 64/OW rows
+// per wavefront
+template <typename V, typename TV, bool SCALE, int OW>
 __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t nrp,
                                                 const V *__restrict__ x, V *out, const double *__restrict__ bdev,
                                                 int step)
 {
+    constexpr int ORB = 256 / OW, RW = 64 / OW;
     extern __shared__ double smem_o[];
     const int nops = p.up.nops;
     TV *s_val = reinterpret_cast<TV *>(smem_o);                       // [ORB][nops]
@@ -124,9 +124,9 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
     const int64_t slab = t / nrp, rp = t - slab * nrp;
     const int64_t u0 = rp * ORB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // descriptor phase: wave `wave` owns rows 8*wave .. 8*wave+7
-    for (int rr = 0; rr < 8; ++rr) {
-        const int r = wave * 8 + rr;
+    // descriptor phase: wave `wave` owns rows RW*wave .. RW*wave+RW-1
+    for (int rr = 0; rr < RW; ++rr) {
+        const int r = wave * RW + rr;
         const int64_t u = u0 + r;
         bool has = false;
         int32_t col = 0;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
         if (lane == 0) s_cnt[r] = __popcll(bal);
     }
     __syncthreads();
-    const int r = wave * 8 + (lane >> 3), c = lane & 7;
+    const int r = wave * RW + lane / OW, c = lane % OW;
     const int64_t u = u0 + r, d = slab * OW + c;
     if (u >= p.dim_up || d >= p.dim_dn) return;
     double sc = 1.0;
@@ -153,10 +153,19 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
     const int n = s_cnt[r];
     const int32_t *cl = s_col + r * nops;
     const TV *vl = s_val + r * nops;
-    for (int j = 0; j < n; ++j) {
-        V xv = x[(int64_t)cl[j] * p.dim_dn + d];
-        if (SCALE) xv = VO<V>::scale(sc, xv);
-        acc = omac(vl[j], xv, acc);
+    // loads in batches of 8 so several are in flight per lane; the FMAs stay in call order
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        V xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (j0 + k < n) xv[k] = x[(int64_t)cl[j0 + k] * p.dim_dn + d];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (j0 + k < n) {
+                V xk = xv[k];
+                if (SCALE) xk = VO<V>::scale(sc, xk);
+                acc = omac(vl[j0 + k], xk, acc);
+            }
     }
     out[u * p.dim_dn + d] = acc;
 }
@@ -204,20 +213,42 @@ __global__ __launch_bounds__(256) void k_otf_dn(OtfArgs p, int64_t ntiles, int64
         }
         const uint32_t sd = (uint32_t)p.state_dn[d];
         const int nops = p.dn.nops;
-        for (int o = 0; o < nops; ++o) {
-            const OtfOp op = p.dn.ops[o];
-            int32_t col;
-            TV a;
-            if (!op_entry<TV>(op, sd, (int32_t)d, p.dn.addr, col, a)) continue;
+        // calls in batches of 4: the 4 rank lookups, then 4*R gathers, are issued before the
+        // FMAs, which stay in call order; a call without an element in this column points at
+        // the own column with value 0 (adds an exact 0, as the reference's empty DN slots do)
+        constexpr int KB = 4;
+        for (int o0 = 0; o0 < nops; o0 += KB) {
+            int32_t col[KB];
+            TV a[KB];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int64_t u = u0 + r;
-                if (u < p.dim_up) {
-                    V xv = x[u * dim_dn + col];
-                    if (FLAGS & HV_SCALE) xv = VO<V>::scale(sc, xv);
-                    acc[r] = omac(a, xv, acc[r]);
+            for (int k = 0; k < KB; ++k) {
+                col[k] = (int32_t)d;
+                a[k] = tv_make<TV>(0.0, 0.0);
+                if (o0 + k < nops) {
+                    int32_t ck;
+                    TV ak;
+                    if (op_entry<TV>(p.dn.ops[o0 + k], sd, (int32_t)d, p.dn.addr, ck, ak)) {
+                        col[k] = ck;
+                        a[k] = ak;
+                    }
                 }
             }
+            V xv[KB][R];
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int64_t u = (u0 + r < p.dim_up) ? u0 + r : u0;
+                    xv[k][r] = x[u * dim_dn + col[k]];
+                }
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    V xk = xv[k][r];
+                    if (FLAGS & HV_SCALE) xk = VO<V>::scale(sc, xk);
+                    acc[r] = omac(a[k], xk, acc[r]);
+                }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -241,7 +272,7 @@ __global__ __launch_bounds__(256) void k_otf_dn(OtfArgs p, int64_t ntiles, int64
 }
 
 struct Plan3 {
-    int R;
+    int R, W;
     int64_t nslab, nrp1, tiles1, per1, ncb, nrp2, tiles2, per2;
 };
 
@@ -249,8 +280,9 @@ static Plan3 plan3(const Hubbard &h)
 {
     Plan3 P;
     const int64_t du = h.args.dim_up, dd = h.args.dim_dn;
-    P.nslab = (dd + OW - 1) / OW;
-    P.nrp1 = (du + ORB - 1) / ORB;
+    P.W = (h.hv_w == 16 || h.hv_w == 32 || h.hv_w == 64) ? h.hv_w : 8;
+    P.nslab = (dd + P.W - 1) / P.W;
+    P.nrp1 = (du + 256 / P.W - 1) / (256 / P.W);
     P.tiles1 = P.nslab * P.nrp1;
     P.per1 = (P.tiles1 + 7) / 8;
     P.ncb = (dd + 255) / 256;
@@ -279,9 +311,26 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
     V *upbuf = (FLAGS & HV_WRITEV) ? vout : y;
     constexpr bool SC = (FLAGS & HV_SCALE) != 0;
     if (has_up) {
-        const size_t sh = (size_t)ORB * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
-        hipLaunchKernelGGL((k_otf_up<V, TV, SC>), dim3((unsigned)(P.per1 * 8)), dim3(256), sh, st, h.otf, P.tiles1,
-                           P.per1, P.nrp1, x, upbuf, bdev, step);
+        const size_t sh = (size_t)(256 / P.W) * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
+        const dim3 g1((unsigned)(P.per1 * 8));
+        switch (P.W) {
+        case 64:
+            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 64>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
+                               upbuf, bdev, step);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 32>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
+                               upbuf, bdev, step);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 16>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
+                               upbuf, bdev, step);
+            break;
+        default:
+            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 8>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
+                               upbuf, bdev, step);
+            break;
+        }
         ED_CHECK_LAUNCH();
     }
     dim3 g2((unsigned)(P.per2 * 8)), b2(256);
