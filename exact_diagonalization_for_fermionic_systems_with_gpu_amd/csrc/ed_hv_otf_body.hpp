@@ -1,0 +1,230 @@
+// ed_hv_otf_body.hpp -- device bodies of the on-the-fly H.v passes, shared by the
+// single-operator kernels (ed_hv_otf.hip) and the batched multi-instance kernels
+// (ed_batch.hip).  See ed_hv_otf.hip for the algorithm.
+#pragma once
+
+#include "ed_common.hpp"
+#include "ed_hubbard.hpp"
+
+namespace ed {
+
+template <typename V>
+struct VO;
+template <>
+struct VO<double> {
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double scale(double s, double x) { return s * x; }
+    static __device__ __forceinline__ double fmr(double a, double x, double acc) { return fma(a, x, acc); }
+    static __device__ __forceinline__ double dotre(double a, double b) { return a * b; }
+};
+template <>
+struct VO<cplx> {
+    static __device__ __forceinline__ cplx zero() { return cplx{0.0, 0.0}; }
+    static __device__ __forceinline__ cplx scale(double s, cplx x) { return cplx{s * x.re, s * x.im}; }
+    static __device__ __forceinline__ cplx fmr(double a, cplx x, cplx acc)
+    {
+        return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)};
+    }
+    static __device__ __forceinline__ double dotre(cplx a, cplx b) { return a.re * b.re + a.im * b.im; }
+};
+
+__device__ __forceinline__ double omac(double a, double x, double acc) { return fma(a, x, acc); }
+__device__ __forceinline__ cplx omac(double a, cplx x, cplx acc) { return cplx{fma(a, x.re, acc.re), fma(a, x.im, acc.im)}; }
+__device__ __forceinline__ cplx omac(cplx a, cplx x, cplx acc)
+{
+    return cplx{fma(a.re, x.re, fma(-a.im, x.im, acc.re)), fma(a.re, x.im, fma(a.im, x.re, acc.im))};
+}
+
+template <typename TV>
+__device__ __forceinline__ TV tv_make(double re, double im);
+template <>
+__device__ __forceinline__ double tv_make<double>(double re, double) { return re; }
+template <>
+__device__ __forceinline__ cplx tv_make<cplx>(double re, double im) { return cplx{re, im}; }
+
+template <typename TV>
+__device__ __forceinline__ TV tv_neg(TV a);
+template <>
+__device__ __forceinline__ double tv_neg<double>(double a) { return -a; }
+template <>
+__device__ __forceinline__ cplx tv_neg<cplx>(cplx a) { return cplx{-a.re, -a.im}; }
+
+// Contribution of call o to the row of state s (row index `self`): returns true and
+// (col, value) when the call has a matrix element in this row.  Values are the table
+// values of fermi_operator.h: sign * t for Make_H, sign * (+-t) for Make_J, t for Make_N /
+// Insert_2Nop, sign * t e^{+-i phi} for the phase hop.
+template <typename TV>
+__device__ __forceinline__ bool op_entry(const OtfOp &op, uint32_t s, int32_t self, const int32_t *__restrict__ addr,
+                                         int32_t &col, TV &val)
+{
+    const uint32_t oi = (s >> op.i) & 1u, oj = (s >> op.j) & 1u;
+    switch (op.kind) {
+    case ED_OP_DENSITY:
+        if (!oi) return false;
+        col = self;
+        val = tv_make<TV>(op.vf_re, op.vf_im);
+        return true;
+    case ED_OP_DENSITY2:
+        if (!(oi & oj)) return false;
+        col = self;
+        val = tv_make<TV>(op.vf_re, op.vf_im);
+        return true;
+    default: {
+        if (oi == oj) return false;
+        col = addr[s ^ op.m];
+        const TV v = oi ? tv_make<TV>(op.vf_re, op.vf_im) : tv_make<TV>(op.vb_re, op.vb_im);
+        val = (__popc(s & op.between) & 1) ? tv_neg<TV>(v) : v;
+        return true;
+    }
+    }
+}
+
+
+// ---------------------------------------------------------------------------------
+// pass 1 body: UP term of logical tile t (slab-major, OW columns x 256/OW rows)
+// ---------------------------------------------------------------------------------
+template <typename V, typename TV, bool SCALE, int OW>
+__device__ __forceinline__ void otf_up_body(const OtfArgs &p, int64_t t, int64_t nrp, const V *__restrict__ x, V *out,
+                                            double sc, TV *s_val, int32_t *s_col, int *s_cnt)
+{
+    constexpr int RW = 64 / OW;
+    const int nops = p.up.nops;
+    const int64_t slab = t / nrp, rp = t - slab * nrp;
+    const int64_t u0 = rp * (256 / OW);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int rr = 0; rr < RW; ++rr) {
+        const int r = wave * RW + rr;
+        const int64_t u = u0 + r;
+        bool has = false;
+        int32_t col = 0;
+        TV val{};
+        if (u < p.dim_up && lane < nops) {
+            const OtfOp op = p.up.ops[lane];
+            has = op_entry<TV>(op, (uint32_t)p.state_up[u], (int32_t)u, p.up.addr, col, val);
+        }
+        const uint64_t bal = __ballot(has);
+        if (has) {
+            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+            s_col[r * nops + pos] = col;
+            s_val[r * nops + pos] = val;
+        }
+        if (lane == 0) s_cnt[r] = __popcll(bal);
+    }
+    __syncthreads();
+    const int r = wave * RW + lane / OW, c = lane % OW;
+    const int64_t u = u0 + r, d = slab * OW + c;
+    if (u >= p.dim_up || d >= p.dim_dn) return;
+    V acc = VO<V>::zero();
+    const int n = s_cnt[r];
+    const int32_t *cl = s_col + r * nops;
+    const TV *vl = s_val + r * nops;
+    // loads in batches of 8 so several are in flight per lane; the FMAs stay in call order
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        V xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (j0 + k < n) xv[k] = x[(int64_t)cl[j0 + k] * p.dim_dn + d];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (j0 + k < n) {
+                V xk = xv[k];
+                if (SCALE) xk = VO<V>::scale(sc, xk);
+                acc = omac(vl[j0 + k], xk, acc);
+            }
+    }
+    out[u * p.dim_dn + d] = acc;
+}
+
+__device__ __forceinline__ double diag_of(int32_t su, int32_t sd, const OtfArgs &p)
+{
+    const uint32_t dbl = (uint32_t)(su & sd) & (uint32_t)p.mask;
+    if (!p.Uvec) return p.U * (double)__popc(dbl);
+    double s = 0.0;
+    uint32_t m = dbl;
+    while (m) {
+        const int i = __ffs(m) - 1;
+        s += p.Uvec[i];
+        m &= m - 1;
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------------
+// pass 2 body: DN term + diagonal + epilogue of logical tile t (R rows x 256 columns);
+// returns the block's Re<v,y> partial (valid in thread 0) when HV_DOT is set
+// ---------------------------------------------------------------------------------
+template <typename V, typename TV, int R, int FLAGS>
+__device__ __forceinline__ double otf_dn_body(const OtfArgs &p, int64_t t, int64_t ncb, bool has_up,
+                                              const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
+                                              double sc, double c, double *s_red)
+{
+    const int64_t rp = t / ncb, cb = t - rp * ncb;
+    const int64_t u0 = rp * R, d = cb * 256 + threadIdx.x;
+    const int64_t dim_dn = p.dim_dn;
+    double dot = 0.0;
+    if (d < dim_dn) {
+        V acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            acc[r] = (has_up && u < p.dim_up) ? ypart[u * dim_dn + d] : VO<V>::zero();
+        }
+        const uint32_t sd = (uint32_t)p.state_dn[d];
+        const int nops = p.dn.nops;
+        // calls in batches of 4: the 4 rank lookups, then 4*R gathers, are issued before the
+        // FMAs, which stay in call order; a call without an element in this column points at
+        // the own column with value 0 (adds an exact 0, as the reference's empty DN slots do)
+        constexpr int KB = 4;
+        for (int o0 = 0; o0 < nops; o0 += KB) {
+            int32_t col[KB];
+            TV a[KB];
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                col[k] = (int32_t)d;
+                a[k] = tv_make<TV>(0.0, 0.0);
+                if (o0 + k < nops) {
+                    int32_t ck;
+                    TV ak;
+                    if (op_entry<TV>(p.dn.ops[o0 + k], sd, (int32_t)d, p.dn.addr, ck, ak)) {
+                        col[k] = ck;
+                        a[k] = ak;
+                    }
+                }
+            }
+            V xv[KB][R];
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int64_t u = (u0 + r < p.dim_up) ? u0 + r : u0;
+                    xv[k][r] = x[u * dim_dn + col[k]];
+                }
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    V xk = xv[k][r];
+                    if (FLAGS & HV_SCALE) xk = VO<V>::scale(sc, xk);
+                    acc[r] = omac(a[k], xk, acc[r]);
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t u = u0 + r;
+            if (u >= p.dim_up) continue;
+            const int64_t i = u * dim_dn + d;
+            V xd = x[i];
+            if (FLAGS & HV_SCALE) xd = VO<V>::scale(sc, xd);
+            if (p.has_diag) acc[r] = VO<V>::fmr(diag_of(p.state_up[u], (int32_t)sd, p), xd, acc[r]);
+            V o = acc[r];
+            if (FLAGS & HV_AXPY) o = VO<V>::fmr(c, w[i], o);
+            y[i] = o;
+            if (FLAGS & HV_WRITEV) vout[i] = xd;
+            if (FLAGS & HV_DOT) dot += VO<V>::dotre(xd, o);
+        }
+    }
+    if (FLAGS & HV_DOT) return block_sum<256>(dot, s_red);
+    return 0.0;
+}
+
+} // namespace ed

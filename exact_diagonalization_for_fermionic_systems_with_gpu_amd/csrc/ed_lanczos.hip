@@ -246,7 +246,8 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
                                 double *eval, void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream)
 {
     const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
-    if (!h || !d_f || !d_evec || !eval || d_f == d_evec) return fail(ED_EINVAL, "ed_lanczos_eigen: bad arguments");
+    if (!h || !d_f || !eval || d_f == d_evec) return fail(ED_EINVAL, "ed_lanczos_eigen: bad arguments");
+    const bool want_vec = d_evec != nullptr;
     const bool cvec = complex_vectors != 0;
     if (h->cplx_tables && !cvec) return fail(ED_EINVAL, "ed_lanczos_eigen: complex tables need complex vectors");
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
@@ -260,7 +261,7 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     ED_TRY(alloc_work(w, *h, nreal, maxN, true));
     double *ev = (double *)d_evec;
     // evec = f keeps the start vector for the rebuild (HubMatrix_GPU.cuh:566)
-    ED_HIP(hipMemcpyAsync(ev, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    if (want_vec) ED_HIP(hipMemcpyAsync(ev, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     int64_t nb;
     ED_TRY(dot_partials(nreal, (const double *)d_f, (const double *)d_f, w.partial, &nb, st));
     ED_TRY(finalize_sum(w.partial, nb, w.bdev, 1, st));
@@ -317,6 +318,14 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     if (lastN < 0) return fail(ED_EINVAL, "ed_lanczos_eigen: no steps run");
     const int Nk = lastN;
 
+    if (!want_vec) {
+        ED_HIP(hipStreamSynchronize(st));
+        if (a_out) memcpy(a_out, a.data(), sizeof(double) * (size_t)(Nk + 1));
+        if (b_out) memcpy(b_out, b.data(), sizeof(double) * (size_t)(Nk + 2));
+        if (nsteps) *nsteps = Nk + 1;
+        if (status != ED_OK) return fail(status, "ed_lanczos_eigen: convergence test not met in %d steps", maxN);
+        return ED_OK;
+    }
     // Ritz vector: rerun the recurrence from the saved start (HubMatrix_GPU.cuh:600-612)
     A = (double *)d_f;
     B = w.buf[0];

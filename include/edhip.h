@@ -132,7 +132,7 @@ typedef struct {
 
 /* Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613).  d_f: start vector in (any norm),
  * clobbered.  d_evec: normalized Ritz vector out, rebuilt by re-running the
- * recurrence.  a_out/b_out (host, max_steps+2 entries, may be NULL) receive the
+ * recurrence (NULL: eigenvalue only, no rebuild).  a_out/b_out (host, max_steps+2 entries, may be NULL) receive the
  * tridiagonal; *nsteps = N+1 steps kept.  Returns ED_ENOCONV (with the last Ritz
  * pair) when the test is not met by max_steps.  Synchronises the stream. */
 int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, double *eval,
@@ -148,6 +148,23 @@ int ed_lanczos_tridiagonal(ed_hubbard h, int complex_vectors, int nsteps, const 
 int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
 /* all eigenvalues ascending */
 int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
+
+/* ---------------- batched ground states ----------------------------------------- */
+
+/* Many independent instances (sectors, twist fluxes, disorder realisations), each built by
+ * ed_hubbard_create_ops, solved together: every Lanczos step of all active instances is one
+ * launch per kernel over a shared tile list, and ten steps (the convergence-check cadence)
+ * are captured in a hipGraph and replayed.  Each instance runs the recurrence and
+ * convergence test of Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613) from the seed-`seed`
+ * start vector of its own length; only eigenvalues are produced.  No counterpart in the
+ * reference, whose drivers loop over instances (e.g. drude_v2_cal_Kagome_randomHop.cu:129-175). */
+typedef struct ed_batch_s *ed_batch;
+int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vectors, ed_batch *out);
+int ed_batch_destroy(ed_batch b);
+/* evals[ninst], nsteps[ninst], status[ninst] (ED_OK or ED_ENOCONV) out; use_graph = 0 launches
+ * the same kernels without capture.  Synchronises. */
+int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
+                    int *nsteps, int *status, void *stream);
 
 /* ---------------- observables -------------------------------------------------- */
 
