@@ -166,6 +166,23 @@ int ed_batch_destroy(ed_batch b);
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
 
+/* ---------------- adjoint / Hellmann-Feynman --------------------------------------- */
+
+/* For every call o of one species' list (species 0 = up, 1 = down) of an operator built by
+ * ed_hubbard_create_ops: out[4o..4o+3] = (Re F_o, Im F_o, Re G_o, Im G_o) with
+ *   F_o = <a| part of call o acting on rows with site i occupied |b>,
+ *   G_o = <a| part acting on rows with site j occupied |b>
+ * (matrix elements as built: sign * value * e^{+-i phase}).  H is linear in each call's value,
+ * so dH/dt_o = (F_o + G_o)/t_o and the phase derivative is i (F_o - G_o).  Density calls
+ * return F only.  These are the parameter cotangents of y = H(theta) x for a = grad_y, b = x,
+ * and the Hellmann-Feynman derivatives for a = b = psi.  No counterpart in the reference.
+ * Synchronises. */
+int ed_bond_bilinear(ed_hubbard h, int species, int complex_vectors, const void *d_a, const void *d_b, double *out,
+                     void *stream);
+/* out[i] = Re sum over states with site i doubly occupied of conj(a) b, i < N (dH/dU_i). */
+int ed_site_double_occupancy(ed_hubbard h, int complex_vectors, const void *d_a, const void *d_b, double *out,
+                             void *stream);
+
 /* ---------------- observables -------------------------------------------------- */
 
 /* TotalSpin::Hubbard (HubMatrix_TotalSpin.cuh:126-173): y = (S+S- + S-S+)/2 x, real or complex. */
