@@ -35,8 +35,9 @@ def _ring6():
 
 
 def _unit_parts(N, Q, op_ij):
-    """P^f, P^b of one phase-hop call on the Q sector (from the oracle's tables)."""
-    j, i = op_ij
+    """P^f, P^b of the call for bond i -> j (c+_j c_i carries e^{+i phi}) on the Q sector,
+    from the oracle's tables: M(phi) = e^{i phi} P^f + e^{-i phi} P^b."""
+    i, j = op_ij
 
     def mat(phase):
         sp = O.Species(N, Q, [(L.OP_HOP_PHASE, j, i, 1.0, phase)], complex_tables=True)
