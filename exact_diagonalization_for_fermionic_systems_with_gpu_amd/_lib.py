@@ -61,11 +61,14 @@ SIGNATURES = {
     "ed_lanczos_tridiagonal": (_i32, [_p, _i32, _i32, _p, _pd, _pd, _p]),
     "ed_tridiag_lowest": (_i32, [_i32, _pd, _pd, _pd, _pd]),
     "ed_tridiag_eigvals": (_i32, [_i32, _pd, _pd, _pd]),
+    "ed_eigs_lowest": (_i32, [_p, _i32, _i32, _i32, _i32, _d, C.c_uint32, _pd, _p, _pi, _pi, _p]),
+    "ed_hubbard_apply_host": (_i32, [_p, _i32, _p, _p, _p]),
     "ed_batch_create": (_i32, [_i32, _p, _i32, C.POINTER(_p)]),
     "ed_batch_destroy": (_i32, [_p]),
     "ed_batch_lowest": (_i32, [_p, C.POINTER(LanczosCfg), C.c_uint32, _i32, _pd, _pi, _pi, _p]),
     "ed_bond_bilinear": (_i32, [_p, _i32, _i32, _p, _p, _pd, _p]),
     "ed_site_double_occupancy": (_i32, [_p, _i32, _p, _p, _pd, _p]),
+    "ed_site_density": (_i32, [_p, _i32, _p, _p, _pd, _p]),
     "ed_s2_create": (_i32, [_i32, _i64, _p, _p, _i64, _p, _p, C.POINTER(_p)]),
     "ed_s2_destroy": (_i32, [_p]),
     "ed_s2_apply": (_i32, [_p, _i32, _p, _p, _p]),

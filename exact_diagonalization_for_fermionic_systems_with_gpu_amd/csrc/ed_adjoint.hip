@@ -113,6 +113,40 @@ __global__ __launch_bounds__(256) void k_site_docc(OtfArgs p, int N, const doubl
     }
 }
 
+// partial[blk][sigma * N + i] = Re sum conj(a) b over elements whose species-sigma state has
+// site i occupied: the gradient of an on-site potential eps_{i sigma} n_{i sigma}
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_site_dens(OtfArgs p, int N, const double *__restrict__ a,
+                                                   const double *__restrict__ b, double *__restrict__ partial)
+{
+    __shared__ double red[4];
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t D = p.dim_up * p.dim_dn;
+    double au[32], ad[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) au[i] = ad[i] = 0.0;
+    if (g < D) {
+        const int64_t u = g / p.dim_dn, d = g - u * p.dim_dn;
+        const uint32_t su = (uint32_t)p.state_up[u], sd = (uint32_t)p.state_dn[d];
+        double re;
+        if (CPLX) re = a[2 * g] * b[2 * g] + a[2 * g + 1] * b[2 * g + 1];
+        else re = a[g] * b[g];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            if ((su >> i) & 1u) au[i] += re;
+            if ((sd >> i) & 1u) ad[i] += re;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        if (i >= N) break; // uniform
+        const double r = block_sum<256>(au[i], red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 2 * N + i] = r;
+        const double q = block_sum<256>(ad[i], red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 2 * N + N + i] = q;
+    }
+}
+
 // sum nrows x ncols partials over rows (fixed order) -> out[ncols]
 __global__ __launch_bounds__(256) void k_colsum(const double *__restrict__ partial, int64_t nrows, int ncols,
                                                 double *__restrict__ out)
@@ -197,6 +231,38 @@ extern "C" int ed_site_double_occupancy(ed_hubbard hh, int complex_vectors, cons
             hipMemcpyAsync(out, dout, sizeof(double) * N, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             s = fail(ED_EHIP, "ed_site_double_occupancy: kernel failed");
+    }
+    dfree(dpart);
+    dfree(dout);
+    return s;
+}
+
+extern "C" int ed_site_density(ed_hubbard hh, int complex_vectors, const void *d_a, const void *d_b, double *out,
+                               void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_a || !d_b || !out) return fail(ED_EINVAL, "ed_site_density: bad arguments");
+    hipStream_t st = stream_of(stream);
+    const int N = h->N;
+    OtfArgs p = h->otf;
+    p.dim_up = h->args.dim_up;
+    p.dim_dn = h->args.dim_dn;
+    p.state_up = h->d_state_up;
+    p.state_dn = h->d_state_dn;
+    const int64_t D = p.dim_up * p.dim_dn, nb = (D + 255) / 256;
+    double *dpart = nullptr, *dout = nullptr;
+    ED_TRY(dmalloc((void **)&dpart, sizeof(double) * (size_t)nb * 2 * N));
+    int s = dmalloc((void **)&dout, sizeof(double) * 2 * N);
+    if (!s) {
+        if (complex_vectors)
+            hipLaunchKernelGGL(k_site_dens<true>, dim3((unsigned)nb), dim3(256), 0, st, p, N, (const double *)d_a, (const double *)d_b, dpart);
+        else
+            hipLaunchKernelGGL(k_site_dens<false>, dim3((unsigned)nb), dim3(256), 0, st, p, N, (const double *)d_a, (const double *)d_b, dpart);
+        hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, dpart, nb, 2 * N, dout);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            s = fail(ED_EHIP, "ed_site_density: kernel failed");
     }
     dfree(dpart);
     dfree(dout);

@@ -1,0 +1,107 @@
+"""Disorder ensembles of the open Hubbard chain (the MBL drivers' model,
+MBL_test/src/Hubbard_EE_test.cu:50-140: Line16 bonds with matrix element -t and an on-site
+potential Make_N(i, eps_i) per species).
+
+For each seeded realisation eps_i ~ Uniform[-W, W] (the same potential for both species;
+numpy PCG64 seeded with seed0 + r) this module computes
+  * E0 of every realisation in one batched Lanczos launch (ed_batch_lowest),
+  * the nev lowest eigenvalues and Ritz vectors (ed_eigs_lowest, the cuSpectra solver role),
+  * the half-chain entanglement entropy S_vN and the reference's S_ref = -sum sigma ln sigma of
+    the ground state (particle-number-block rho_A, ed_entanglement_entropy),
+  * the Hellmann-Feynman gradient dE0/d eps_i = <n_{i up}> + <n_{i dn}> (ed_site_density).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import call, dptr, stream_ptr
+from .batched import BatchedLanczos
+from .fermi_operator import H_TBq
+from .hubmatrix import Hubbard
+from .spectra import HubbardOp, SymEigsSolver
+from . import lattices as L
+
+
+def chain_disorder_ops(L_sites: int, eps, t: float = 1.0):
+    """Line16 (Hubbard_EE_test.cu:119-140) with uniform hopping: Make_H(i, i+1, -t), then
+    Make_N(i, eps_i)."""
+    ops = [(L.OP_HOP, i, i + 1, -float(t)) for i in range(L_sites - 1)]
+    ops += [(L.OP_DENSITY, i, i, float(eps[i])) for i in range(L_sites)]
+    return ops
+
+
+def realisation_eps(r: int, L_sites: int, W: float, seed0: int = 2024):
+    return np.random.default_rng(seed0 + r).uniform(-W, W, L_sites)
+
+
+def build_realisation(eps, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0) -> Hubbard:
+    ops = chain_disorder_ops(L_sites, eps, t)
+    hu = H_TBq(L_sites, n_up)
+    hu.add_ops(ops)
+    hd = H_TBq(L_sites, n_dn)
+    hd.add_ops(ops)
+    return Hubbard(hu, hd, U)
+
+
+@dataclass
+class EnsembleResult:
+    eps: np.ndarray          # [R, L]
+    e0_batched: np.ndarray   # [R]
+    evals: np.ndarray        # [R, nev]
+    entropy: np.ndarray      # [R, 2] (S_vN, S_ref)
+    grad_eps: np.ndarray     # [R, L]
+    seconds: float
+    seconds_batched_e0: float
+
+    @property
+    def realisations_per_s(self):
+        return len(self.eps) / self.seconds
+
+
+def site_density(H: Hubbard, psi: torch.Tensor) -> np.ndarray:
+    """[2, N]: <psi|n_{i sigma}|psi> (psi normalised)."""
+    out = np.zeros(2 * H.N)
+    call("ed_site_density", H.handle, int(psi.is_complex()), dptr(psi), dptr(psi),
+         out.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
+    return out.reshape(2, H.N)
+
+
+def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: int):
+    out = np.zeros(2)
+    nsv = C.c_int64(0)
+    call("ed_entanglement_entropy", H.N, n_up, n_dn, (1 << LA) - 1, int(psi.is_complex()), dptr(psi),
+         out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nsv), stream_ptr())
+    return out
+
+
+def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
+                 tol=1e-10, maxit=1000) -> EnsembleResult:
+    eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hs = [build_realisation(eps[r], L_sites, n_up, n_dn, U, t) for r in range(n_real)]
+    tb = time.perf_counter()
+    e0b, _, _ = BatchedLanczos(hs, False).lowest(tol=1e-12)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - tb
+    evals = np.zeros((n_real, nev))
+    S = np.zeros((n_real, 2))
+    g = np.zeros((n_real, L_sites))
+    for r, H in enumerate(hs):
+        es = SymEigsSolver(HubbardOp(H), nev, ncv)
+        es.init()
+        es.compute("SmallestAlge", maxit, tol)
+        if es.info() != "Successful":
+            raise RuntimeError(f"realisation {r}: eigensolver did not converge")
+        evals[r] = es.eigenvalues()
+        psi = es.eigenvectors()[0]
+        S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2)
+        g[r] = site_density(H, psi).sum(0)
+        del es, psi
+    torch.cuda.synchronize()
+    return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb)

@@ -149,6 +149,24 @@ int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, d
 /* all eigenvalues ascending */
 int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
 
+/* ---------------- several lowest eigenpairs (cuSpectra) ------------------------- */
+
+/* cuSpectra::SymEigsSolver<cuSpectra::Hubbard<T>>(op, nev, ncv) + compute(SmallestAlge, maxit,
+ * tol) (MBL_test/include/SymEigsSolver.cuh:36-475, ArnoldiOp.cuh:132-219): the nev lowest
+ * eigenvalues (ascending) by thick-restart Lanczos with full reorthogonalisation on a basis of
+ * ncv device vectors (1 <= nev < ncv <= 64); maxit restarts at most; convergence as the
+ * reference: |residual_i| <= tol * max(eps^(2/3), |theta_i|).  Start vector: the seed-`seed`
+ * GenRand vector (the reference uses Spectra's SimpleRandom; eigenvalues do not depend on it).
+ * d_evecs (NULL or nev vectors, vector i at offset i*D elements) receives the Ritz vectors.
+ * Returns ED_ENOCONV when fewer than nev pairs converged (evals then hold the last Ritz values). */
+int ed_eigs_lowest(ed_hubbard h, int complex_vectors, int nev, int ncv, int maxit, double tol, uint32_t seed,
+                   double *evals, void *d_evecs, int *nconv, int *niter, void *stream);
+
+/* cuSpectra::Hubbard<T>::perform_op(x_host, y_host) (HubMatrix_TotalSpin.cuh:181-322): y = H x
+ * on host arrays of D elements; device buffers are allocated, filled and freed per call, as in
+ * the reference (synchronous). */
+int ed_hubbard_apply_host(ed_hubbard h, int complex_vectors, const void *x, void *y, void *stream);
+
 /* ---------------- batched ground states ----------------------------------------- */
 
 /* Many independent instances (sectors, twist fluxes, disorder realisations), each built by
@@ -182,6 +200,11 @@ int ed_bond_bilinear(ed_hubbard h, int species, int complex_vectors, const void 
 /* out[i] = Re sum over states with site i doubly occupied of conj(a) b, i < N (dH/dU_i). */
 int ed_site_double_occupancy(ed_hubbard h, int complex_vectors, const void *d_a, const void *d_b, double *out,
                              void *stream);
+/* out[sigma*N + i] = Re <a| n_{i sigma} |b> (sigma = 0 up, 1 down): the gradient of on-site
+ * potentials eps_{i sigma} n_{i sigma} (the Make_N(i, eps) disorder terms of the MBL drivers,
+ * Hubbard_EE_test.cu:119-140); no counterpart in the reference, which has no gradients. */
+int ed_site_density(ed_hubbard h, int complex_vectors, const void *d_a, const void *d_b, double *out,
+                    void *stream);
 
 /* ---------------- observables -------------------------------------------------- */
 

@@ -26,6 +26,7 @@
 #include <complex.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -651,6 +652,86 @@ int or_lanczos_eigen(const or_op *op, int maxN, double tolconv, double *f, doubl
     *nsteps = N + 1;
     free(v0);
     free(v1);
+    free(z);
+    return status;
+}
+
+/* ---- a14, two-vector form for sectors whose three D-vectors do not fit host memory
+ * (18-site Kagome: D = 2.36e9, 18.9 GB per vector).  Same recurrence, start vector
+ * (seed-235 GenRand, normalized) and convergence test as or_lanczos_eigen, eigenvalue
+ * only.  The buffer holding v0 receives f = H v1 - b v0 row by row (each row of f needs
+ * only v1 and the same row of v0), so two D-vectors suffice. */
+static void apply_minus_rows(const or_op *op, const double *x, double *y, double c)
+{
+    const int64_t du = op->dim_up, dd = op->dim_dn;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t u = 0; u < du; u++) {
+        for (int64_t d = 0; d < dd; d++) {
+            double sum = 0.0;
+            for (int s = 0; s < op->S_up; s++) {
+                double a = op->AxUP[(int64_t)s * du + u];
+                if (a != 0) sum += a * x[(int64_t)op->AjUP[(int64_t)s * du + u] * dd + d];
+            }
+            for (int s = 0; s < op->S_dn; s++) {
+                double a = op->AxDN[(int64_t)s * dd + d];
+                int64_t col = op->AjDN[(int64_t)s * dd + d];
+                sum += a * x[u * dd + col];
+            }
+            if (op->U != 0 || op->Uvec)
+                sum += diag_u(op->stateUP[u], op->stateDN[d], op->U, op->mask, op->Uvec) * x[u * dd + d];
+            y[u * dd + d] = (c != 0.0) ? -c * y[u * dd + d] + sum : sum;
+        }
+    }
+}
+
+int or_lanczos_lowest_2vec(const or_op *op, int maxN, double tolconv, uint32_t seed, double *eval, double *a,
+                           double *b, int *nsteps, int verbose)
+{
+    int64_t n = op->dim_up * op->dim_dn;
+    if (op->cplx) return OR_EINVAL;
+    double *P = malloc(sizeof(double) * (size_t)n), *V = malloc(sizeof(double) * (size_t)n);
+    double *z = malloc(sizeof(double) * (size_t)(maxN + 1)), anorm = 0.0;
+    int N, status = OR_ENOCONV;
+    if (!P || !V || !z) {
+        free(P);
+        free(V);
+        free(z);
+        return OR_ENOMEM;
+    }
+    or_random_vector(n, seed, V);
+    or_normalize(n, V); /* random_vector (HubMatrix_GPU.cuh:531-539) */
+    b[0] = sqrt(dot_r(n, V, V));
+    *nsteps = 0;
+    for (N = 0; N < maxN; ++N) {
+        double inv = 1.0 / b[N];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) V[i] = inv * V[i]; /* v1 = f * (1/b) */
+        apply_minus_rows(op, V, P, N != 0 ? b[N] : 0.0); /* P: v0 -> f = H v1 - b v0 */
+        a[N] = dot_r(n, V, P);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) P[i] = -a[N] * V[i] + P[i];
+        b[N + 1] = sqrt(dot_r(n, P, P));
+        double *t = P; /* v0 <- v1, f stays in the other buffer */
+        P = V;
+        V = t;
+        if (N == 0) anorm = fabs(a[0] + b[1]);
+        else anorm = fmax(anorm, b[N] + fabs(a[N] + b[N + 1]));
+        if (N % 10 == 9 || N == maxN - 1 || b[N + 1] == 0.0) {
+            *eval = or_tridiag_lowest(N + 1, a, b + 1, z);
+            *nsteps = N + 1;
+            if (verbose) {
+                fprintf(stderr, "[oracle 2vec] step %d E0 %.15f |b z_N| %.3e\n", N + 1, *eval, fabs(b[N + 1] * z[N]));
+                fflush(stderr);
+            }
+            if (b[N + 1] == 0.0 ||
+                (fabs(b[N + 1] * z[N]) < tolconv * anorm && fabs(z[0]) > tolconv * anorm)) {
+                status = OR_OK;
+                break;
+            }
+        }
+    }
+    free(P);
+    free(V);
     free(z);
     return status;
 }

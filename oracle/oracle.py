@@ -76,6 +76,8 @@ def lib():
         L.or_op_apply.argtypes = [_p, _p, _p]
         L.or_lanczos_eigen.restype = C.c_int
         L.or_lanczos_eigen.argtypes = [_p, C.c_int, C.c_double, _p, _p, _p, _p, _p, _p]
+        L.or_lanczos_lowest_2vec.restype = C.c_int
+        L.or_lanczos_lowest_2vec.argtypes = [_p, C.c_int, C.c_double, C.c_uint32, _p, _p, _p, _p, C.c_int]
         L.or_lanczos_tridiagonal.restype = C.c_int
         L.or_lanczos_tridiagonal.argtypes = [_p, C.c_int, _p, _p, _p]
         L.or_continued_fraction.restype = None
@@ -193,6 +195,20 @@ class Hubbard:
                                     C.byref(ns))
         n = ns.value
         return st, e0.value, evec, a[:n].copy(), b[: n + 1].copy(), n
+
+    def lanczos_lowest_2vec(self, maxN=1000, tol=None, seed=235, verbose=False):
+        """Eigenvalue-only Lanczos from the seed-235 start with two D-vectors (same recurrence and
+        test as lanczos_eigen).  Returns (status, E0, a, b, nsteps)."""
+        if tol is None:
+            tol = 0.01 * np.sqrt(np.finfo(float).eps)
+        a = np.zeros(maxN + 1)
+        b = np.zeros(maxN + 2)
+        e0 = C.c_double(0.0)
+        ns = C.c_int(0)
+        st = lib().or_lanczos_lowest_2vec(self._op, maxN, tol, seed, C.byref(e0), _ptr(a), _ptr(b), C.byref(ns),
+                                          int(verbose))
+        n = ns.value
+        return st, e0.value, a[:n].copy(), b[: n + 1].copy(), n
 
     def lanczos_tridiagonal(self, f, maxN=200):
         """Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): a[0..maxN-1], b[0..maxN]."""

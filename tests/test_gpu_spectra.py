@@ -1,0 +1,161 @@
+"""Several lowest eigenpairs (ed_eigs_lowest, the cuSpectra::SymEigsSolver role), the host-array
+perform_op, and the disorder-ensemble pipeline (lowest eigenvalues, half-chain entropy, dE0/d eps_i)
+against the CPU oracle: dense numpy eigendecompositions of the oracle's H on small sectors, ARPACK
+(scipy eigsh) over the oracle's H.v on larger ones, and the committed 14-site chain fixture
+(tests/golden/chain14_eigs_oracle.json, written by tests/golden/make_chain14_eigs.py).
+
+Bars: eigenvalues 1e-10 relative; entropies and site densities 1e-8 (ground states converged to a
+residual of 1e-13 |E0|; the 8-site W = 8 realisations have gaps down to 1e-3); perform_op within
+1e-14 of max|y| (same kernels as H.v).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+from scipy.sparse.linalg import LinearOperator, eigsh
+
+from oracle import oracle as O
+from oracle import ee_oracle as EE
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard, Lanczos
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd.spectra import HubbardOp, SymEigsSolver
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "chain14_eigs_oracle.json")
+
+
+def _pair(N, Qu, Qd, ops, U, cplx=False):
+    hu = H_TBq(N, Qu, complex_tables=cplx)
+    hu.add_ops(ops)
+    hd = H_TBq(N, Qd, complex_tables=cplx)
+    hd.add_ops(ops)
+    return Hubbard(hu, hd, U), O.Hubbard(O.Species(N, Qu, ops, complex_tables=cplx),
+                                         O.Species(N, Qd, ops, complex_tables=cplx), U)
+
+
+def _dense(Ho):
+    D = Ho.dim
+    e = np.zeros(D, dtype=Ho.dtype())
+    M = np.zeros((D, D), dtype=Ho.dtype())
+    for k in range(D):
+        e[:] = 0
+        e[k] = 1
+        M[:, k] = Ho.apply(e)
+    return M
+
+
+def test_eigs_and_perform_op_vs_dense_chain8():
+    eps = DS.realisation_eps(0, 8, 8.0)
+    ops = DS.chain_disorder_ops(8, eps)
+    Hg, Ho = _pair(8, 4, 4, ops, 4.0)
+    print("eigs chain8 4/4 W=8", flush=True)
+    w, V = np.linalg.eigh(_dense(Ho))
+    es = SymEigsSolver(HubbardOp(Hg), 4, 16)
+    es.init()
+    assert es.compute("SmallestAlge", 1000, 1e-13) == 4 and es.info() == "Successful"
+    ev = es.eigenvalues()
+    assert np.allclose(ev, w[:4], rtol=1e-10, atol=0), (ev, w[:4])
+    X = es.eigenvectors()
+    for k in range(4):
+        y = Hg.apply(X[k])
+        r = float((y - ev[k] * X[k]).norm())
+        assert r < 1e-9 * abs(ev[k]), (k, r)
+    # perform_op on host arrays
+    x = np.random.default_rng(1).normal(size=Hg.dim)
+    y = np.empty_like(x)
+    HubbardOp(Hg).perform_op(x, y)
+    yo = Ho.apply(x)
+    assert np.abs(y - yo).max() <= 1e-14 * np.abs(yo).max()
+
+
+def test_eigs_complex_twist_lieb12_vs_arpack():
+    ops = L.lieb_twisted_ops(2 * math.pi * 3 / 16, t=1.0)
+    Hg, Ho = _pair(12, 3, 9, ops, 4.0, cplx=True)
+    print("eigs lieb12 3/9 twisted (complex)", flush=True)
+    A = LinearOperator((Ho.dim, Ho.dim), matvec=lambda x: Ho.apply(np.ascontiguousarray(x).ravel()),
+                       dtype=np.complex128)
+    wa = np.sort(eigsh(A, k=5, which="SA", tol=1e-13, ncv=30)[0])
+    es = SymEigsSolver(HubbardOp(Hg, True), 5, 24)
+    es.init()
+    assert es.compute("SmallestAlge", 1000, 1e-12) == 5
+    assert np.allclose(es.eigenvalues(), wa, rtol=1e-10, atol=1e-12), (es.eigenvalues(), wa)
+
+
+def test_eigs_kagome12_vs_lanczos_and_arpack():
+    ops = L.hubbard_ops(L.KAGOME12, t=-1.0)
+    Hg, Ho = _pair(12, 6, 6, ops, 4.0)
+    print("eigs kagome12 6/6", flush=True)
+    es = SymEigsSolver(HubbardOp(Hg), 4, 20)
+    es.init()
+    assert es.compute("SmallestAlge", 1000, 1e-12) == 4
+    ev = es.eigenvalues()
+    f = torch.empty(Hg.dim, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(f)
+    lz = Lanczos(Hg.dimUP, Hg.dimDN, tol=1e-12)
+    lz.random_vector(f)
+    e0 = lz.eigen(Hg, f, psi)
+    assert abs(ev[0] - e0) < 1e-10 * abs(e0)
+    A = LinearOperator((Ho.dim, Ho.dim), matvec=lambda x: Ho.apply(np.ascontiguousarray(x).ravel()),
+                       dtype=np.float64)
+    wa = np.sort(eigsh(A, k=4, which="SA", tol=1e-13, ncv=24, v0=O.random_vector(Ho.dim, 235))[0])
+    assert np.allclose(ev, wa, rtol=1e-10, atol=0), (ev, wa)
+
+
+def test_disorder_ensemble_vs_dense_chain8():
+    print("disorder ensemble chain8 x4", flush=True)
+    res = DS.run_ensemble(n_real=4, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=4, ncv=16, tol=1e-13)
+    for r in range(4):
+        ops = DS.chain_disorder_ops(8, res.eps[r])
+        Ho = O.Hubbard(O.Species(8, 4, ops), O.Species(8, 4, ops), 4.0)
+        w, V = np.linalg.eigh(_dense(Ho))
+        assert np.allclose(res.evals[r], w[:4], rtol=1e-10, atol=0)
+        assert abs(res.e0_batched[r] - w[0]) < 1e-10 * abs(w[0])
+        psi = V[:, 0]
+        lam, sig = EE.block_spectra(8, 4, 4, 0b1111, psi)
+        assert abs(res.entropy[r, 0] - EE.entropy_from_lambda(lam)) < 1e-8
+        assert abs(res.entropy[r, 1] - EE.entropy_from_sigma(sig)) < 1e-8
+        up, dn = Ho.up, Ho.dn
+        p2 = psi.reshape(up.count, dn.count) ** 2
+        nu = np.array([[(int(s) >> i) & 1 for i in range(8)] for s in up.state], float)
+        nd = np.array([[(int(s) >> i) & 1 for i in range(8)] for s in dn.state], float)
+        assert np.allclose(res.grad_eps[r], p2.sum(1) @ nu + p2.sum(0) @ nd, atol=1e-8)
+        assert abs(res.grad_eps[r].sum() - 8.0) < 1e-10  # particle number
+
+
+def test_disorder_gradient_vs_finite_differences():
+    eps = DS.realisation_eps(3, 8, 8.0)
+    h = 1e-5
+
+    def e0(e):
+        H = DS.build_realisation(e, 8, 4, 4)
+        es = SymEigsSolver(HubbardOp(H), 2, 12)
+        es.compute("SmallestAlge", 1000, 1e-13)
+        return es.eigenvalues()[0], es.eigenvectors()[0], H
+
+    E, psi, H = e0(eps)
+    g = DS.site_density(H, psi).sum(0)
+    for i in (0, 3, 7):
+        ep, em = eps.copy(), eps.copy()
+        ep[i] += h
+        em[i] -= h
+        fd = (e0(ep)[0] - e0(em)[0]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6, (i, fd, g[i])
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
+def test_chain14_against_fixture():
+    gold = json.load(open(GOLD))
+    print("disorder chain14 7/7 x2 vs fixture", flush=True)
+    res = DS.run_ensemble(n_real=2, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=20, tol=1e-13)
+    for rec in gold["realisations"]:
+        r = rec["r"]
+        assert np.allclose(res.eps[r], rec["eps"], rtol=0, atol=0)
+        assert np.allclose(res.evals[r], rec["evals"], rtol=1e-10, atol=0), (res.evals[r], rec["evals"])
+        assert abs(res.e0_batched[r] - rec["evals"][0]) < 1e-10 * abs(rec["evals"][0])
+        assert abs(res.entropy[r, 0] - rec["S_vN"]) < 1e-8
+        assert abs(res.entropy[r, 1] - rec["S_ref"]) < 1e-8
+        assert np.allclose(res.grad_eps[r], rec["dE0_deps"], atol=1e-8)
