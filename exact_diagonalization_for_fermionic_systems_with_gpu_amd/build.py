@@ -32,19 +32,51 @@ def sources():
     return sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")))
 
 
-def build(verbose=False):
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(REPO, "include"), "-o", tmp, *sources(),
-           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libedhip.so")
+        raise RuntimeError(f"hipcc failed: {' '.join(cmd[-3:])}")
+
+
+def build(verbose=False, jobs=None):
+    """Compile every csrc/*.hip to an object (in parallel), then link libedhip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(REPO, "_build", "edhip")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"]
+    objs, cmds = [], []
+    for src in sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *cflags, "-I", os.path.join(REPO, "include"), "-c", "-o", obj, src])
+    jobs = jobs or min(8, os.cpu_count() or 1, len(cmds))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocsolver",
+          "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     return LIB
 
 
+def build_drivers(verbose=False):
+    """Compile the C++ drivers under tests/cpp (written against include/ed/ed.hpp) into
+    tests/cpp/bin, linked to the in-tree libedhip.so by a relative rpath."""
+    out = []
+    srcdir = os.path.join(REPO, "tests", "cpp")
+    bindir = os.path.join(srcdir, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for src in sorted(glob.glob(os.path.join(srcdir, "*.cpp"))):
+        exe = os.path.join(bindir, os.path.splitext(os.path.basename(src))[0])
+        _run([HIPCC, "-std=c++17", "-O2", "-I", os.path.join(REPO, "include"), "-o", exe, src, "-L", PKG, "-ledhip",
+              "-Wl,-rpath,$ORIGIN/../../../" + os.path.basename(PKG)], verbose)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_drivers(verbose=True))

@@ -1,0 +1,103 @@
+// A driver written against the reference's C++ interfaces (through include/ed/ed.hpp): the
+// 12-site Kagome Hubbard model at N_up = N_dn = 6, U = 4, hopping -1, following the steps of
+// the DrudeWeight and TotalSpin drivers (drude_v2_cal_Kagome_randomHop.cu:129-220 and
+// ConservedTotalSpin_Kagome_randomHop.cu:110-125): ground state by Lanczos, S(S+1), the Kubo
+// Drude weight D_x, the 6|6 entanglement entropy, the three lowest eigenvalues from the
+// cuSpectra solver and one host-array perform_op.  Prints one JSON object.
+#include <cstdio>
+#include <vector>
+
+#include "ed/ed.hpp"
+
+using namespace ed;
+
+// KagomeLattice12 bonds (ConservedTotalSpin_Kagome_randomHop.cu:140-156); the first 8 are the
+// horizontal A-B bonds (x projection 1), the other 16 have x projection 1/2
+static const int kFirst[24] = {0, 1, 3, 4, 6, 7, 9, 10, 0, 2, 6, 8, 3, 5, 9, 11, 2, 1, 8, 7, 5, 4, 11, 10};
+static const int kSecond[24] = {1, 3, 4, 0, 7, 9, 10, 6, 2, 6, 8, 0, 5, 9, 11, 3, 1, 11, 7, 5, 4, 8, 10, 2};
+
+static void KagomeLattice12(H_TBq &Hq)
+{
+    for (int b = 0; b < 24; ++b) Hq.Make_H(kFirst[b], kSecond[b], -1.0);
+}
+static void KineticX(H_TBq &Hq) // K_x: d_x^2 weights (drude_v2_cal_Kagome_randomHop.cu:810-839)
+{
+    for (int b = 0; b < 24; ++b) Hq.Make_H(kFirst[b], kSecond[b], -(b < 8 ? 1.0 : 0.25));
+}
+static void CurrentX(H_TBq &Hq) // J_x: d_x weights (drude_v2_cal_Kagome_randomHop.cu:841-870)
+{
+    for (int b = 0; b < 24; ++b) Hq.Make_J(kFirst[b], kSecond[b], b < 8 ? 1.0 : 0.5);
+}
+
+int main()
+{
+    const int N = 12, QUP = 6, QDN = 6;
+    const double U = 4.0, VOL = 8.0 * std::sqrt(3.0);
+    try {
+        H_TBq HqUP(N, QUP), HqDN(N, QDN);
+        KagomeLattice12(HqUP);
+        KagomeLattice12(HqDN);
+        Hubbard Hub(HqUP, HqDN, U);
+        const int64_t D = Hub.dimUP * Hub.dimDN;
+
+        Lanczos<double> lanc(Hub.dimUP, Hub.dimDN);
+        device_vector<double> f(D), evec(D);
+        double eval = 0.0;
+        lanc.random_vector(f);
+        lanc.eigen(Hub, f, eval, evec);
+
+        TotalSpin::Hubbard S2(HqUP, HqDN);
+        device_vector<double> s2v(D);
+        S2.Hubbard_mv(evec, s2v);
+        const double Sxy = lanc.thrust_Dotprod(evec, s2v);
+        const double SS = 0.25 * (QUP - QDN) * (QUP - QDN) + Sxy;
+
+        // Kubo Drude weight (drude_weight_Jx)
+        H_TBq JUP(N, QUP), JDN(N, QDN), KUP(N, QUP), KDN(N, QDN);
+        CurrentX(JUP);
+        CurrentX(JDN);
+        KineticX(KUP);
+        KineticX(KDN);
+        Hubbard Jx(JUP, JDN, 0.0), Kx(KUP, KDN, 0.0);
+        const int NOMEGA = 8000;
+        std::vector<std::complex<double>> omega(NOMEGA), Lambda;
+        for (int k = 0; k < NOMEGA; ++k) omega[k] = std::complex<double>(0.004 * (k + 1), 0.01);
+        Lanczos<double> lanctri(Hub.dimUP, Hub.dimDN);
+        compute_sigma_omega(lanctri, Hub, Jx, eval, evec, omega, Lambda);
+        double vertex = 0.0;
+        for (int k = 0; k < NOMEGA; ++k) {
+            const double sigma = -1.0 / (M_PI * omega[k].real()) * Lambda[k].imag();
+            if (omega[k].real() > 0.01) vertex += sigma * 0.004;
+        }
+        device_vector<double> kv(D);
+        Kx.Hubbard_mv(evec, kv);
+        const double kinetic = lanc.thrust_Dotprod(evec, kv);
+        const double Dw = (-kinetic - 2.0 * vertex) / VOL;
+
+        // 6|6 entanglement entropy (Asite = {0..5})
+        int Asite[6] = {0, 1, 2, 3, 4, 5};
+        EEmatHubAB_GPU<double> EE(Asite, 6, HqUP, HqDN);
+        const auto ent = EE.entropy(evec);
+
+        // three lowest eigenvalues (cuSpectra solver) and one host-array perform_op
+        cuSpectra::Hubbard<double> op(HqUP, HqDN, U);
+        cuSpectra::SymEigsSolver<cuSpectra::Hubbard<double>> eigs(op, 3, 16);
+        eigs.init();
+        const int nconv = eigs.compute(cuSpectra::SortRule::SmallestAlge, 1000, 1e-12);
+        std::vector<double> xh = evec.to_host(), yh(D);
+        op.perform_op(xh.data(), yh.data());
+        double rq = 0.0;
+        for (int64_t i = 0; i < D; ++i) rq += xh[i] * yh[i];
+
+        const auto &ev = eigs.eigenvalues();
+        std::printf("{\"E0\": %.17g, \"nsteps\": %d, \"SS\": %.17g, \"D\": %.17g, \"kinetic\": %.17g, "
+                    "\"S_vN\": %.17g, \"S_ref\": %.17g, \"n_schmidt\": %lld, \"nconv\": %d, "
+                    "\"evals\": [%.17g, %.17g, %.17g], \"rayleigh_host\": %.17g}\n",
+                    eval, lanc.nsteps, SS, Dw, kinetic, ent.S_vN, ent.S_ref, (long long)ent.n_schmidt, nconv, ev[0],
+                    ev[1], ev[2], rq);
+    } catch (const Error &e) {
+        std::fprintf(stderr, "kagome12_driver: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

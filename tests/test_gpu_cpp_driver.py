@@ -1,0 +1,58 @@
+"""The C++ interface (include/ed/ed.hpp): tests/cpp/kagome12_driver.cpp is written against the
+reference's class names (H_TBq, Hubbard, Lanczos<T>, TotalSpin::Hubbard, compute_sigma_omega,
+EEmatHubAB_GPU, cuSpectra::Hubbard / SymEigsSolver) and built by build().  Its output for the
+12-site Kagome model (6/6, U = 4, hopping -1) is checked against the CPU oracle.
+
+Bars: E0, the cuSpectra eigenvalues and the host-array Rayleigh quotient 1e-10 relative; S(S+1)
+1e-8; entropies 1e-8; the Kubo D_x 1e-7 relative (oracle run from its own ground state: the
+GPU-vs-oracle difference from the same state is 6e-9, the rest is the ~1e-9 difference of the
+two converged ground states amplified by the 200 unreorthogonalised Lanczos steps).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import LinearOperator, eigsh
+
+from oracle import oracle as O
+from oracle import ee_oracle as EE
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "bin", "kagome12_driver")
+
+
+def test_kagome12_cpp_driver_against_oracle():
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} missing: run __graft_entry__.build()")
+    print("C++ driver kagome12", flush=True)
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    cl = L.KAGOME12
+    ops = L.hubbard_ops(cl, t=-1.0)
+    up, dn = O.Species(12, 6, ops), O.Species(12, 6, ops)
+    H = O.Hubbard(up, dn, 4.0)
+    st, e0, psi, a, b, n = H.lanczos_eigen(O.random_vector(H.dim, 235))
+    assert st == 0
+    assert abs(out["E0"] - e0) < 1e-10 * abs(e0)
+    assert out["nsteps"] == n
+    ss = 0.25 * 0 + float(psi @ O.s2_apply(up, dn, psi))
+    assert abs(out["SS"] - ss) < 1e-8
+    lam, sig = EE.block_spectra(12, 6, 6, 0b111111, psi)
+    assert abs(out["S_vN"] - EE.entropy_from_lambda(lam)) < 1e-8
+    assert abs(out["S_ref"] - EE.entropy_from_sigma(sig)) < 1e-8
+    kx, _ = L.kinetic_ops(cl)
+    jx, _ = L.current_ops(cl)
+    J = O.Hubbard(O.Species(12, 6, jx), O.Species(12, 6, jx), 0.0)
+    K = O.Hubbard(O.Species(12, 6, kx), O.Species(12, 6, kx), 0.0)
+    D, kin, _ = O.drude_weight(H, J, K, e0, psi)
+    assert abs(out["kinetic"] - kin) < 1e-10 * abs(kin)
+    assert abs(out["D"] - D) < 1e-7 * abs(D), (out["D"], D)
+    A = LinearOperator((H.dim, H.dim), matvec=lambda x: H.apply(np.ascontiguousarray(x).ravel()), dtype=np.float64)
+    wa = np.sort(eigsh(A, k=3, which="SA", tol=1e-13, ncv=20, v0=O.random_vector(H.dim, 235))[0])
+    assert out["nconv"] == 3
+    assert np.allclose(out["evals"], wa, rtol=1e-10, atol=0)
+    assert abs(out["rayleigh_host"] - e0) < 1e-10 * abs(e0)
