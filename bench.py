@@ -66,6 +66,39 @@ def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
     return 16.0 * elems / dt2 / 1e9, rows2, dt2, O.num_threads()
 
 
+def batched_drude_line():
+    """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
+    weight per sector; batched (ed_batch_lowest, hipGraph) vs the plain per-instance loop on the
+    same GPU, both checked against the oracle's Drude weights (tests/golden fixture)."""
+    import numpy as np
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import batched as BT
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "lieb12_drude_oracle.json")))
+    Dg = np.array(gold["D"])
+    BT.lieb12_drude_sweep(batched=True)  # warm-up (allocations, graph capture paths)
+    E0b, Db, tb, _ = BT.lieb12_drude_sweep(batched=True)
+    E0p, Dp, tp, _ = BT.lieb12_drude_sweep(batched=False)
+    nz = np.abs(Dg) > 1e-6
+    err = float(max(np.max(np.abs(Db[nz] - Dg[nz]) / np.abs(Dg[nz])), np.max(np.abs(Dp[nz] - Dg[nz]) / np.abs(Dg[nz]))))
+    log(f"[bench] batched Drude sweep {tb:.3f} s, plain {tp:.3f} s, max rel err vs oracle {err:.2e}")
+    return {"value": round(13 / tb, 3), "unit": "Drude weights/s", "plain": round(13 / tp, 3),
+            "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
+            "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
+
+
+def ensemble_line(n_real, ncv=32):
+    """14-site open chain, 7/7, U = 4, W = 8: E0 of all realisations in one batched launch, then per
+    realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
+    res = DS.run_ensemble(n_real=n_real, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=ncv, tol=1e-10,
+                          log=lambda m: log("[bench] " + m), with_sref=False)
+    log(f"[bench] disorder ensemble: {n_real} realisations in {res.seconds:.2f} s "
+        f"(batched E0 of all: {res.seconds_batched_e0:.2f} s)")
+    return {"value": round(res.realisations_per_s, 3), "unit": "realisations/s", "realisations": n_real,
+            "batched_e0_s": round(res.seconds_batched_e0, 3),
+            "max_abs_e0_batched_minus_eigs": float(abs(res.e0_batched - res.evals[:, 0]).max()),
+            "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +106,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sites", type=int, default=18, choices=[12, 18])
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched Drude sweep and the disorder-ensemble lines")
+    ap.add_argument("--ensemble", type=int, default=64, help="realisations of the 14-site chain ensemble")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run one device copy of a vector (16 D bytes) as the byte-count reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -162,6 +198,11 @@ def main():
                                "sample": f"oracle H.v (C restatement of Hubbard_mv_), rows {16}..{16 + rows} "
                                          f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
         del x_host
+    if not args.no_extras:
+        del x, y
+        torch.cuda.empty_cache()
+        out["batched_drude"] = batched_drude_line()
+        out["disorder_ensemble"] = ensemble_line(args.ensemble)
     print(json.dumps(out), flush=True)
 
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "ed_s2_apply": (_i32, [_p, _i32, _p, _p, _p]),
     "ed_continued_fraction": (_i32, [_i32, _pd, _pd, _d, _d, _i32, _pd, _pd]),
     "ed_entanglement_entropy": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _pi64, _p]),
+    "ed_entanglement_entropy_vn": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _p]),
     "ed_entanglement_spectrum": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _i64, _pi64, _p]),
 }
 

@@ -5,7 +5,8 @@
 // (HubMatrix_TotalSpin.cuh:181-322).
 //
 // Method: thick-restart Lanczos (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602) with
-// full reorthogonalisation (two classical Gram-Schmidt passes per step), which computes the
+// full reorthogonalisation (two classical Gram-Schmidt passes against the whole basis every
+// step: "twice is enough"), which computes the
 // same invariant subspace as the reference's implicitly restarted Lanczos (ARPACK dsaupd
 // logic) but restarts by keeping Ritz vectors instead of applying shifts.  Convergence test of
 // the reference: residual |beta_m y_{m,i}| <= tol * max(eps^(2/3), |theta_i|)
@@ -255,10 +256,10 @@ static int eigs_run(const Hubbard *h, int nev, int ncv, int maxit, double tol, u
         for (int j = k; j < m; ++j) {
             double *w = vec(j + 1);
             ED_TRY(hv_plain(*h, CPLX, vec(j), w, st));
+            int64_t nbv;
+            double b = 0.0;
             ED_TRY(proj_subtract<CPLX>(n, ld, j + 1, W.V, w, W, nb, c1.data(), st));
             ED_TRY(proj_subtract<CPLX>(n, ld, j + 1, W.V, w, W, nb, c2.data(), st));
-            double b = 0.0;
-            int64_t nbv;
             ED_TRY(norm2(nreal, w, W, &nbv, &b, st)); // synchronizes: c1, c2 are on the host now
             if (j == k) {
                 for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = c1[2 * i] + c2[2 * i];

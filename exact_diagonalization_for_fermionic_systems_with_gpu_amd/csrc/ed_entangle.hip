@@ -257,7 +257,7 @@ static rocblas_handle solver_handle()
 
 // spectra of all blocks: lambda (rho eigenvalues, clipped at 0) and sigma (Schmidt values)
 static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const void *d_psi, hipStream_t st,
-                         std::vector<double> &lambda, std::vector<double> &sigma)
+                         std::vector<double> &lambda, std::vector<double> &sigma, bool want_sigma)
 {
     if (N < 2 || N > 31 || Qu < 0 || Qd < 0 || Qu > N || Qd > N || !d_psi)
         return fail(ED_EINVAL, "entanglement: bad arguments");
@@ -357,6 +357,7 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 break;
             }
             for (double v : w) lambda.push_back(v > 0.0 ? v : 0.0);
+            if (!want_sigma) continue;
             // Schmidt values: SVD of the block (column-major K x k, ld K); A is overwritten
             if (cplx)
                 rs = rocsolver_zgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k,
@@ -394,7 +395,7 @@ extern "C" int ed_entanglement_entropy(int N, int Q_up, int Q_dn, int64_t maskA,
 {
     if (!out) return fail(ED_EINVAL, "ed_entanglement_entropy: null output");
     std::vector<double> lam, sig;
-    ED_TRY(block_spectra(N, Q_up, Q_dn, maskA, complex_vectors != 0, d_psi, stream_of(stream), lam, sig));
+    ED_TRY(block_spectra(N, Q_up, Q_dn, maskA, complex_vectors != 0, d_psi, stream_of(stream), lam, sig, true));
     double svn = 0.0, sref = 0.0;
     int64_t nsv = 0;
     for (double l : lam)
@@ -415,10 +416,23 @@ extern "C" int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA
                                         void *stream)
 {
     std::vector<double> lam, sig;
-    ED_TRY(block_spectra(N, Q_up, Q_dn, maskA, complex_vectors != 0, d_psi, stream_of(stream), lam, sig));
+    ED_TRY(block_spectra(N, Q_up, Q_dn, maskA, complex_vectors != 0, d_psi, stream_of(stream), lam, sig, false));
     std::sort(lam.begin(), lam.end(), [](double a, double b) { return a > b; });
     if (nlambda) *nlambda = (int64_t)lam.size();
     if (lambda_out)
         for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)lam.size()); ++i) lambda_out[i] = lam[(size_t)i];
+    return ED_OK;
+}
+
+extern "C" int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors,
+                                          const void *d_psi, double *s_vn, void *stream)
+{
+    if (!s_vn) return fail(ED_EINVAL, "ed_entanglement_entropy_vn: null output");
+    std::vector<double> lam, sig;
+    ED_TRY(block_spectra(N, Q_up, Q_dn, maskA, complex_vectors != 0, d_psi, stream_of(stream), lam, sig, false));
+    double svn = 0.0;
+    for (double l : lam)
+        if (l > 0.0) svn -= l * std::log(l);
+    *s_vn = svn;
     return ED_OK;
 }

@@ -6,8 +6,9 @@ For each seeded realisation eps_i ~ Uniform[-W, W] (the same potential for both 
 numpy PCG64 seeded with seed0 + r) this module computes
   * E0 of every realisation in one batched Lanczos launch (ed_batch_lowest),
   * the nev lowest eigenvalues and Ritz vectors (ed_eigs_lowest, the cuSpectra solver role),
-  * the half-chain entanglement entropy S_vN and the reference's S_ref = -sum sigma ln sigma of
-    the ground state (particle-number-block rho_A, ed_entanglement_entropy),
+  * the half-chain entanglement entropy S_vN of the ground state from the particle-number blocks
+    of rho_A, and optionally the reference's S_ref = -sum sigma ln sigma (per-block SVDs,
+    ed_entanglement_entropy; S_vN alone: ed_entanglement_entropy_vn),
   * the Hellmann-Feynman gradient dE0/d eps_i = <n_{i up}> + <n_{i dn}> (ed_site_density).
 """
 from __future__ import annotations
@@ -71,16 +72,21 @@ def site_density(H: Hubbard, psi: torch.Tensor) -> np.ndarray:
     return out.reshape(2, H.N)
 
 
-def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: int):
-    out = np.zeros(2)
-    nsv = C.c_int64(0)
-    call("ed_entanglement_entropy", H.N, n_up, n_dn, (1 << LA) - 1, int(psi.is_complex()), dptr(psi),
-         out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nsv), stream_ptr())
+def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: int, with_sref=True):
+    """(S_vN, S_ref) of sites 0..LA-1; S_ref (per-block SVDs) is nan when with_sref is False."""
+    out = np.full(2, np.nan)
+    if with_sref:
+        nsv = C.c_int64(0)
+        call("ed_entanglement_entropy", H.N, n_up, n_dn, (1 << LA) - 1, int(psi.is_complex()), dptr(psi),
+             out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nsv), stream_ptr())
+    else:
+        call("ed_entanglement_entropy_vn", H.N, n_up, n_dn, (1 << LA) - 1, int(psi.is_complex()), dptr(psi),
+             out.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
     return out
 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
-                 tol=1e-10, maxit=1000) -> EnsembleResult:
+                 tol=1e-10, maxit=1000, log=None, with_sref=True) -> EnsembleResult:
     eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -99,8 +105,11 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
         if es.info() != "Successful":
             raise RuntimeError(f"realisation {r}: eigensolver did not converge")
         evals[r] = es.eigenvalues()
+        if log is not None:
+            log(f"realisation {r}: {es.niter} restarts, E0 {evals[r, 0]:.12f}, "
+                f"{time.perf_counter() - t0:.2f} s elapsed")
         psi = es.eigenvectors()[0]
-        S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2)
+        S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, with_sref)
         g[r] = site_density(H, psi).sum(0)
         del es, psi
     torch.cuda.synchronize()

@@ -228,6 +228,10 @@ int ed_continued_fraction(int n, const double *a, const double *b, double energy
  * number of nonzero Schmidt values.  Synchronises the stream. */
 int ed_entanglement_entropy(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                             double *out, int64_t *nsv_out, void *stream);
+/* S_vN only (out[0] of ed_entanglement_entropy without the per-block SVDs, which dominate the
+ * cost for blocks of a few hundred rows and more) */
+int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
+                               double *s_vn, void *stream);
 /* all eigenvalues of rho_A, descending, into lambda_out (capacity cap); *nlambda = total count */
 int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                              double *lambda_out, int64_t cap, int64_t *nlambda, void *stream);

@@ -661,26 +661,72 @@ int or_lanczos_eigen(const or_op *op, int maxN, double tolconv, double *f, doubl
  * (seed-235 GenRand, normalized) and convergence test as or_lanczos_eigen, eigenvalue
  * only.  The buffer holding v0 receives f = H v1 - b v0 row by row (each row of f needs
  * only v1 and the same row of v0), so two D-vectors suffice. */
-static void apply_minus_rows(const or_op *op, const double *x, double *y, double c)
+/* compact copy of one species' ELL table: the nonzero slots of each row, in slot order */
+typedef struct {
+    int64_t *start;
+    double *val;
+    int32_t *col;
+} csr_t;
+
+static int csr_of(int64_t count, int S, const double *Ax, const int32_t *Aj, csr_t *c)
+{
+    int64_t nnz = 0;
+    for (int64_t r = 0; r < count; r++)
+        for (int s = 0; s < S; s++) nnz += Ax[(int64_t)s * count + r] != 0;
+    c->start = malloc(sizeof(int64_t) * (size_t)(count + 1));
+    c->val = malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    c->col = malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    if (!c->start || !c->val || !c->col) return OR_ENOMEM;
+    int64_t k = 0;
+    for (int64_t r = 0; r < count; r++) {
+        c->start[r] = k;
+        for (int s = 0; s < S; s++) {
+            double a = Ax[(int64_t)s * count + r];
+            if (a != 0) {
+                c->val[k] = a;
+                c->col[k++] = Aj[(int64_t)s * count + r];
+            }
+        }
+    }
+    c->start[count] = k;
+    return OR_OK;
+}
+
+static void csr_free(csr_t *c)
+{
+    free(c->start);
+    free(c->val);
+    free(c->col);
+}
+
+/* y = H x - c y, row by row.  Per element the same summation order as or_hubbard_apply_rows:
+ * UP slots in order (zeros skipped, as there), DN slots in order (the zero slots that
+ * or_hubbard_apply_rows adds contribute exact zeros and are skipped here), then U.  The UP
+ * sums run with the element index innermost (a row of x at a time), which keeps each
+ * element's order. */
+static void apply_minus_rows(const or_op *op, const csr_t *cu, const csr_t *cd, const double *x, double *y, double c)
 {
     const int64_t du = op->dim_up, dd = op->dim_dn;
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t u = 0; u < du; u++) {
-        for (int64_t d = 0; d < dd; d++) {
-            double sum = 0.0;
-            for (int s = 0; s < op->S_up; s++) {
-                double a = op->AxUP[(int64_t)s * du + u];
-                if (a != 0) sum += a * x[(int64_t)op->AjUP[(int64_t)s * du + u] * dd + d];
+#pragma omp parallel
+    {
+        double *t = malloc(sizeof(double) * (size_t)dd);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t u = 0; u < du; u++) {
+            for (int64_t d = 0; d < dd; d++) t[d] = 0.0;
+            for (int64_t k = cu->start[u]; k < cu->start[u + 1]; k++) {
+                const double a = cu->val[k];
+                const double *xr = x + (int64_t)cu->col[k] * dd;
+                for (int64_t d = 0; d < dd; d++) t[d] += a * xr[d];
             }
-            for (int s = 0; s < op->S_dn; s++) {
-                double a = op->AxDN[(int64_t)s * dd + d];
-                int64_t col = op->AjDN[(int64_t)s * dd + d];
-                sum += a * x[u * dd + col];
+            const double *xu = x + u * dd;
+            for (int64_t d = 0; d < dd; d++) {
+                double sum = t[d];
+                for (int64_t k = cd->start[d]; k < cd->start[d + 1]; k++) sum += cd->val[k] * xu[cd->col[k]];
+                if (op->U != 0 || op->Uvec) sum += diag_u(op->stateUP[u], op->stateDN[d], op->U, op->mask, op->Uvec) * xu[d];
+                y[u * dd + d] = (c != 0.0) ? -c * y[u * dd + d] + sum : sum;
             }
-            if (op->U != 0 || op->Uvec)
-                sum += diag_u(op->stateUP[u], op->stateDN[d], op->U, op->mask, op->Uvec) * x[u * dd + d];
-            y[u * dd + d] = (c != 0.0) ? -c * y[u * dd + d] + sum : sum;
         }
+        free(t);
     }
 }
 
@@ -692,7 +738,9 @@ int or_lanczos_lowest_2vec(const or_op *op, int maxN, double tolconv, uint32_t s
     double *P = malloc(sizeof(double) * (size_t)n), *V = malloc(sizeof(double) * (size_t)n);
     double *z = malloc(sizeof(double) * (size_t)(maxN + 1)), anorm = 0.0;
     int N, status = OR_ENOCONV;
-    if (!P || !V || !z) {
+    csr_t cu, cd;
+    if (!P || !V || !z || csr_of(op->dim_up, op->S_up, op->AxUP, op->AjUP, &cu) ||
+        csr_of(op->dim_dn, op->S_dn, op->AxDN, op->AjDN, &cd)) {
         free(P);
         free(V);
         free(z);
@@ -706,7 +754,7 @@ int or_lanczos_lowest_2vec(const or_op *op, int maxN, double tolconv, uint32_t s
         double inv = 1.0 / b[N];
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) V[i] = inv * V[i]; /* v1 = f * (1/b) */
-        apply_minus_rows(op, V, P, N != 0 ? b[N] : 0.0); /* P: v0 -> f = H v1 - b v0 */
+        apply_minus_rows(op, &cu, &cd, V, P, N != 0 ? b[N] : 0.0); /* P: v0 -> f = H v1 - b v0 */
         a[N] = dot_r(n, V, P);
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) P[i] = -a[N] * V[i] + P[i];
@@ -733,6 +781,8 @@ int or_lanczos_lowest_2vec(const or_op *op, int maxN, double tolconv, uint32_t s
     free(P);
     free(V);
     free(z);
+    csr_free(&cu);
+    csr_free(&cd);
     return status;
 }
 
