@@ -103,10 +103,10 @@ __global__ __launch_bounds__(256) void k_b_dn(const BInst *__restrict__ in, int 
     double r;
     // ypart (the UP partial) is also the v output; y holds v_{N-1} on entry (w)
     if (step == 0)
-        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_DOT>(I.a, T - I.t2_off, I.ncb2, I.has_up != 0, xv, yp,
+        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_DOT, true>(I.a, T - I.t2_off, I.ncb2, I.has_up != 0, xv, yp,
                                                                  yy, yp, yy, sc, 0.0, s_red);
     else
-        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_AXPY | HV_DOT>(I.a, T - I.t2_off, I.ncb2,
+        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_AXPY | HV_DOT, true>(I.a, T - I.t2_off, I.ncb2,
                                                                            I.has_up != 0, xv, yp, yy, yp, yy, sc,
                                                                            -bN, s_red);
     if (threadIdx.x == 0) partial[I.p_off + (T - I.t2_off)] = r;

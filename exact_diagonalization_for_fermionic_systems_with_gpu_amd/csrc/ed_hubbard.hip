@@ -735,6 +735,9 @@ extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *
     if (const char *e = getenv("ED_HV_ALGO")) h->algo = atoi(e) == 1 ? 1 : 2;
     if (const char *e = getenv("ED_HV_W")) h->hv_w = atoi(e);
     if (const char *e = getenv("ED_HV_RMAX")) h->hv_rmax = atoi(e);
+    if (const char *e = getenv("ED_HV_DNMASK")) h->hv_dnmask = atoi(e) != 0;
+    if (const char *e = getenv("ED_HV_DN2")) h->hv_dn2 = atoi(e) != 0;
+    if (const char *e = getenv("ED_HV_DNK")) h->hv_dnk = atoi(e);
     // pass 1 keeps 256/W rows of UP lists in LDS: rows with very many slots use the
     // single-pass kernel instead (its LDS holds one row's list)
     if ((size_t)32 * h->up.max_nnz * (2 * sizeof(double) + sizeof(int32_t)) + 256 > 65536) h->algo = 1;
@@ -838,10 +841,10 @@ extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int n
         if (algo == 3 && (nops_up > 64 || nops_dn > 64)) algo = 2;
         if (algo < 1 || algo > 3) algo = 2;
         h->algo = algo;
-        // on-the-fly pass 1 reads 64-column slabs (one 512 B row segment per wavefront load);
-        // narrower slabs re-walk every row of x once per slab (rocprofv3, 18-site H.v:
-        // W=8 82 ms, W=16 ~55 ms, W=64 37 ms for the UP pass)
-        if (algo == 3 && !getenv("ED_HV_W")) h->hv_w = 64;
+        // on-the-fly pass 1 reads 128-column slabs (one 1 KB row segment per wavefront load,
+        // 16 B per lane); narrower slabs re-walk every row of x once per slab (18-site H.v, UP
+        // pass: W=8 82 ms, W=16 ~55 ms, W=64 37 ms, W=128 ~33 ms)
+        if (algo == 3 && !getenv("ED_HV_W")) h->hv_w = 128;
     } while (0);
     cleanup();
     if (s) {

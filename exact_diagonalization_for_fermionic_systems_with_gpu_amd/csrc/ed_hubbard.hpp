@@ -83,6 +83,9 @@ struct Hubbard {
     int32_t *d_addr_up = nullptr, *d_addr_dn = nullptr;
     int hv_w = 8;    // pass-1 slab width (doubles)
     int hv_rmax = 8; // largest pass-2 row panel
+    int hv_dnmask = 1; // pass 2 loads only for calls with an element in the column
+    int hv_dn2 = 1;    // pass 2 with per-column descriptors cached in LDS over K row panels
+    int hv_dnk = 2;
     void release();
 };
 

@@ -32,12 +32,12 @@
 
 namespace ed {
 
-template <typename V, typename TV, bool SCALE, int OW>
+template <typename V, typename TV, bool SCALE, int TPR, int VEC>
 __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t nrp,
                                                 const V *__restrict__ x, V *out, const double *__restrict__ bdev,
                                                 int step)
 {
-    constexpr int ORB = 256 / OW;
+    constexpr int ORB = 256 / TPR;
     extern __shared__ double smem_o[];
     const int nops = p.up.nops;
     TV *s_val = reinterpret_cast<TV *>(smem_o);                       // [ORB][nops]
@@ -47,10 +47,10 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
     const int64_t t = (b & 7) * per_xcd + (b >> 3);
     if (t >= ntiles) return; // uniform over the workgroup
     const double sc = SCALE ? 1.0 / bdev[step] : 1.0;
-    otf_up_body<V, TV, SCALE, OW>(p, t, nrp, x, out, sc, s_val, s_col, s_cnt);
+    otf_up_body<V, TV, SCALE, TPR, VEC>(p, t, nrp, x, out, sc, s_val, s_col, s_cnt);
 }
 
-template <typename V, typename TV, int R, int FLAGS>
+template <typename V, typename TV, int R, int FLAGS, bool MASKED>
 __global__ __launch_bounds__(256) void k_otf_dn(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t ncb, bool has_up,
                                                 const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
                                                 const double *__restrict__ bdev, int step,
@@ -62,12 +62,126 @@ __global__ __launch_bounds__(256) void k_otf_dn(OtfArgs p, int64_t ntiles, int64
     if (t >= ntiles) return; // uniform over the workgroup
     const double sc = (FLAGS & HV_SCALE) ? 1.0 / bdev[step] : 1.0;
     const double c = (FLAGS & HV_AXPY) ? -bdev[step] : 0.0;
-    const double r = otf_dn_body<V, TV, R, FLAGS>(p, t, ncb, has_up, x, ypart, y, vout, w, sc, c, s_red);
+    const double r = otf_dn_body<V, TV, R, FLAGS, MASKED>(p, t, ncb, has_up, x, ypart, y, vout, w, sc, c, s_red);
     if ((FLAGS & HV_DOT) && threadIdx.x == 0) partial[t] = r;
 }
 
+// Pass 2, descriptor-cached form: a workgroup owns 256 columns and a group of K row panels
+// of R rows.  Each lane first writes its column's element of every call into LDS as
+// (column << 8 | code), code = 4 * call + {+v_f, -v_f, +v_b, -v_b} (all ones: no element),
+// with the values in a 4 * nops table; then it walks the K panels, gathering only active hops.  The descriptors
+// are built once per K*R rows while a panel stays R rows tall, so the rows a workgroup has
+// in flight (and the XCD's L2 working set) stay small.  Needs dim_dn < 2^24, nops <= 64.
+template <typename V, typename TV, int R, int K, int FLAGS>
+__global__ __launch_bounds__(256) void k_otf_dn2(OtfArgs p, int64_t ntiles, int64_t per_xcd, int64_t ncb, bool has_up,
+                                                 const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
+                                                 const double *__restrict__ bdev, int step,
+                                                 double *__restrict__ partial)
+{
+    extern __shared__ double smem_d[];
+    __shared__ double s_red[4];
+    const int nops = p.dn.nops;
+    TV *s_tab = reinterpret_cast<TV *>(smem_d);                      // [4 * nops]
+    uint32_t *s_desc = reinterpret_cast<uint32_t *>(s_tab + 4 * nops); // [nops][256]
+    const int64_t b = blockIdx.x;
+    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const double sc = (FLAGS & HV_SCALE) ? 1.0 / bdev[step] : 1.0;
+    const double c = (FLAGS & HV_AXPY) ? -bdev[step] : 0.0;
+    const int64_t grp = t / ncb, cb = t - grp * ncb;
+    const int64_t dim_dn = p.dim_dn;
+    const int64_t d = cb * 256 + threadIdx.x;
+    for (int q = threadIdx.x; q < 4 * nops; q += 256) {
+        const OtfOp op = p.dn.ops[q >> 2];
+        const int k = q & 3;
+        const TV v = (k < 2) ? tv_make<TV>(op.vf_re, op.vf_im) : tv_make<TV>(op.vb_re, op.vb_im);
+        s_tab[q] = (k & 1) ? tv_neg<TV>(v) : v;
+    }
+    uint32_t sd = 0;
+    if (d < dim_dn) sd = (uint32_t)p.state_dn[d];
+    for (int o = 0; o < nops; ++o) {
+        uint32_t desc = 0xffffffffu; // no element of call o in this column
+        if (d < dim_dn) {
+            const OtfOp op = p.dn.ops[o];
+            const uint32_t oi = (sd >> op.i) & 1u, oj = (sd >> op.j) & 1u;
+            if (op.kind == ED_OP_DENSITY) {
+                if (oi) desc = ((uint32_t)d << 8) | (uint32_t)(4 * o);
+            } else if (op.kind == ED_OP_DENSITY2) {
+                if (oi & oj) desc = ((uint32_t)d << 8) | (uint32_t)(4 * o);
+            } else if (oi != oj) {
+                const uint32_t code = (oi ? 0u : 2u) | (uint32_t)(__popc(sd & op.between) & 1);
+                desc = ((uint32_t)p.dn.addr[sd ^ op.m] << 8) | (uint32_t)(4 * o) | code;
+            }
+        }
+        s_desc[o * 256 + threadIdx.x] = desc;
+    }
+    __syncthreads();
+    double dot = 0.0;
+    for (int pn = 0; pn < K; ++pn) {
+        const int64_t u0 = (grp * K + pn) * R;
+        if (u0 >= p.dim_up) break; // uniform
+        if (d < dim_dn) {
+            V acc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t u = u0 + r;
+                acc[r] = (has_up && u < p.dim_up) ? ypart[u * dim_dn + d] : VO<V>::zero();
+            }
+            // calls in order, uniform over the wave (lanes gather for the same call, whose
+            // targets are nearly contiguous); lanes without an element of the call do not load
+            constexpr int KB = 4;
+            for (int o0 = 0; o0 < nops; o0 += KB) {
+                uint32_t dk[KB];
+                V xv[KB][R];
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    dk[k] = (o0 + k < nops) ? s_desc[(o0 + k) * 256 + threadIdx.x] : 0xffffffffu;
+                    if (dk[k] != 0xffffffffu) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int64_t u = (u0 + r < p.dim_up) ? u0 + r : u0;
+                            xv[k][r] = x[u * dim_dn + (dk[k] >> 8)];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    if (dk[k] != 0xffffffffu) {
+                        const TV a = s_tab[dk[k] & 0xffu];
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            V xk = xv[k][r];
+                            if (FLAGS & HV_SCALE) xk = VO<V>::scale(sc, xk);
+                            acc[r] = omac(a, xk, acc[r]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t u = u0 + r;
+                if (u >= p.dim_up) continue;
+                const int64_t i = u * dim_dn + d;
+                V xd = x[i];
+                if (FLAGS & HV_SCALE) xd = VO<V>::scale(sc, xd);
+                if (p.has_diag) acc[r] = VO<V>::fmr(diag_of(p.state_up[u], (int32_t)sd, p), xd, acc[r]);
+                V o = acc[r];
+                if (FLAGS & HV_AXPY) o = VO<V>::fmr(c, w[i], o);
+                y[i] = o;
+                if (FLAGS & HV_WRITEV) vout[i] = xd;
+                if (FLAGS & HV_DOT) dot += VO<V>::dotre(xd, o);
+            }
+        }
+    }
+    if (FLAGS & HV_DOT) {
+        const double r = block_sum<256>(dot, s_red);
+        if (threadIdx.x == 0) partial[t] = r;
+    }
+}
+
 struct Plan3 {
-    int R, W;
+    int R, W, TPR, VEC, K;
+    bool dn2;
     int64_t nslab, nrp1, tiles1, per1, ncb, nrp2, tiles2, per2;
 };
 
@@ -75,9 +189,14 @@ static Plan3 plan3(const Hubbard &h)
 {
     Plan3 P;
     const int64_t du = h.args.dim_up, dd = h.args.dim_dn;
-    P.W = (h.hv_w == 16 || h.hv_w == 32 || h.hv_w == 64) ? h.hv_w : 8;
+    // slab width W = TPR threads per row x VEC columns per thread; VEC = 2 (16 B loads per lane)
+    // needs an even dim_dn so that every row chunk stays 16 B aligned
+    const int w = h.hv_w;
+    P.TPR = (w == 16 || w == 32 || w == 64) ? w : (w == 128 ? 64 : (w == 256 ? 128 : (w == 512 ? 256 : 8)));
+    P.VEC = (w >= 128 && dd % 2 == 0) ? 2 : 1;
+    P.W = P.TPR * P.VEC;
     P.nslab = (dd + P.W - 1) / P.W;
-    P.nrp1 = (du + 256 / P.W - 1) / (256 / P.W);
+    P.nrp1 = (du + 256 / P.TPR - 1) / (256 / P.TPR);
     P.tiles1 = P.nslab * P.nrp1;
     P.per1 = (P.tiles1 + 7) / 8;
     P.ncb = (dd + 255) / 256;
@@ -88,7 +207,16 @@ static Plan3 plan3(const Hubbard &h)
             break;
         }
     }
-    P.nrp2 = (du + P.R - 1) / P.R;
+    // descriptor-cached pass 2: panels of 4 rows, 16 panels per workgroup
+    const size_t tv = h.cplx_tables ? 16 : 8;
+    P.dn2 = h.hv_dn2 && dd < (1 << 24) && h.otf.dn.nops <= 64 &&
+            4 * h.otf.dn.nops * tv + (size_t)h.otf.dn.nops * 256 * 4 <= 65536;
+    P.K = 1;
+    if (P.dn2) {
+        P.R = (h.hv_rmax >= 8) ? 8 : 4;
+        P.K = (h.hv_dnk == 2 || h.hv_dnk == 4 || h.hv_dnk == 16) ? h.hv_dnk : 8;
+    }
+    P.nrp2 = P.dn2 ? (du + P.R * P.K - 1) / (P.R * P.K) : (du + P.R - 1) / P.R;
     P.tiles2 = P.nrp2 * P.ncb;
     P.per2 = (P.tiles2 + 7) / 8;
     return P;
@@ -106,47 +234,73 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
     V *upbuf = (FLAGS & HV_WRITEV) ? vout : y;
     constexpr bool SC = (FLAGS & HV_SCALE) != 0;
     if (has_up) {
-        const size_t sh = (size_t)(256 / P.W) * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
+        const size_t sh = (size_t)(256 / P.TPR) * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
         const dim3 g1((unsigned)(P.per1 * 8));
-        switch (P.W) {
-        case 64:
-            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 64>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
-                               upbuf, bdev, step);
-            break;
-        case 32:
-            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 32>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
-                               upbuf, bdev, step);
-            break;
-        case 16:
-            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 16>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
-                               upbuf, bdev, step);
-            break;
-        default:
-            hipLaunchKernelGGL((k_otf_up<V, TV, SC, 8>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, x,
-                               upbuf, bdev, step);
-            break;
+#define ED_UP(TPR_, VEC_)                                                                                   \
+    hipLaunchKernelGGL((k_otf_up<V, TV, SC, TPR_, VEC_>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, \
+                       x, upbuf, bdev, step)
+        if (P.VEC == 2) {
+            if (P.TPR == 256) ED_UP(256, 2);
+            else if (P.TPR == 128) ED_UP(128, 2);
+            else ED_UP(64, 2);
+        } else {
+            switch (P.TPR) {
+            case 256: ED_UP(256, 1); break;
+            case 128: ED_UP(128, 1); break;
+            case 64: ED_UP(64, 1); break;
+            case 32: ED_UP(32, 1); break;
+            case 16: ED_UP(16, 1); break;
+            default: ED_UP(8, 1); break;
+            }
         }
+#undef ED_UP
         ED_CHECK_LAUNCH();
     }
     dim3 g2((unsigned)(P.per2 * 8)), b2(256);
-    switch (P.R) {
-    case 8:
-        hipLaunchKernelGGL((k_otf_dn<V, TV, 8, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
-                           upbuf, y, vout, w, bdev, step, partial);
-        break;
-    case 4:
-        hipLaunchKernelGGL((k_otf_dn<V, TV, 4, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
-                           upbuf, y, vout, w, bdev, step, partial);
-        break;
-    case 2:
-        hipLaunchKernelGGL((k_otf_dn<V, TV, 2, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
-                           upbuf, y, vout, w, bdev, step, partial);
-        break;
-    default:
-        hipLaunchKernelGGL((k_otf_dn<V, TV, 1, FLAGS>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x,
-                           upbuf, y, vout, w, bdev, step, partial);
-        break;
+    if (P.dn2) {
+        const size_t sh2 = (size_t)4 * h.otf.dn.nops * sizeof(TV) + (size_t)h.otf.dn.nops * 256 * sizeof(uint32_t);
+#define ED_DN2(R_, K_)                                                                                          \
+    hipLaunchKernelGGL((k_otf_dn2<V, TV, R_, K_, FLAGS>), g2, b2, sh2, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x, \
+                       upbuf, y, vout, w, bdev, step, partial)
+        if (P.R == 8) {
+            switch (P.K) {
+            case 2: ED_DN2(8, 2); break;
+            case 4: ED_DN2(8, 4); break;
+            case 16: ED_DN2(8, 16); break;
+            default: ED_DN2(8, 8); break;
+            }
+        } else {
+            switch (P.K) {
+            case 2: ED_DN2(4, 2); break;
+            case 4: ED_DN2(4, 4); break;
+            case 16: ED_DN2(4, 16); break;
+            default: ED_DN2(4, 8); break;
+            }
+        }
+#undef ED_DN2
+        ED_CHECK_LAUNCH();
+        if (nblocks_out) *nblocks_out = P.tiles2;
+        return ED_OK;
     }
+#define ED_DN(R_, M_)                                                                                          \
+    hipLaunchKernelGGL((k_otf_dn<V, TV, R_, FLAGS, M_>), g2, b2, 0, st, h.otf, P.tiles2, P.per2, P.ncb, has_up, x, \
+                       upbuf, y, vout, w, bdev, step, partial)
+    if (h.hv_dnmask) {
+        switch (P.R) {
+        case 8: ED_DN(8, true); break;
+        case 4: ED_DN(4, true); break;
+        case 2: ED_DN(2, true); break;
+        default: ED_DN(1, true); break;
+        }
+    } else {
+        switch (P.R) {
+        case 8: ED_DN(8, false); break;
+        case 4: ED_DN(4, false); break;
+        case 2: ED_DN(2, false); break;
+        default: ED_DN(1, false); break;
+        }
+    }
+#undef ED_DN
     ED_CHECK_LAUNCH();
     if (nblocks_out) *nblocks_out = P.tiles2;
     return ED_OK;

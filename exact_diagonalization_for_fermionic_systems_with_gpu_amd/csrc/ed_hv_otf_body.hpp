@@ -81,19 +81,55 @@ __device__ __forceinline__ bool op_entry(const OtfOp &op, uint32_t s, int32_t se
 
 
 // ---------------------------------------------------------------------------------
-// pass 1 body: UP term of logical tile t (slab-major, OW columns x 256/OW rows)
+// pass 1 body: UP term of logical tile t (slab-major).  A tile is 256/TPR rows x TPR*VEC
+// columns: TPR threads per row, each owning VEC consecutive columns (VEC = 2 loads 16 B per
+// lane for FP64).  Rows are evaluated one per wave (rows r = wave, wave + 4, ...): the lanes
+// take one operator call each, a ballot orders the hopping ones into LDS.
 // ---------------------------------------------------------------------------------
-template <typename V, typename TV, bool SCALE, int OW>
+template <typename V, int VEC>
+struct VecLoad;
+template <typename V>
+struct VecLoad<V, 1> {
+    static __device__ __forceinline__ void load(const V *p, V *o) { o[0] = *p; }
+    static __device__ __forceinline__ void store(V *p, const V *o) { *p = o[0]; }
+};
+template <>
+struct VecLoad<double, 2> {
+    static __device__ __forceinline__ void load(const double *p, double *o)
+    {
+        const double2 v = *reinterpret_cast<const double2 *>(p);
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+    static __device__ __forceinline__ void store(double *p, const double *o)
+    {
+        *reinterpret_cast<double2 *>(p) = make_double2(o[0], o[1]);
+    }
+};
+template <>
+struct VecLoad<cplx, 2> {
+    static __device__ __forceinline__ void load(const cplx *p, cplx *o)
+    {
+        o[0] = p[0];
+        o[1] = p[1];
+    }
+    static __device__ __forceinline__ void store(cplx *p, const cplx *o)
+    {
+        p[0] = o[0];
+        p[1] = o[1];
+    }
+};
+
+template <typename V, typename TV, bool SCALE, int TPR, int VEC = 1>
 __device__ __forceinline__ void otf_up_body(const OtfArgs &p, int64_t t, int64_t nrp, const V *__restrict__ x, V *out,
                                             double sc, TV *s_val, int32_t *s_col, int *s_cnt)
 {
-    constexpr int RW = 64 / OW;
+    constexpr int ORB = 256 / TPR, OW = TPR * VEC;
     const int nops = p.up.nops;
     const int64_t slab = t / nrp, rp = t - slab * nrp;
-    const int64_t u0 = rp * (256 / OW);
+    const int64_t u0 = rp * ORB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int rr = 0; rr < RW; ++rr) {
-        const int r = wave * RW + rr;
+    for (int r = wave; r < ORB; r += 4) {
         const int64_t u = u0 + r;
         bool has = false;
         int32_t col = 0;
@@ -111,28 +147,54 @@ __device__ __forceinline__ void otf_up_body(const OtfArgs &p, int64_t t, int64_t
         if (lane == 0) s_cnt[r] = __popcll(bal);
     }
     __syncthreads();
-    const int r = wave * RW + lane / OW, c = lane % OW;
+    const int r = threadIdx.x / TPR, c = (threadIdx.x % TPR) * VEC;
     const int64_t u = u0 + r, d = slab * OW + c;
     if (u >= p.dim_up || d >= p.dim_dn) return;
-    V acc = VO<V>::zero();
     const int n = s_cnt[r];
     const int32_t *cl = s_col + r * nops;
     const TV *vl = s_val + r * nops;
-    // loads in batches of 8 so several are in flight per lane; the FMAs stay in call order
+    if (VEC == 1 || d + VEC > p.dim_dn) { // scalar columns (also the ragged last slab)
+        for (int q = 0; q < VEC && d + q < p.dim_dn; ++q) {
+            V acc = VO<V>::zero();
+            for (int j0 = 0; j0 < n; j0 += 8) {
+                V xv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (j0 + k < n) xv[k] = x[(int64_t)cl[j0 + k] * p.dim_dn + d + q];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (j0 + k < n) {
+                        V xk = xv[k];
+                        if (SCALE) xk = VO<V>::scale(sc, xk);
+                        acc = omac(vl[j0 + k], xk, acc);
+                    }
+            }
+            out[u * p.dim_dn + d + q] = acc;
+        }
+        return;
+    }
+    // VEC consecutive columns per lane, loaded together (the launcher only picks VEC = 2 when
+    // dim_dn is even, so every row chunk is 16 B aligned); the FMAs stay in call order
+    V acc[VEC];
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) acc[q] = VO<V>::zero();
     for (int j0 = 0; j0 < n; j0 += 8) {
-        V xv[8];
+        V xv[8][VEC];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (j0 + k < n) xv[k] = x[(int64_t)cl[j0 + k] * p.dim_dn + d];
+            if (j0 + k < n) VecLoad<V, VEC>::load(x + (int64_t)cl[j0 + k] * p.dim_dn + d, xv[k]);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             if (j0 + k < n) {
-                V xk = xv[k];
-                if (SCALE) xk = VO<V>::scale(sc, xk);
-                acc = omac(vl[j0 + k], xk, acc);
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) {
+                    V xk = xv[k][q];
+                    if (SCALE) xk = VO<V>::scale(sc, xk);
+                    acc[q] = omac(vl[j0 + k], xk, acc[q]);
+                }
             }
     }
-    out[u * p.dim_dn + d] = acc;
+    VecLoad<V, VEC>::store(out + u * p.dim_dn + d, acc);
 }
 
 __device__ __forceinline__ double diag_of(int32_t su, int32_t sd, const OtfArgs &p)
@@ -153,7 +215,7 @@ __device__ __forceinline__ double diag_of(int32_t su, int32_t sd, const OtfArgs 
 // pass 2 body: DN term + diagonal + epilogue of logical tile t (R rows x 256 columns);
 // returns the block's Re<v,y> partial (valid in thread 0) when HV_DOT is set
 // ---------------------------------------------------------------------------------
-template <typename V, typename TV, int R, int FLAGS>
+template <typename V, typename TV, int R, int FLAGS, bool MASKED = false>
 __device__ __forceinline__ double otf_dn_body(const OtfArgs &p, int64_t t, int64_t ncb, bool has_up,
                                               const V *__restrict__ x, const V *ypart, V *y, V *vout, const V *w,
                                               double sc, double c, double *s_red)
@@ -178,20 +240,50 @@ __device__ __forceinline__ double otf_dn_body(const OtfArgs &p, int64_t t, int64
         for (int o0 = 0; o0 < nops; o0 += KB) {
             int32_t col[KB];
             TV a[KB];
+            bool act[KB];
 #pragma unroll
             for (int k = 0; k < KB; ++k) {
                 col[k] = (int32_t)d;
                 a[k] = tv_make<TV>(0.0, 0.0);
+                act[k] = false;
                 if (o0 + k < nops) {
                     int32_t ck;
                     TV ak;
                     if (op_entry<TV>(p.dn.ops[o0 + k], sd, (int32_t)d, p.dn.addr, ck, ak)) {
                         col[k] = ck;
                         a[k] = ak;
+                        act[k] = true;
                     }
                 }
             }
             V xv[KB][R];
+            if (MASKED) {
+                // only calls with an element in this column load; a call no lane of the wave
+                // needs issues no load at all (the exact zeros of the reference's empty slots
+                // are not added)
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    if (act[k]) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int64_t u = (u0 + r < p.dim_up) ? u0 + r : u0;
+                            xv[k][r] = x[u * dim_dn + col[k]];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    if (act[k]) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            V xk = xv[k][r];
+                            if (FLAGS & HV_SCALE) xk = VO<V>::scale(sc, xk);
+                            acc[r] = omac(a[k], xk, acc[r]);
+                        }
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < KB; ++k)
 #pragma unroll

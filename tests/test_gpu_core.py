@@ -113,17 +113,26 @@ def _hv_cases():
     }
 
 
-@pytest.mark.parametrize("path", ["otf", "tables", "rowkernel"])
+_OTF_ENV = {"otf": {}, "otf_w16": {"ED_HV_W": "16"}, "otf_w128": {"ED_HV_W": "128"},
+            "otf_w256": {"ED_HV_W": "256"}, "otf_w512": {"ED_HV_W": "512"},
+            "otf_nomask_r4": {"ED_HV_DNMASK": "0", "ED_HV_RMAX": "4"}, "otf_r1": {"ED_HV_RMAX": "1"}}
+
+
+@pytest.mark.parametrize("path", list(_OTF_ENV) + ["tables", "rowkernel"])
 @pytest.mark.parametrize("name", list(_hv_cases().keys()))
 def test_hv_parity(name, path, monkeypatch):
-    """otf: descriptors generated from the call list (default); tables: two-pass kernels on the
-    ELL tables; rowkernel: the single-pass row kernel (ED_HV_ALGO=1)."""
+    """otf*: descriptors generated from the call list (default W = 64 columns per UP slab, masked
+    DN loads; the variants change the slab width -- 128 and more use 16 B loads per lane -- the
+    DN row panel and the masking); tables: two-pass kernels on the ELL tables; rowkernel: the
+    single-pass row kernel (ED_HV_ALGO=1)."""
     N, Qu, Qd, ops, U, mask, Uvec, cplx = _hv_cases()[name]
     print(f"hv {name} {path}", flush=True)
     gu, gd = gpu_species(N, Qu, ops, cplx), gpu_species(N, Qd, ops, cplx)
     if path == "rowkernel":
         monkeypatch.setenv("ED_HV_ALGO", "1")
-    H = Hubbard(gu, gd, U, dilution=mask, Uvec=Uvec, from_tables=(path != "otf"))
+    for k, v in _OTF_ENV.get(path, {}).items():
+        monkeypatch.setenv(k, v)
+    H = Hubbard(gu, gd, U, dilution=mask, Uvec=Uvec, from_tables=not path.startswith("otf"))
     ou, od = O.Species(N, Qu, ops, cplx), O.Species(N, Qd, ops, cplx)
     OH = O.Hubbard(ou, od, U, mask=mask, Uvec=Uvec)
     rng = np.random.default_rng(1)
