@@ -66,6 +66,32 @@ def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
     return 16.0 * elems / dt2 / 1e9, rows2, dt2, O.num_threads()
 
 
+def ground_state_line(H, hq, x, y, N, Q):
+    """Lanczos ground state (reference convergence test) of the benchmark Hamiltonian from the
+    seed-235 start, Ritz vector rebuilt by recomputation, then S_vN of sites 0..N/2-1."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import Lanczos
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
+    lz = Lanczos(hq.count, hq.count)
+    lz.random_vector(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = lz.eigen(H, x, y)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"[bench] ground state E0 {e0:.15f} in {lz.nsteps} steps, {t1 - t0:.1f} s")
+    s = np.zeros(1)
+    call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(y), s.ctypes.data_as(C.POINTER(C.c_double)),
+         stream_ptr())
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f"[bench] S_vN({N // 2}|{N - N // 2}) {s[0]:.15f} in {t2 - t1:.1f} s")
+    return {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "S_vN": float(s[0]),
+            "entropy_s": round(t2 - t1, 2)}
+
+
 def batched_drude_line():
     """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
     weight per sector; batched (ed_batch_lowest, hipGraph) vs the plain per-instance loop on the
@@ -109,6 +135,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched Drude sweep and the disorder-ensemble lines")
     ap.add_argument("--ensemble", type=int, default=64, help="realisations of the 14-site chain ensemble")
+    ap.add_argument("--ground-state", action="store_true",
+                    help="also run the Lanczos ground state of the same workload (Ritz vector rebuilt) and its "
+                         "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run one device copy of a vector (16 D bytes) as the byte-count reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -198,6 +227,8 @@ def main():
                                "sample": f"oracle H.v (C restatement of Hubbard_mv_), rows {16}..{16 + rows} "
                                          f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
         del x_host
+    if args.ground_state:
+        out["ground_state"] = ground_state_line(H, hq, x, y, N, Q)
     if not args.no_extras:
         del x, y
         torch.cuda.empty_cache()
