@@ -4,18 +4,25 @@
 // MBL_test/include/HubMatrix_GPU.cuh:561-613, called once per sample/sector by the drivers,
 // e.g. drude_v2_cal_Kagome_randomHop.cu:129-175).  For sweeps over uneven sectors, twist
 // fluxes or disorder realisations that leaves the GPU mostly idle on the small instances and
-// pays a host round trip per check on every one.  Here all instances advance together:
-//   * their vectors are packed into three shared buffers (the rotation A/B/C of the
-//     single-instance driver), each instance at its own offset;
-//   * one launch of each kernel covers every tile of every active instance (a tile list
-//     with per-instance offsets; instances that have converged skip their tiles);
-//   * reductions stay per instance and deterministic (per-tile partials, one finalize
-//     block per instance);
+// pays a host round trip per check on every one.  Here all instances advance together.
+//
+// Groups.  Instances whose operators have the same structure -- same sector, the same
+// H_TBq calls (kind, i, j) in the same order, the same U, dilution mask and per-site U -- and
+// differ only in call values (hopping amplitudes, twist phases, on-site potentials) form a
+// group of up to 16 copies.  A group's vectors are interleaved, psi[(i_up*dim_dn + i_dn)*G + k]
+// for copy k, so one hop descriptor (neighbour rank, sign, call) serves all copies and every
+// gather reads G consecutive amplitudes; the call values sit in a per-copy table.  Instances
+// that share nothing form groups of one.
+//
+//   * one launch of each kernel covers every tile of every group (tile lists with per-group
+//     offsets, binary search for the owner);
+//   * each copy keeps its own Lanczos scalars, active flag and deterministic reductions
+//     (per-tile, per-copy partials; one finalize block per copy);
 //   * the step number and the active flags live in device memory, so ten steps (the
 //     reference's convergence-check cadence) are captured once in a hipGraph per buffer
-//     rotation phase and replayed; the host only reads a/b at the checks.
-// Each instance follows exactly the recurrence and convergence test of ed_lanczos_eigen
-// (eigenvalue only; no Ritz vector).
+//     rotation phase and replayed; the host reads a/b only at the checks.
+// Each copy follows the recurrence and convergence test of ed_lanczos_eigen (eigenvalue only),
+// with its own instance's start vector (GenRand of the sector length).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -28,154 +35,332 @@
 
 namespace ed {
 
-constexpr int BW = 16; // pass-1 slab width in the batch (small sectors have narrow rows)
-constexpr int BR = 4;  // pass-2 rows per tile
-constexpr int BCH = 2048; // reals per update chunk
+constexpr int GMAX = 16;  // copies per group
+constexpr int GRMAX = 8;  // DN rows per tile: 4 or 8 (Batch::gr)
+constexpr int GUMAX = 4;  // UP rows per tile: 1 or 4 (Batch::gu), one wave per row's calls
+constexpr int GCH = 8;    // elements per thread per update chunk
 
-struct BInst {
-    OtfArgs a;
-    int64_t voff;          // offset of the instance's vector in doubles
-    int64_t nreal;         // doubles in the instance's vector
-    int64_t t1_off, t1_n;  // pass-1 tiles
-    int64_t t2_off, t2_n;  // pass-2 tiles
-    int64_t c_off, c_n;    // update chunks
-    int64_t nrp1, ncb2;
-    int64_t p_off;         // offset in the partial buffer (max(t2_n, c_n) entries)
-    int32_t has_up, pad;
+struct GGroup {
+    OtfArgs a;               // structure (calls of copy 0), dims, basis, U / mask / per-site U
+    const double *vt_up;     // [nops_up][G] x {vf, vb}, TV each (real or complex)
+    const double *vt_dn;
+    int G, lg, ncopy, cbase; // copies (power of 2), log2 G, real copies, first global copy
+    int CW, has_up;          // columns per tile = 256 / G
+    int64_t voff;            // group vector offset in doubles
+    int64_t nelem;           // dim_up * dim_dn
+    int64_t ncb;             // column blocks per row
+    int64_t t1_off, t1_n;    // UP tiles (GU rows x CW columns, slab-major)
+    int64_t t2_off, t2_n;    // DN tiles (GR rows x CW columns)
+    int64_t c_off, c_n;      // update chunks (GCH * CW elements)
+    int64_t p_off;           // partials: max(t2_n, c_n) * G
 };
 
-// instance owning global tile T: last i with off[i] <= T (offsets non-decreasing)
-template <int FIELD>
-__device__ __forceinline__ int find_inst(const BInst *__restrict__ in, int n, int64_t T)
+
+
+// value of descriptor code (4 * call + {+v_f, -v_f, +v_b, -v_b}) for copy k, from the
+// group's value table staged in LDS
+template <typename TV>
+__device__ __forceinline__ TV gval(const TV *t, int G, uint32_t code, int k)
 {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int64_t o = FIELD == 1 ? in[mid].t1_off : FIELD == 2 ? in[mid].t2_off : in[mid].c_off;
-        if (o <= T) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    const TV v = t[((code >> 2) * G + k) * 2 + ((code >> 1) & 1)];
+    return (code & 1) ? tv_neg<TV>(v) : v;
 }
 
-template <typename V, typename TV>
-__global__ __launch_bounds__(256) void k_b_up(const BInst *__restrict__ in, int n, int64_t ntiles, int64_t per_xcd,
-                                              int maxnops, const int *__restrict__ active,
-                                              const double *__restrict__ bdev, int ldb,
-                                              const int *__restrict__ dstep, const double *x, double *out)
+// copy a group's value table ([nops][G] x {vf, vb}) into LDS
+template <typename TV>
+__device__ __forceinline__ void stage_table(const double *src, int nops, int G, TV *dst)
 {
-    constexpr int ORB = 256 / BW;
-    extern __shared__ double smem_b[];
-    TV *s_val = reinterpret_cast<TV *>(smem_b);
-    int32_t *s_col = reinterpret_cast<int32_t *>(s_val + ORB * maxnops);
-    __shared__ int s_cnt[ORB];
-    const int64_t b = blockIdx.x;
-    const int64_t T = (b & 7) * per_xcd + (b >> 3);
-    if (T >= ntiles) return;
-    const int i = find_inst<1>(in, n, T);
-    if (!active[i] || !in[i].has_up) return;
-    const BInst &I = in[i];
-    const int step = *dstep;
-    const double sc = 1.0 / bdev[(int64_t)i * ldb + step];
-    otf_up_body<V, TV, true, BW>(I.a, T - I.t1_off, I.nrp1, reinterpret_cast<const V *>(x + I.voff),
-                                 reinterpret_cast<V *>(out + I.voff), sc, s_val, s_col, s_cnt);
+    const TV *s = reinterpret_cast<const TV *>(src);
+    for (int q = threadIdx.x; q < nops * G * 2; q += 256) dst[q] = s[q];
 }
 
-template <typename V, typename TV>
-__global__ __launch_bounds__(256) void k_b_dn(const BInst *__restrict__ in, int n, int64_t ntiles, int64_t per_xcd,
+// descriptor of call op in the row/column of state s with rank self: (col << 8 | code), or ~0
+__device__ __forceinline__ uint32_t gdesc(const OtfOp &op, int o, uint32_t s, uint32_t self,
+                                          const int32_t *__restrict__ addr)
+{
+    const uint32_t oi = (s >> op.i) & 1u, oj = (s >> op.j) & 1u;
+    if (op.kind == ED_OP_DENSITY) return oi ? ((self << 8) | (uint32_t)(4 * o)) : 0xffffffffu;
+    if (op.kind == ED_OP_DENSITY2) return (oi & oj) ? ((self << 8) | (uint32_t)(4 * o)) : 0xffffffffu;
+    if (oi == oj) return 0xffffffffu;
+    const uint32_t code = (oi ? 0u : 2u) | (uint32_t)(__popc(s & op.between) & 1);
+    return ((uint32_t)addr[s ^ op.m] << 8) | (uint32_t)(4 * o) | code;
+}
+
+// Pass 1: UP term of GU rows x CW columns x G copies -> y (the partial), x scaled by 1/b_N.
+// Tiles run slab-major (all rows of a CW-column slab, then the next slab), so the rows the
+// hops gather from -- dim_up x CW x G amplitudes, 3.7 MB for the 924 x 924 x 16 sector --
+// stay in the XCD's L2 while the slab is processed.
+template <typename V, typename TV, int GU>
+__global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
                                               const int *__restrict__ active, const double *__restrict__ bdev,
-                                              int ldb, const int *__restrict__ dstep, const double *x, double *ypart,
-                                              double *y, double *partial)
+                                              int ldb, const int *__restrict__ dstep, const double *x, double *y)
 {
-    __shared__ double s_red[4];
+    extern __shared__ double s_upv[]; // value table, TV [nops_up][G][2]
+    __shared__ uint32_t s_desc[GUMAX][64];
+    __shared__ int s_cnt[GUMAX];
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
     if (T >= ntiles) return;
-    const int i = find_inst<2>(in, n, T);
-    if (!active[i]) return;
-    const BInst &I = in[i];
-    const int step = *dstep;
-    const double bN = bdev[(int64_t)i * ldb + step];
-    const double sc = 1.0 / bN;
-    const V *xv = reinterpret_cast<const V *>(x + I.voff);
-    V *yp = reinterpret_cast<V *>(ypart + I.voff);
-    V *yy = reinterpret_cast<V *>(y + I.voff);
-    double r;
-    // ypart (the UP partial) is also the v output; y holds v_{N-1} on entry (w)
-    if (step == 0)
-        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_DOT, true>(I.a, T - I.t2_off, I.ncb2, I.has_up != 0, xv, yp,
-                                                                 yy, yp, yy, sc, 0.0, s_red);
-    else
-        r = otf_dn_body<V, TV, BR, HV_SCALE | HV_WRITEV | HV_AXPY | HV_DOT, true>(I.a, T - I.t2_off, I.ncb2,
-                                                                           I.has_up != 0, xv, yp, yy, yp, yy, sc,
-                                                                           -bN, s_red);
-    if (threadIdx.x == 0) partial[I.p_off + (T - I.t2_off)] = r;
+    const GGroup g = gr[tgrp[T]];
+    if (!g.has_up) return;
+    TV *s_vt = reinterpret_cast<TV *>(s_upv);
+    stage_table<TV>(g.vt_up, g.a.up.nops, g.G, s_vt);
+    const int64_t du = g.a.dim_up, dd = g.a.dim_dn;
+    const int64_t nrp = (du + GU - 1) / GU;
+    const int64_t tl = T - g.t1_off, cb = tl / nrp, u0 = (tl - cb * nrp) * GU;
+    if ((threadIdx.x >> 6) < GU) { // wave r evaluates the calls of row u0 + r
+        const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int64_t u = u0 + r;
+        uint32_t dsc = 0xffffffffu;
+        if (u < du && lane < g.a.up.nops)
+            dsc = gdesc(g.a.up.ops[lane], lane, (uint32_t)g.a.state_up[u], (uint32_t)u, g.a.up.addr);
+        const bool has = dsc != 0xffffffffu;
+        const uint64_t bal = __ballot(has);
+        if (has) s_desc[r][__popcll(bal & ((1ull << lane) - 1ull))] = dsc;
+        if (lane == 0) s_cnt[r] = __popcll(bal);
+    }
+    __syncthreads();
+    const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
+    const int64_t d = cb * g.CW + c;
+    if (k >= g.ncopy || d >= dd) return;
+    const int gi = g.cbase + k;
+    if (!active[gi]) return;
+    const double sc = 1.0 / bdev[(int64_t)gi * ldb + *dstep];
+    const V *xv = reinterpret_cast<const V *>(x + g.voff);
+    V *yv = reinterpret_cast<V *>(y + g.voff);
+    const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k, du32 = (uint32_t)d;
+    for (int r = 0; r < GU; ++r) {
+        const int64_t u = u0 + r;
+        if (u >= du) break;
+        const int n = s_cnt[r];
+        V acc = VO<V>::zero();
+        for (int j0 = 0; j0 < n; j0 += 8) {
+            V xl[8];
+            uint32_t ds[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                ds[q] = (j0 + q < n) ? s_desc[r][j0 + q] : 0u;
+                if (j0 + q < n) xl[q] = xv[(((ds[q] >> 8) * ddu + du32) << lg) + ku];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (j0 + q < n) acc = omac(gval<TV>(s_vt, g.G, ds[q] & 0xffu, k), VO<V>::scale(sc, xl[q]), acc);
+        }
+        yv[(((uint32_t)u * ddu + du32) << lg) + ku] = acc;
+    }
 }
 
-// one block per instance: dst[i*ld + step(+1)] = sum (mode 0) or sqrt(sum) (mode 1) of its
-// partials, in the fixed order of the single-instance finalize
-__global__ __launch_bounds__(1024) void k_b_finalize(const BInst *__restrict__ in, const int *__restrict__ active,
-                                                     const double *__restrict__ partial, int which, double *dst,
-                                                     int ld, const int *dstep)
+// Pass 2 of one step on a tile of GR rows x CW columns x G copies (row-panel order, so the
+// rows the DN hops gather from stay in L2): y = (UP partial in y) + DN term + diagonal
+// - b_N v_{N-1}, with v = x / b_N and v_{N-1} = xp / b_{N-1} recomputed from the previous
+// start vector instead of being stored; per-copy partials of Re<v, y>.  Per element the UP
+// calls, then the DN calls, then the diagonal, as in the single-operator kernels.  Step 0 has
+// no v_{N-1} term.
+template <typename V, typename TV, bool AXPY, int GR>
+__global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
+                                              const int *__restrict__ active, const double *__restrict__ bdev,
+                                              int ldb, const int *__restrict__ dstep, const double *x,
+                                              const double *xp, double *y, double *partial)
 {
-    __shared__ double red[16];
-    const int i = blockIdx.x;
-    const int step = *dstep;
-    if (active[i]) {
-        const BInst &I = in[i];
-        const int64_t np = which == 0 ? I.t2_n : I.c_n;
-        double s = 0.0;
-        for (int64_t k = threadIdx.x; k < np; k += 1024) s += partial[I.p_off + k];
-        s = block_sum<1024>(s, red);
-        if (threadIdx.x == 0) {
-            if (which == 0) dst[(int64_t)i * ld + step] = s;
-            else dst[(int64_t)i * ld + step + 1] = sqrt(s);
+    extern __shared__ double s_dnv[]; // value table TV [nops_dn][G][2], then descriptors [nops_dn][CW]
+    __shared__ double s_red[256];
+    const int64_t b = blockIdx.x;
+    const int64_t T = (b & 7) * per_xcd + (b >> 3);
+    if (T >= ntiles) return;
+    const GGroup g = gr[tgrp[T]];
+    TV *s_vt = reinterpret_cast<TV *>(s_dnv);
+    uint32_t *s_gd = reinterpret_cast<uint32_t *>(s_vt + g.a.dn.nops * g.G * 2);
+    stage_table<TV>(g.vt_dn, g.a.dn.nops, g.G, s_vt);
+    const int64_t tl = T - g.t2_off, rp = tl / g.ncb, cb = tl - rp * g.ncb;
+    const int64_t du = g.a.dim_up, dd = g.a.dim_dn, u0 = rp * GR;
+    const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
+    const int64_t d = cb * g.CW + c;
+    const int nops = g.a.dn.nops;
+    const uint32_t sd = d < dd ? (uint32_t)g.a.state_dn[d] : 0u;
+    for (int o = k; o < nops; o += g.G) // the G lanes of a column share its descriptors
+        s_gd[o * g.CW + c] = d < dd ? gdesc(g.a.dn.ops[o], o, sd, (uint32_t)d, g.a.dn.addr) : 0xffffffffu;
+    __syncthreads();
+    const int gi = g.cbase + k;
+    const bool live = k < g.ncopy && d < dd && active[gi];
+    double dot = 0.0;
+    if (live) {
+        const int step = *dstep;
+        const double bN = bdev[(int64_t)gi * ldb + step], sc = 1.0 / bN;
+        const double scp = AXPY ? 1.0 / bdev[(int64_t)gi * ldb + step - 1] : 0.0;
+        const V *xv = reinterpret_cast<const V *>(x + g.voff);
+        const V *xpv = reinterpret_cast<const V *>(xp + g.voff);
+        V *yy = reinterpret_cast<V *>(y + g.voff);
+        // 32-bit element offsets inside the group (ed_batch_create checks nelem * G < 2^31):
+        // row r's amplitude of column col and copy k sits at ((rowbase[r] + col) << lg) + k
+        const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k;
+        uint32_t rowbase[GR];
+#pragma unroll
+        for (int r = 0; r < GR; ++r) rowbase[r] = (uint32_t)((u0 + r < du) ? u0 + r : u0) * ddu;
+        V acc[GR];
+#pragma unroll
+        for (int r = 0; r < GR; ++r)
+            acc[r] = (g.has_up && u0 + r < du) ? yy[((rowbase[r] + (uint32_t)d) << lg) + ku] : VO<V>::zero();
+        constexpr int KB = 4;
+        for (int o0 = 0; o0 < nops; o0 += KB) {
+            uint32_t dk[KB];
+            V xl[KB][GR];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                dk[q] = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
+                if (dk[q] != 0xffffffffu) {
+                    const uint32_t col = dk[q] >> 8;
+#pragma unroll
+                    for (int r = 0; r < GR; ++r) xl[q][r] = xv[((rowbase[r] + col) << lg) + ku];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                if (dk[q] != 0xffffffffu) {
+                    const TV a = gval<TV>(s_vt, g.G, dk[q] & 0xffu, k);
+#pragma unroll
+                    for (int r = 0; r < GR; ++r) acc[r] = omac(a, VO<V>::scale(sc, xl[q][r]), acc[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < GR; ++r) {
+            const int64_t u = u0 + r;
+            if (u >= du) continue;
+            const uint32_t i = ((rowbase[r] + (uint32_t)d) << lg) + ku;
+            const V xd = VO<V>::scale(sc, xv[i]);
+            if (g.a.has_diag) acc[r] = VO<V>::fmr(diag_of(g.a.state_up[u], (int32_t)sd, g.a), xd, acc[r]);
+            V o = acc[r];
+            if (AXPY) o = VO<V>::fmr(-bN, VO<V>::scale(scp, xpv[i]), o);
+            yy[i] = o;
+            dot += VO<V>::dotre(xd, o);
         }
     }
+    // per-copy block sums in a fixed order: thread (c, k) -> s_red[c * G + k]
+    s_red[threadIdx.x] = dot;
+    __syncthreads();
+    if (threadIdx.x < g.G) {
+        double s = 0.0;
+        for (int cc = 0; cc < g.CW; ++cc) s += s_red[cc * g.G + threadIdx.x];
+        partial[g.p_off + tl * g.G + threadIdx.x] = s;
+    }
 }
 
-// last kernel of a step: the step counter lives in device memory so captured steps replay
-__global__ void k_b_bump(int *dstep) { dstep[0] += 1; }
-
-// f = -a v + f ; partial |f|^2 per 2048-real chunk
-__global__ __launch_bounds__(256) void k_b_update(const BInst *__restrict__ in, int n, int64_t ntiles,
+// f = -a_k v + f with v = x / b_N over GCH * CW elements x G copies; per-copy partials of |f|^2
+__global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles,
                                                   const int *__restrict__ active, const double *__restrict__ adev,
-                                                  int lda, const int *__restrict__ dstep, const double *v, double *f,
-                                                  double *partial)
+                                                  const double *__restrict__ bdev, int lda,
+                                                  const int *__restrict__ dstep, int cvec, const double *x,
+                                                  double *f, double *partial)
 {
-    __shared__ double red[4];
+    __shared__ double s_red[256];
     const int64_t T = blockIdx.x;
     if (T >= ntiles) return;
-    const int i = find_inst<3>(in, n, T);
-    if (!active[i]) return;
-    const BInst &I = in[i];
-    const double c = -adev[(int64_t)i * lda + *dstep];
-    const int64_t ch = T - I.c_off;
-    const int64_t base = I.voff + ch * BCH + threadIdx.x, end = I.voff + I.nreal;
+    const GGroup g = gr[tgrp[T]];
+    const int64_t ch = T - g.c_off;
+    const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
+    const int gi = g.cbase + k;
     double s = 0.0;
+    if (k < g.ncopy && active[gi]) {
+        const double a = -adev[(int64_t)gi * lda + *dstep];
+        const double sc = 1.0 / bdev[(int64_t)gi * lda + *dstep];
+        const int NC = cvec ? 2 : 1;
 #pragma unroll
-    for (int k = 0; k < BCH / 256; ++k) {
-        const int64_t e = base + (int64_t)k * 256;
-        if (e < end) {
-            const double r = fma(c, v[e], f[e]);
-            f[e] = r;
-            s = fma(r, r, s);
+        for (int j = 0; j < GCH; ++j) {
+            const int64_t e = (ch * GCH + j) * g.CW + c;
+            if (e < g.nelem) {
+                const int64_t i = g.voff + (e * g.G + k) * NC;
+                for (int q = 0; q < NC; ++q) {
+                    const double r = fma(a, sc * x[i + q], f[i + q]);
+                    f[i + q] = r;
+                    s = fma(r, r, s);
+                }
+            }
         }
     }
-    s = block_sum<256>(s, red);
-    if (threadIdx.x == 0) partial[I.p_off + ch] = s;
+    s_red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < g.G) {
+        double t = 0.0;
+        for (int cc = 0; cc < g.CW; ++cc) t += s_red[cc * g.G + threadIdx.x];
+        partial[g.p_off + ch * g.G + threadIdx.x] = t;
+    }
+}
+
+// one block per global copy: dst[copy*ld + step(+1)] = sum (which 0) or sqrt(sum) (which 1)
+__global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ gr, const int *__restrict__ cgroup,
+                                                     const int *__restrict__ active, const double *__restrict__ partial,
+                                                     int which, double *dst, int ld, const int *dstep)
+{
+    __shared__ double red[16];
+    const int gi = blockIdx.x;
+    if (!active[gi]) return;
+    const GGroup &g = gr[cgroup[gi]];
+    const int k = gi - g.cbase;
+    const int64_t np = which == 0 ? g.t2_n : g.c_n;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < np; t += 1024) s += partial[g.p_off + t * g.G + k];
+    s = block_sum<1024>(s, red);
+    if (threadIdx.x == 0) {
+        const int step = *dstep;
+        if (which == 0) dst[(int64_t)gi * ld + step] = s;
+        else dst[(int64_t)gi * ld + step + 1] = sqrt(s);
+    }
+}
+
+__global__ void k_g_bump(int *dstep) { dstep[0] += 1; }
+
+// start vectors: copy k of a group <- src (its instance's GenRand vector); NC doubles per element
+__global__ __launch_bounds__(256) void k_g_scatter(int64_t nelem, int G, int k, int NC, const double *__restrict__ src,
+                                                   double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nelem) return;
+    for (int q = 0; q < NC; ++q) dst[(e * G + k) * NC + q] = src[e * NC + q];
 }
 
 struct Batch {
-    int n = 0;
+    int n = 0, ng = 0;
     bool cvec = false, ctab = false;
-    std::vector<const Hubbard *> ops;
-    std::vector<BInst> host;
-    BInst *dinst = nullptr;
+    std::vector<GGroup> host;
+    std::vector<int> copy_of;  // instance -> global copy index
+    std::vector<int> inst_of;  // global copy index -> instance (-1: padding)
+    std::vector<int> cgroup;   // global copy -> group
+    int ncopies = 0;
+    GGroup *dgr = nullptr;
+    int *dcgroup = nullptr;
+    int *dtg1 = nullptr, *dtg2 = nullptr, *dtgc = nullptr; // tile -> group of the UP, DN, update grids
+    double *dvt = nullptr;
     int64_t total_real = 0, t1 = 0, t2 = 0, tc = 0, pcap = 0;
-    int maxnops = 1;
-    ~Batch() { dfree(dinst); }
+    int maxnops_dn = 1;
+    size_t lds_up = 0, lds_dn = 0; // dynamic LDS of the UP / DN kernels (largest group)
+    int gr = 4, gu = 1;            // DN / UP rows per tile (ED_BATCH_GR, ED_BATCH_GU)
+    ~Batch()
+    {
+        dfree(dgr);
+        dfree(dcgroup);
+        dfree(dtg1);
+        dfree(dtg2);
+        dfree(dtgc);
+        dfree(dvt);
+    }
 };
+
+static bool same_structure(const Hubbard *a, const Hubbard *b, const std::vector<OtfOp> &ua,
+                           const std::vector<OtfOp> &da, const std::vector<OtfOp> &ub, const std::vector<OtfOp> &db,
+                           const std::vector<double> &uva, const std::vector<double> &uvb)
+{
+    if (a->N != b->N || a->otf.dim_up != b->otf.dim_up || a->otf.dim_dn != b->otf.dim_dn ||
+        a->otf.state_up != b->otf.state_up || a->otf.state_dn != b->otf.state_dn || a->U != b->U ||
+        a->mask != b->mask || ua.size() != ub.size() || da.size() != db.size() || uva != uvb ||
+        a->otf.has_diag != b->otf.has_diag)
+        return false;
+    auto eq = [](const std::vector<OtfOp> &x, const std::vector<OtfOp> &y) {
+        for (size_t o = 0; o < x.size(); ++o)
+            if (x[o].kind != y[o].kind || x[o].i != y[o].i || x[o].j != y[o].j) return false;
+        return true;
+    };
+    return eq(ua, ub) && eq(da, db);
+}
 
 } // namespace ed
 
@@ -187,11 +372,15 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
     *out = nullptr;
     Batch *B = new Batch();
     B->n = ninst;
+    if (const char *e = getenv("ED_BATCH_GR")) B->gr = atoi(e) == 8 ? 8 : 4;
+    if (const char *e = getenv("ED_BATCH_GU")) B->gu = atoi(e) == 4 ? 4 : 1;
     B->cvec = complex_vectors != 0;
-    int64_t voff = 0, t1 = 0, t2 = 0, tc = 0, poff = 0;
+    std::vector<const Hubbard *> hs(ninst);
+    std::vector<std::vector<OtfOp>> opu(ninst), opd(ninst);
+    std::vector<std::vector<double>> uv(ninst);
     for (int i = 0; i < ninst; ++i) {
         const Hubbard *h = reinterpret_cast<const Hubbard *>(ops[i]);
-        if (!h || h->algo != 3) {
+        if (!h || h->algo != 3 || !h->otf.up.ops_own || !h->otf.dn.ops_own) {
             delete B;
             return fail(ED_EINVAL, "ed_batch_create: instance %d must come from ed_hubbard_create_ops", i);
         }
@@ -200,40 +389,151 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
             delete B;
             return fail(ED_EINVAL, "ed_batch_create: instances must share the table type (complex needs complex vectors)");
         }
-        B->ops.push_back(h);
-        BInst I{};
-        I.a = h->otf;
-        const int64_t du = h->otf.dim_up, dd = h->otf.dim_dn;
-        I.nreal = du * dd * (B->cvec ? 2 : 1);
-        I.voff = voff;
-        voff += (I.nreal + 31) / 32 * 32; // 256 B aligned instance vectors
-        I.has_up = h->otf.up.nops > 0 ? 1 : 0;
-        I.nrp1 = (du + 256 / BW - 1) / (256 / BW);
-        I.t1_n = ((dd + BW - 1) / BW) * I.nrp1;
-        I.t1_off = t1;
-        t1 += I.t1_n;
-        I.ncb2 = (dd + 255) / 256;
-        I.t2_n = ((du + BR - 1) / BR) * I.ncb2;
-        I.t2_off = t2;
-        t2 += I.t2_n;
-        I.c_n = (I.nreal + BCH - 1) / BCH;
-        I.c_off = tc;
-        tc += I.c_n;
-        I.p_off = poff;
-        poff += std::max(I.t2_n, I.c_n);
-        B->maxnops = std::max(B->maxnops, h->otf.up.nops);
-        B->host.push_back(I);
+        if (h->otf.dim_dn >= (1 << 24) || h->otf.dim_up >= (1 << 24)) {
+            delete B;
+            return fail(ED_EUNSUPPORTED, "ed_batch_create: sector dimension %lld x %lld too large for the batch",
+                        (long long)h->otf.dim_up, (long long)h->otf.dim_dn);
+        }
+        hs[i] = h;
+        opu[i].resize(h->otf.up.nops);
+        opd[i].resize(h->otf.dn.nops);
+        if ((h->otf.up.nops && hipMemcpy(opu[i].data(), h->otf.up.ops, sizeof(OtfOp) * h->otf.up.nops,
+                                         hipMemcpyDeviceToHost) != hipSuccess) ||
+            (h->otf.dn.nops && hipMemcpy(opd[i].data(), h->otf.dn.ops, sizeof(OtfOp) * h->otf.dn.nops,
+                                         hipMemcpyDeviceToHost) != hipSuccess)) {
+            delete B;
+            return fail(ED_EHIP, "ed_batch_create: operator readback failed");
+        }
+        if (h->d_uvec) {
+            uv[i].resize(h->N);
+            if (hipMemcpy(uv[i].data(), h->d_uvec, sizeof(double) * h->N, hipMemcpyDeviceToHost) != hipSuccess) {
+                delete B;
+                return fail(ED_EHIP, "ed_batch_create: per-site U readback failed");
+            }
+        }
     }
+    // greedy grouping, first-seen order
+    std::vector<std::vector<int>> groups;
+    for (int i = 0; i < ninst; ++i) {
+        int gsel = -1;
+        for (int gq = 0; gq < (int)groups.size() && gsel < 0; ++gq) {
+            const int r = groups[gq][0];
+            if ((int)groups[gq].size() < GMAX && same_structure(hs[r], hs[i], opu[r], opd[r], opu[i], opd[i], uv[r], uv[i]))
+                gsel = gq;
+        }
+        if (gsel < 0) {
+            groups.push_back({});
+            gsel = (int)groups.size() - 1;
+        }
+        groups[gsel].push_back(i);
+    }
+    const size_t tv = B->ctab ? 2 : 1; // doubles per value
+    std::vector<double> vt;
+    std::vector<int64_t> vt_up_off, vt_dn_off;
+    int64_t voff = 0, t1 = 0, t2 = 0, tc = 0, poff = 0;
+    B->copy_of.assign(ninst, -1);
+    for (const auto &mem : groups) {
+        const Hubbard *h = hs[mem[0]];
+        GGroup g{};
+        g.a = h->otf;
+        g.ncopy = (int)mem.size();
+        g.G = 1;
+        g.lg = 0;
+        // DN descriptors of CW = 256 / G columns and the value table must fit 62 KB of LDS
+        auto dn_lds = [&](int G) {
+            return (size_t)h->otf.dn.nops * (256 / G) * 4 + (size_t)h->otf.dn.nops * G * 2 * 8 * tv;
+        };
+        while (g.G < g.ncopy || dn_lds(g.G) > 63488) g.G <<= 1, ++g.lg;
+        B->lds_dn = std::max(B->lds_dn, dn_lds(g.G));
+        B->lds_up = std::max(B->lds_up, (size_t)h->otf.up.nops * g.G * 2 * 8 * tv);
+        g.CW = 256 / g.G;
+        g.cbase = B->ncopies;
+        for (int k = 0; k < g.G; ++k) {
+            B->inst_of.push_back(k < g.ncopy ? mem[k] : -1);
+            B->cgroup.push_back((int)B->host.size());
+            if (k < g.ncopy) B->copy_of[mem[k]] = g.cbase + k;
+        }
+        B->ncopies += g.G;
+        g.has_up = h->otf.up.nops > 0 ? 1 : 0;
+        const int64_t du = h->otf.dim_up, dd = h->otf.dim_dn;
+        g.nelem = du * dd;
+        if (g.nelem * g.G >= ((int64_t)1 << 31)) {
+            delete B;
+            return fail(ED_EUNSUPPORTED, "ed_batch_create: a group of %d copies of a %lld-element sector is too large for "
+                        "the batch (use ed_lanczos_eigen per instance)", g.G, (long long)g.nelem);
+        }
+        const int64_t nreal = g.nelem * g.G * (B->cvec ? 2 : 1);
+        g.voff = voff;
+        voff += (nreal + 31) / 32 * 32;
+        g.ncb = (dd + g.CW - 1) / g.CW;
+        g.t1_off = t1;
+        g.t1_n = ((du + B->gu - 1) / B->gu) * g.ncb;
+        t1 += g.t1_n;
+        g.t2_off = t2;
+        g.t2_n = ((du + B->gr - 1) / B->gr) * g.ncb;
+        t2 += g.t2_n;
+        g.c_off = tc;
+        g.c_n = (g.nelem + (int64_t)GCH * g.CW - 1) / ((int64_t)GCH * g.CW);
+        tc += g.c_n;
+        g.p_off = poff;
+        poff += std::max(g.t2_n, g.c_n) * g.G;
+        B->maxnops_dn = std::max(B->maxnops_dn, h->otf.dn.nops);
+        // value tables: [call][copy] x {vf, vb}
+        auto table = [&](bool up) {
+            const int nops = up ? h->otf.up.nops : h->otf.dn.nops;
+            const int64_t off = (int64_t)vt.size();
+            vt.resize(vt.size() + (size_t)nops * g.G * 2 * tv, 0.0);
+            for (int o = 0; o < nops; ++o)
+                for (int k = 0; k < g.ncopy; ++k) {
+                    const OtfOp &op = up ? opu[mem[k]][o] : opd[mem[k]][o];
+                    double *p = &vt[off + ((size_t)o * g.G + k) * 2 * tv];
+                    p[0] = op.vf_re;
+                    if (tv == 2) p[1] = op.vf_im;
+                    p[tv] = op.vb_re;
+                    if (tv == 2) p[tv + 1] = op.vb_im;
+                }
+            return off;
+        };
+        vt_up_off.push_back(table(true));
+        vt_dn_off.push_back(table(false));
+        B->host.push_back(g);
+    }
+    B->ng = (int)B->host.size();
     B->total_real = voff;
     B->t1 = t1;
     B->t2 = t2;
     B->tc = tc;
     B->pcap = poff;
-    if (int s = dmalloc((void **)&B->dinst, sizeof(BInst) * ninst)) {
+    int s = dmalloc((void **)&B->dvt, sizeof(double) * std::max<size_t>(vt.size(), 1));
+    if (!s) s = dmalloc((void **)&B->dgr, sizeof(GGroup) * B->ng);
+    if (!s) s = dmalloc((void **)&B->dcgroup, sizeof(int) * B->ncopies);
+    std::vector<int> tg1((size_t)B->t1), tg2((size_t)B->t2), tgc((size_t)B->tc);
+    for (int gq = 0; gq < B->ng; ++gq) {
+        const GGroup &g = B->host[gq];
+        std::fill(tg1.begin() + g.t1_off, tg1.begin() + g.t1_off + g.t1_n, gq);
+        std::fill(tg2.begin() + g.t2_off, tg2.begin() + g.t2_off + g.t2_n, gq);
+        std::fill(tgc.begin() + g.c_off, tgc.begin() + g.c_off + g.c_n, gq);
+    }
+    if (!s) s = dmalloc((void **)&B->dtg1, sizeof(int) * std::max<int64_t>(B->t1, 1));
+    if (!s) s = dmalloc((void **)&B->dtg2, sizeof(int) * std::max<int64_t>(B->t2, 1));
+    if (!s) s = dmalloc((void **)&B->dtgc, sizeof(int) * std::max<int64_t>(B->tc, 1));
+    if (s) {
         delete B;
         return s;
     }
-    if (hipMemcpy(B->dinst, B->host.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(B->dtg1, tg1.data(), sizeof(int) * B->t1, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dtg2, tg2.data(), sizeof(int) * B->t2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dtgc, tgc.data(), sizeof(int) * B->tc, hipMemcpyHostToDevice) != hipSuccess) {
+        delete B;
+        return fail(ED_EHIP, "ed_batch_create: tile table upload failed");
+    }
+    for (int gq = 0; gq < B->ng; ++gq) {
+        B->host[gq].vt_up = B->dvt + vt_up_off[gq];
+        B->host[gq].vt_dn = B->dvt + vt_dn_off[gq];
+    }
+    if ((!vt.empty() && hipMemcpy(B->dvt, vt.data(), sizeof(double) * vt.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(B->dgr, B->host.data(), sizeof(GGroup) * B->ng, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dcgroup, B->cgroup.data(), sizeof(int) * B->ncopies, hipMemcpyHostToDevice) != hipSuccess) {
         delete B;
         return fail(ED_EHIP, "ed_batch_create: upload failed");
     }
@@ -247,23 +547,37 @@ extern "C" int ed_batch_destroy(ed_batch bb)
     return ED_OK;
 }
 
+// one step: x = f_N, xp = f_{N-1}, y <- f_{N+1}
 template <typename V, typename TV>
-static int enqueue_step(const Batch &B, double *A, double *Bb, double *C, double *partial, const int *active,
-                        double *adev, double *bdev, int ld, int *dstep, hipStream_t st)
+static int enqueue_step(const Batch &B, const double *x, const double *xp, double *y, double *partial,
+                        const int *active, double *adev, double *bdev, int ld, int *dstep, bool first, hipStream_t st)
 {
     const int64_t per1 = (B.t1 + 7) / 8, per2 = (B.t2 + 7) / 8;
-    const size_t sh = (size_t)(256 / BW) * B.maxnops * (sizeof(TV) + sizeof(int32_t));
-    // pass 1: UP partial of every active instance -> C (the v buffer)
-    hipLaunchKernelGGL((k_b_up<V, TV>), dim3((unsigned)(per1 * 8)), dim3(256), sh, st, B.dinst, B.n, B.t1, per1,
-                       B.maxnops, active, bdev, ld, dstep, A, C);
-    // pass 2: B = H v - b B, C = v, partial <v, B>
-    hipLaunchKernelGGL((k_b_dn<V, TV>), dim3((unsigned)(per2 * 8)), dim3(256), 0, st, B.dinst, B.n, B.t2, per2,
-                       active, bdev, ld, dstep, A, C, Bb, partial);
-    hipLaunchKernelGGL(k_b_finalize, dim3(B.n), dim3(1024), 0, st, B.dinst, active, partial, 0, adev, ld, dstep);
-    hipLaunchKernelGGL(k_b_update, dim3((unsigned)B.tc), dim3(256), 0, st, B.dinst, B.n, B.tc, active, adev, ld,
-                       dstep, C, Bb, partial);
-    hipLaunchKernelGGL(k_b_finalize, dim3(B.n), dim3(1024), 0, st, B.dinst, active, partial, 1, bdev, ld, dstep);
-    hipLaunchKernelGGL(k_b_bump, dim3(1), dim3(1), 0, st, dstep);
+    if (B.gu == 4)
+        hipLaunchKernelGGL((k_g_up<V, TV, 4>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
+                           B.t1, per1, active, bdev, ld, dstep, x, y);
+    else
+        hipLaunchKernelGGL((k_g_up<V, TV, 1>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
+                           B.t1, per1, active, bdev, ld, dstep, x, y);
+    const size_t sh = B.lds_dn;
+#define ED_GDN(AX, R_)                                                                                        \
+    hipLaunchKernelGGL((k_g_dn<V, TV, AX, R_>), dim3((unsigned)(per2 * 8)), dim3(256), sh, st, B.dgr, B.dtg2, B.t2, \
+                       per2, active, bdev, ld, dstep, x, xp, y, partial)
+    if (B.gr == 8) {
+        if (first) ED_GDN(false, 8);
+        else ED_GDN(true, 8);
+    } else {
+        if (first) ED_GDN(false, 4);
+        else ED_GDN(true, 4);
+    }
+#undef ED_GDN
+    hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 0, adev,
+                       ld, dstep);
+    hipLaunchKernelGGL(k_g_update, dim3((unsigned)B.tc), dim3(256), 0, st, B.dgr, B.dtgc, B.tc, active, adev, bdev, ld,
+                       dstep, B.cvec ? 1 : 0, x, y, partial);
+    hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 1, bdev,
+                       ld, dstep);
+    hipLaunchKernelGGL(k_g_bump, dim3(1), dim3(1), 0, st, dstep);
     ED_CHECK_LAUNCH();
     return ED_OK;
 }
@@ -290,14 +604,15 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         if (ev) (void)hipEventDestroy(ev);
         return fail(ED_EHIP, "ed_batch_lowest: stream setup failed");
     }
-    stream = st;
-    const int n = B->n, ld = maxN + 3;
-    double *buf[3] = {nullptr, nullptr, nullptr}, *partial = nullptr, *adev = nullptr, *bdev = nullptr;
+    const int nc = B->ncopies, ld = maxN + 3;
+    double *buf[3] = {nullptr, nullptr, nullptr}, *partial = nullptr, *adev = nullptr, *bdev = nullptr,
+           *tmp = nullptr;
     int *active = nullptr, *dstep = nullptr;
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     int s = ED_OK;
     auto cleanup = [&] {
+        (void)hipStreamSynchronize(st);
         for (int k = 0; k < 3; ++k) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
@@ -308,70 +623,91 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         dfree(bdev);
         dfree(active);
         dfree(dstep);
-        (void)hipStreamSynchronize(st);
+        dfree(tmp);
         (void)hipStreamDestroy(st);
         (void)hipEventDestroy(ev);
     };
     do {
         for (int k = 0; k < 3 && !s; ++k) s = dmalloc((void **)&buf[k], sizeof(double) * (size_t)B->total_real);
         if (s) break;
+        int64_t maxe = 1;
+        for (const GGroup &g : B->host) maxe = std::max(maxe, g.nelem);
         if ((s = dmalloc((void **)&partial, sizeof(double) * (size_t)std::max<int64_t>(B->pcap, 1))) ||
-            (s = dmalloc((void **)&adev, sizeof(double) * (size_t)n * ld)) ||
-            (s = dmalloc((void **)&bdev, sizeof(double) * (size_t)n * ld)) ||
-            (s = dmalloc((void **)&active, sizeof(int) * n)) || (s = dmalloc((void **)&dstep, sizeof(int) * 2)))
+            (s = dmalloc((void **)&adev, sizeof(double) * (size_t)nc * ld)) ||
+            (s = dmalloc((void **)&bdev, sizeof(double) * (size_t)nc * ld)) ||
+            (s = dmalloc((void **)&active, sizeof(int) * nc)) || (s = dmalloc((void **)&dstep, sizeof(int) * 2)) ||
+            (s = dmalloc((void **)&tmp, sizeof(double) * (size_t)maxe * (B->cvec ? 2 : 1))))
             break;
-        // start vectors (the single-instance GenRand vector of each instance's length) and b_0
-        std::vector<double> hb0(n);
-        for (int i = 0; i < n && !s; ++i) {
-            const BInst &I = B->host[i];
-            const int64_t len = I.nreal / (B->cvec ? 2 : 1);
-            s = ed_random_vector(len, seed, B->cvec, buf[0] + I.voff, 1, stream);
-            if (!s) {
-                double out2[2];
-                s = ed_dot(len, B->cvec, buf[0] + I.voff, buf[0] + I.voff, out2, stream);
-                hb0[i] = std::sqrt(out2[0]);
+        // start vectors: each copy gets the normalised GenRand vector of its sector length (the
+        // padding copies stay zero and inactive); b_0 = |f| as in the single-instance driver
+        const int NC = B->cvec ? 2 : 1;
+        std::vector<double> hb0(nc, 0.0);
+        for (const GGroup &g : B->host) {
+            if (s) break;
+            if (hipMemsetAsync(buf[0] + g.voff, 0, sizeof(double) * (size_t)g.nelem * g.G * NC, st) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_batch_lowest: clear failed");
+                break;
+            }
+            if ((s = ed_random_vector(g.nelem, seed, B->cvec, tmp, 1, st))) break;
+            double out2[2];
+            if ((s = ed_dot(g.nelem, B->cvec, tmp, tmp, out2, st))) break;
+            for (int k = 0; k < g.ncopy; ++k) {
+                hipLaunchKernelGGL(k_g_scatter, dim3((unsigned)((g.nelem + 255) / 256)), dim3(256), 0, st, g.nelem,
+                                   g.G, k, NC, (const double *)tmp, buf[0] + g.voff);
+                hb0[g.cbase + k] = std::sqrt(out2[0]);
             }
         }
         if (s) break;
-        std::vector<double> hb(n * (size_t)ld, 0.0), ha(n * (size_t)ld, 0.0);
-        for (int i = 0; i < n; ++i) hb[(size_t)i * ld] = hb0[i];
-        std::vector<int> hact(n, 1), hstep2 = {0, 0};
-        if (hipStreamSynchronize(st) != hipSuccess) {
+        std::vector<double> hb(nc * (size_t)ld, 0.0), ha(nc * (size_t)ld, 0.0);
+        for (int c = 0; c < nc; ++c) hb[(size_t)c * ld] = hb0[c] > 0.0 ? hb0[c] : 1.0;
+        std::vector<int> hact(nc, 0), hstep2 = {0, 0};
+        for (int c = 0; c < nc; ++c) hact[c] = B->inst_of[c] >= 0 ? 1 : 0;
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
             s = fail(ED_EHIP, "ed_batch_lowest: start vectors failed");
             break;
         }
-        if (hipMemcpyAsync(bdev, hb.data(), sizeof(double) * n * ld, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(active, hact.data(), sizeof(int) * n, hipMemcpyHostToDevice, st) ||
+        if (hipMemcpyAsync(bdev, hb.data(), sizeof(double) * nc * ld, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(active, hact.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
             hipMemcpyAsync(dstep, hstep2.data(), sizeof(int) * 2, hipMemcpyHostToDevice, st)) {
             s = fail(ED_EHIP, "ed_batch_lowest: upload failed");
             break;
         }
-        std::vector<int> last(n, -1), done(n, 0), conv(n, 0), processed(n, 0);
-        std::vector<double> anorm(n, 0.0), z;
-        int remaining = n, phase = 0;
-        auto enqueue = [&](int ph, int nstep) -> int {
+        std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0);
+        std::vector<double> anorm(nc, 0.0), z, ev_c(nc, 0.0);
+        int remaining = 0;
+        for (int c = 0; c < nc; ++c) {
+            if (B->inst_of[c] >= 0) ++remaining;
+            else done[c] = 1;
+        }
+        int phase = 0, N = 0;
+        auto enqueue = [&](int ph, int n0, int nstep) -> int {
             for (int k = 0; k < nstep; ++k) {
-                double *A = buf[(ph + k) % 3], *Bq = buf[(ph + k + 1) % 3], *Cq = buf[(ph + k + 2) % 3];
-                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, A, Bq, Cq, partial, active, adev, bdev, ld,
-                                                                     dstep, st)
-                                           : enqueue_step<cplx, double>(*B, A, Bq, Cq, partial, active, adev, bdev,
-                                                                       ld, dstep, st))
-                                : enqueue_step<double, double>(*B, A, Bq, Cq, partial, active, adev, bdev, ld, dstep,
-                                                               st);
+                // buffer rotation: f_N in buf[N % 3], f_{N+1} goes to buf[(N + 1) % 3], f_{N-1}
+                // sits in buf[(N + 2) % 3]
+                const double *X = buf[(ph + k) % 3], *XP = buf[(ph + k + 2) % 3];
+                double *Y = buf[(ph + k + 1) % 3];
+                const bool first = (n0 + k) == 0;
+                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, adev, bdev, ld,
+                                                                     dstep, first, st)
+                                           : enqueue_step<cplx, double>(*B, X, XP, Y, partial, active, adev, bdev,
+                                                                       ld, dstep, first, st))
+                                : enqueue_step<double, double>(*B, X, XP, Y, partial, active, adev, bdev, ld, dstep,
+                                                               first, st);
                 if (r) return r;
             }
             return ED_OK;
         };
-        int N = 0;
         while (remaining > 0 && N < maxN) {
             const int chunk = std::min(every, maxN - N);
-            if (use_graph && chunk == every) {
+            // the first ten steps contain the step-0 kernel: run them eagerly; later phases
+            // are captured once and replayed
+            if (use_graph && chunk == every && N > 0) {
                 if (!gexec[phase]) {
                     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                         s = fail(ED_EHIP, "ed_batch_lowest: capture begin failed");
                         break;
                     }
-                    int r = enqueue(phase, chunk);
+                    int r = enqueue(phase, N, chunk);
                     hipError_t e = hipStreamEndCapture(st, &graph[phase]);
                     if (r || e != hipSuccess ||
                         hipGraphInstantiate(&gexec[phase], graph[phase], nullptr, nullptr, 0) != hipSuccess) {
@@ -383,58 +719,63 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     s = fail(ED_EHIP, "ed_batch_lowest: graph launch failed");
                     break;
                 }
-            } else if ((s = enqueue(phase, chunk))) {
+            } else if ((s = enqueue(phase, N, chunk))) {
                 break;
             }
             N += chunk;
             phase = (phase + chunk) % 3;
-            // convergence check of every active instance (same test as ed_lanczos_eigen)
-            if (hipMemcpyAsync(ha.data(), adev, sizeof(double) * n * ld, hipMemcpyDeviceToHost, st) ||
-                hipMemcpyAsync(hb.data(), bdev, sizeof(double) * n * ld, hipMemcpyDeviceToHost, st) ||
+            // convergence check of every active copy (same test as ed_lanczos_eigen)
+            // only the first N (a) and N + 1 (b) entries of each copy's row are new
+            const size_t pitch = sizeof(double) * ld;
+            if (hipMemcpy2DAsync(ha.data(), pitch, adev, pitch, sizeof(double) * N, nc, hipMemcpyDeviceToHost, st) ||
+                hipMemcpy2DAsync(hb.data(), pitch, bdev, pitch, sizeof(double) * (N + 1), nc, hipMemcpyDeviceToHost,
+                                 st) ||
                 hipStreamSynchronize(st)) {
                 s = fail(ED_EHIP, "ed_batch_lowest: readback failed");
                 break;
             }
             bool changed = false;
-            for (int i = 0; i < n; ++i) {
-                if (done[i]) continue;
-                const double *a = &ha[(size_t)i * ld], *b = &hb[(size_t)i * ld];
+            for (int c = 0; c < nc; ++c) {
+                if (done[c]) continue;
+                const double *a = &ha[(size_t)c * ld], *b = &hb[(size_t)c * ld];
                 const int Nl = N - 1;
                 int M = Nl;
-                for (int k = processed[i]; k <= Nl; ++k)
+                for (int k = processed[c]; k <= Nl; ++k)
                     if (!(b[k + 1] > 0.0) || !std::isfinite(a[k])) {
                         M = k;
                         break;
                     }
-                for (int k = processed[i]; k <= M; ++k) {
-                    if (k == 0) anorm[i] = std::fabs(a[0] + b[1]);
-                    else anorm[i] = std::fmax(anorm[i], b[k] + std::fabs(a[k] + b[k + 1]));
+                for (int k = processed[c]; k <= M; ++k) {
+                    if (k == 0) anorm[c] = std::fabs(a[0] + b[1]);
+                    else anorm[c] = std::fmax(anorm[c], b[k] + std::fabs(a[k] + b[k + 1]));
                 }
-                processed[i] = M + 1;
+                processed[c] = M + 1;
                 z.assign(M + 1, 0.0);
                 double lam;
                 ed_tridiag_lowest(M + 1, a, b + 1, &lam, z.data());
-                evals[i] = lam;
-                last[i] = M;
+                ev_c[c] = lam;
+                last[c] = M;
                 bool stop = false;
                 if (M < Nl || !(b[M + 1] > 0.0)) stop = true;
-                else if (std::fabs(b[M + 1] * z[M]) < tol * anorm[i] && std::fabs(z[0]) > tol * anorm[i]) stop = true;
+                else if (std::fabs(b[M + 1] * z[M]) < tol * anorm[c] && std::fabs(z[0]) > tol * anorm[c]) stop = true;
                 if (stop) {
-                    done[i] = conv[i] = 1;
-                    hact[i] = 0;
+                    done[c] = conv[c] = 1;
+                    hact[c] = 0;
                     --remaining;
                     changed = true;
                 }
             }
-            if (changed && hipMemcpyAsync(active, hact.data(), sizeof(int) * n, hipMemcpyHostToDevice, st)) {
+            if (changed && hipMemcpyAsync(active, hact.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
                 s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
                 break;
             }
         }
         if (s) break;
-        for (int i = 0; i < n; ++i) {
-            if (nsteps) nsteps[i] = last[i] + 1;
-            if (status) status[i] = conv[i] ? ED_OK : ED_ENOCONV;
+        for (int i = 0; i < B->n; ++i) {
+            const int c = B->copy_of[i];
+            evals[i] = ev_c[c];
+            if (nsteps) nsteps[i] = last[c] + 1;
+            if (status) status[i] = conv[c] ? ED_OK : ED_ENOCONV;
         }
         if (hipStreamSynchronize(st) != hipSuccess) s = fail(ED_EHIP, "ed_batch_lowest: final sync failed");
     } while (0);

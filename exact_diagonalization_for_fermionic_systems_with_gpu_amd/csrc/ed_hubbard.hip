@@ -738,6 +738,7 @@ extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *
     if (const char *e = getenv("ED_HV_DNMASK")) h->hv_dnmask = atoi(e) != 0;
     if (const char *e = getenv("ED_HV_DN2")) h->hv_dn2 = atoi(e) != 0;
     if (const char *e = getenv("ED_HV_DNK")) h->hv_dnk = atoi(e);
+    if (const char *e = getenv("ED_HV_UPXCD")) h->hv_upxcd = atoi(e) != 0;
     // pass 1 keeps 256/W rows of UP lists in LDS: rows with very many slots use the
     // single-pass kernel instead (its LDS holds one row's list)
     if ((size_t)32 * h->up.max_nnz * (2 * sizeof(double) + sizeof(int32_t)) + 256 > 65536) h->algo = 1;

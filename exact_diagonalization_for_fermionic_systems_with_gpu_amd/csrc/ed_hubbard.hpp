@@ -86,6 +86,7 @@ struct Hubbard {
     int hv_dnmask = 1; // pass 2 loads only for calls with an element in the column
     int hv_dn2 = 1;    // pass 2 with per-column descriptors cached in LDS over K row panels
     int hv_dnk = 2;
+    int hv_upxcd = 1;  // pass 1 tiles partitioned over the 8 XCDs by slab
     void release();
 };
 

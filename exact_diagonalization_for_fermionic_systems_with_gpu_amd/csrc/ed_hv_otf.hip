@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
     int32_t *s_col = reinterpret_cast<int32_t *>(s_val + ORB * nops); // [ORB][nops]
     __shared__ int s_cnt[ORB];
     const int64_t b = blockIdx.x;
-    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    const int64_t t = per_xcd ? (b & 7) * per_xcd + (b >> 3) : b; // per_xcd = 0: plain order
     if (t >= ntiles) return; // uniform over the workgroup
     const double sc = SCALE ? 1.0 / bdev[step] : 1.0;
     otf_up_body<V, TV, SCALE, TPR, VEC>(p, t, nrp, x, out, sc, s_val, s_col, s_cnt);
@@ -235,9 +235,12 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
     constexpr bool SC = (FLAGS & HV_SCALE) != 0;
     if (has_up) {
         const size_t sh = (size_t)(256 / P.TPR) * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
-        const dim3 g1((unsigned)(P.per1 * 8));
+        // XCD-partitioned slabs (each XCD walks its own range of slabs) or, with hv_upxcd = 0,
+        // all XCDs on the same slabs at once (one slab's rows in MALL instead of eight)
+        const int64_t pper = h.hv_upxcd ? P.per1 : 0;
+        const dim3 g1((unsigned)(h.hv_upxcd ? P.per1 * 8 : P.tiles1));
 #define ED_UP(TPR_, VEC_)                                                                                   \
-    hipLaunchKernelGGL((k_otf_up<V, TV, SC, TPR_, VEC_>), g1, dim3(256), sh, st, h.otf, P.tiles1, P.per1, P.nrp1, \
+    hipLaunchKernelGGL((k_otf_up<V, TV, SC, TPR_, VEC_>), g1, dim3(256), sh, st, h.otf, P.tiles1, pper, P.nrp1, \
                        x, upbuf, bdev, step)
         if (P.VEC == 2) {
             if (P.TPR == 256) ED_UP(256, 2);

@@ -1,0 +1,51 @@
+"""The 18-site Kagome Hubbard model at its full size (KagomeLattice18y bonds, hopping -1,
+N_up = N_dn = 9, U = 8, D = 2,363,904,400; three 18.9 GB vectors resident) against the CPU
+oracle's first 100 Lanczos steps from the same seed-235 start (tests/golden/
+kagome18y_lanczos100_oracle.json, written by tests/golden/make_kagome18y_e0.py).
+
+The ground state needs ~330 steps at this size, and the oracle's 100 steps already take about
+half an hour on 8 host cores, so the fixture pins a fixed-length recurrence. Any H.v or
+recurrence error changes alpha, beta and the Ritz value long before convergence.
+
+Bars:
+  * alpha and beta of the first 100 steps: 1e-9 absolute. They are O(10) and differ only by the
+    rounding of two summation orders, amplified along the recurrence (observed below 1e-12
+    over the first 30 steps).
+  * Lowest Ritz value of T_100: 1e-10 relative (the E0 bar).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard, Lanczos
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "kagome18y_lanczos100_oracle.json")
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
+def test_kagome18y_first_100_lanczos_steps():
+    gold = json.load(open(GOLD))
+    n = gold["nsteps"]
+    print("kagome18y 9/9 U=8: 100 Lanczos steps at D = 2.36e9", flush=True)
+    hq = H_TBq(18, 9)
+    hq.add_ops(L.hubbard_ops(L.KAGOME18Y, t=-1.0))
+    H = Hubbard(hq, hq, 8.0)
+    f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+    lz = Lanczos(H.dimUP, H.dimDN)
+    lz.random_vector(f)
+    lz.tridiagonal(H, f, n)
+    a, b = np.array(lz.a), np.array(lz.b)
+    del f
+    torch.cuda.empty_cache()
+    ga, gb = np.array(gold["alpha"]), np.array(gold["beta"])
+    print(f"max |d alpha| {np.abs(a - ga).max():.2e}, max |d beta| {np.abs(b - gb).max():.2e}", flush=True)
+    assert np.abs(a[:30] - ga[:30]).max() < 1e-11
+    assert np.abs(a - ga).max() < 1e-9 and np.abs(b - gb).max() < 1e-9
+    from scipy.linalg import eigvalsh_tridiagonal
+    theta = eigvalsh_tridiagonal(a, b[1:n], select="i", select_range=(0, 0))[0]
+    assert abs(theta - gold["ritz_lowest"]) < 1e-10 * abs(gold["ritz_lowest"]), (theta, gold["ritz_lowest"])
