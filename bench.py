@@ -167,6 +167,8 @@ def main():
     y = torch.empty_like(x)
     Lanczos(hq.count, hq.count).random_vector(x)
     torch.cuda.synchronize()
+    tuning = H.autotune(x, y, reps=2)
+    log(f"[bench] autotuned tile shapes: {tuning}")
 
     for _ in range(args.warmup):
         H.Hubbard_mv(x, y)
@@ -209,7 +211,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seed-235 minstd_rand start vector, the reference's GenRand)",
         "config": {"workload": f"hubbard_hv_{cl.name}_nup{Q}_ndn{Q}_U{U:g}", "sites": N, "dim": D,
-                   "hv_median_ms": round(median_ms, 4), "hv_mean_ms": round(mean_ms, 4)},
+                   "hv_median_ms": round(median_ms, 4), "hv_mean_ms": round(mean_ms, 4),
+                   "hv_tiles": {k: v for k, v in tuning.items() if k != "ms_per_apply"}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": None,

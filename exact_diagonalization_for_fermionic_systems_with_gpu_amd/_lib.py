@@ -32,6 +32,11 @@ class LanczosCfg(C.Structure):
     _fields_ = [("max_steps", C.c_int), ("tol", C.c_double), ("check_every", C.c_int), ("reorth", C.c_int)]
 
 
+class HvTuning(C.Structure):
+    _fields_ = [("slab_width", C.c_int), ("dn_cached", C.c_int), ("dn_panels", C.c_int), ("dn_rows", C.c_int),
+                ("ms_per_apply", C.c_double)]
+
+
 class EdError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"libedhip status {status}: {msg}")
@@ -52,6 +57,7 @@ SIGNATURES = {
     "ed_hubbard_destroy": (_i32, [_p]),
     "ed_hubbard_info": (_i32, [_p, _pi64, _pi64, _pi]),
     "ed_hubbard_apply": (_i32, [_p, _i32, _p, _p, _p]),
+    "ed_hubbard_autotune": (_i32, [_p, _i32, _p, _p, _i32, C.POINTER(HvTuning), _p]),
     "ed_random_vector": (_i32, [_i64, C.c_uint32, _i32, _p, _i32, _p]),
     "ed_dot": (_i32, [_i64, _i32, _p, _p, _pd, _p]),
     "ed_axpy": (_i32, [_i64, _i32, _d, _p, _p, _p]),

@@ -778,6 +778,77 @@ extern "C" int ed_hubbard_apply(ed_hubbard hh, int complex_vectors, const void *
     return hv_plain(*h, complex_vectors != 0, d_x, d_y, stream_of(stream));
 }
 
+// Tile-shape autotuner for the on-the-fly H.v: times every candidate (pass-1 slab width,
+// pass-2 kernel and row panel) on the caller's vectors and keeps the fastest in the handle.
+// All candidates add the same terms in the same order per element, so the result of H.v does
+// not depend on the choice.
+extern "C" int ed_hubbard_autotune(ed_hubbard hh, int complex_vectors, const void *d_x, void *d_y, int reps,
+                                   ed_hv_tuning *best, void *stream)
+{
+    Hubbard *h = reinterpret_cast<Hubbard *>(hh);
+    if (!h || !d_x || !d_y || d_x == d_y || reps < 1) return fail(ED_EINVAL, "ed_hubbard_autotune: bad arguments");
+    if (h->algo != 3) return fail(ED_EUNSUPPORTED, "ed_hubbard_autotune: needs an operator from ed_hubbard_create_ops");
+    hipStream_t st = stream_of(stream);
+    struct Cand {
+        int w, dn2, dnk, rmax;
+    };
+    std::vector<Cand> cands;
+    for (int w : {64, 128, 256})
+        for (Cand c : {Cand{w, 1, 2, 8}, Cand{w, 1, 4, 8}, Cand{w, 1, 2, 4}, Cand{w, 0, 0, 8}, Cand{w, 0, 0, 4}})
+            cands.push_back(c);
+    const Hubbard saved = *h;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ED_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ED_EHIP, "ed_hubbard_autotune: event creation failed");
+    }
+    float best_ms = 0.0f;
+    int bi = -1, s = ED_OK;
+    for (size_t ci = 0; ci < cands.size() && !s; ++ci) {
+        h->hv_w = cands[ci].w;
+        h->hv_dn2 = cands[ci].dn2;
+        h->hv_dnk = cands[ci].dnk ? cands[ci].dnk : h->hv_dnk;
+        h->hv_rmax = cands[ci].rmax;
+        h->hv_dnmask = 1;
+        if ((s = hv_plain(*h, complex_vectors != 0, d_x, d_y, st))) break; // warm-up
+        if (hipEventRecord(e0, st) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_hubbard_autotune: event record failed");
+            break;
+        }
+        for (int r = 0; r < reps && !s; ++r) s = hv_plain(*h, complex_vectors != 0, d_x, d_y, st);
+        float ms = 0.0f;
+        if (!s && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            s = fail(ED_EHIP, "ed_hubbard_autotune: timing failed");
+        if (!s && (bi < 0 || ms < best_ms)) {
+            best_ms = ms;
+            bi = (int)ci;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->hv_w = saved.hv_w;
+    h->hv_dn2 = saved.hv_dn2;
+    h->hv_dnk = saved.hv_dnk;
+    h->hv_rmax = saved.hv_rmax;
+    h->hv_dnmask = saved.hv_dnmask;
+    if (s) return s;
+    const Cand &c = cands[(size_t)bi];
+    h->hv_w = c.w;
+    h->hv_dn2 = c.dn2;
+    if (c.dnk) h->hv_dnk = c.dnk;
+    h->hv_rmax = c.rmax;
+    if (best) {
+        best->slab_width = c.w;
+        best->dn_cached = c.dn2;
+        best->dn_panels = c.dn2 ? c.dnk : 1;
+        best->dn_rows = c.rmax;
+        best->ms_per_apply = best_ms / reps;
+    }
+    return ED_OK;
+}
+
 extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int nops_up, int Q_dn, const ed_op *ops_dn,
                                      int nops_dn, int complex_tables, double U, int64_t mask, const double *Uvec,
                                      ed_hubbard *out)

@@ -87,6 +87,17 @@ class Hubbard:
         self._check(x, y)
         call("ed_hubbard_apply", self._h, _is_complex(x), _raw(x), _raw(y), stream_ptr())
 
+    def autotune(self, x: torch.Tensor, y: torch.Tensor, reps: int = 3) -> dict:
+        """Time the H.v tile-shape candidates on x (y is overwritten) and keep the fastest for this
+        operator; returns the choice.  Results do not depend on it."""
+        self._check(x, y)
+        from ._lib import HvTuning
+        t = HvTuning()
+        call("ed_hubbard_autotune", self.handle, _is_complex(x), _raw(x), _raw(y), int(reps), C.byref(t),
+             stream_ptr())
+        return {"slab_width": t.slab_width, "dn_cached": t.dn_cached, "dn_panels": t.dn_panels,
+                "dn_rows": t.dn_rows, "ms_per_apply": t.ms_per_apply}
+
     def apply(self, x: torch.Tensor) -> torch.Tensor:
         y = torch.empty_like(x)
         self.Hubbard_mv(x, y)

@@ -105,6 +105,21 @@ int ed_hubbard_info(ed_hubbard h, int64_t *dim_up, int64_t *dim_dn, int *complex
  * x != y.  complex_vectors selects M = complex<double>.  Asynchronous on stream. */
 int ed_hubbard_apply(ed_hubbard h, int complex_vectors, const void *d_x, void *d_y, void *stream);
 
+/* Tile shapes of the on-the-fly H.v (no counterpart in the reference, whose kernel has one
+ * launch shape, HubMatrix_GPU.cuh:54-61).  slab_width: columns per pass-1 tile (64, 128, 256;
+ * 128 and up load 16 B per lane); dn_cached: pass 2 keeps per-column call descriptors in LDS
+ * over dn_panels panels of dn_rows rows (else it regenerates them per panel). */
+typedef struct {
+    int slab_width, dn_cached, dn_panels, dn_rows;
+    double ms_per_apply;
+} ed_hv_tuning;
+/* Autotuner: times each candidate shape with `reps` applications of H to d_x (into d_y, which is
+ * overwritten) on the stream and keeps the fastest in the handle; every candidate gives the same
+ * y bit for bit.  Needs an operator from ed_hubbard_create_ops.  best (may be NULL) gets the
+ * choice and its time. */
+int ed_hubbard_autotune(ed_hubbard h, int complex_vectors, const void *d_x, void *d_y, int reps, ed_hv_tuning *best,
+                        void *stream);
+
 /* ---------------- vector BLAS-1 (Thrust_Lin, HubMatrix_GPU.cuh:291-471) -------- */
 
 /* GenRand + random_vector (HubMatrix_GPU.cuh:504-539): minstd_rand seeded with

@@ -260,3 +260,20 @@ def test_s2_parity_and_singlet():
     y = to_np(S.Hubbard_mv_new(torch.from_numpy(x).to(DEV)))
     yo = O.s2_apply(O.Species(12, 5, ops), O.Species(12, 6, ops), x)
     assert np.max(np.abs(y - yo)) <= 1e-14 * np.max(np.abs(yo))
+
+
+def test_autotune_keeps_results_bitwise():
+    """The tile-shape autotuner picks among kernels that add the same terms in the same order:
+    H.v after tuning equals H.v before, bit for bit, for real and complex vectors."""
+    ops = L.hubbard_ops(L.KAGOME12, t=-1.0)
+    gu, gd = gpu_species(12, 6, ops), gpu_species(12, 6, ops)
+    H = Hubbard(gu, gd, 4.0)
+    x = torch.from_numpy(np.random.default_rng(3).normal(size=H.dim)).to(DEV)
+    y0 = H.apply(x)
+    tmp = torch.empty_like(x)
+    best = H.autotune(x, tmp, reps=2)
+    assert best["slab_width"] in (64, 128, 256) and best["ms_per_apply"] > 0
+    y1 = H.apply(x)
+    assert torch.equal(y0, y1)
+    xc = x.to(torch.complex128) * (1 + 0.5j)
+    assert torch.equal(H.apply(xc), torch.complex(y0, 0.5 * y0))
