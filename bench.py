@@ -113,10 +113,11 @@ def batched_drude_line():
 
 def ensemble_line(n_real, ncv=32):
     """14-site open chain, 7/7, U = 4, W = 8: E0 of all realisations in one batched launch, then per
-    realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i."""
+    realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i, eight realisations at
+    a time on separate streams."""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
     res = DS.run_ensemble(n_real=n_real, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=ncv, tol=1e-10,
-                          log=lambda m: log("[bench] " + m), with_sref=False)
+                          log=lambda m: log("[bench] " + m), with_sref=False, workers=8)
     log(f"[bench] disorder ensemble: {n_real} realisations in {res.seconds:.2f} s "
         f"(batched E0 of all: {res.seconds_batched_e0:.2f} s)")
     return {"value": round(res.realisations_per_s, 3), "unit": "realisations/s", "realisations": n_real,
@@ -134,7 +135,7 @@ def main():
     ap.add_argument("--sites", type=int, default=18, choices=[12, 18])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched Drude sweep and the disorder-ensemble lines")
-    ap.add_argument("--ensemble", type=int, default=64, help="realisations of the 14-site chain ensemble")
+    ap.add_argument("--ensemble", type=int, default=128, help="realisations of the 14-site chain ensemble")
     ap.add_argument("--ground-state", action="store_true",
                     help="also run the Lanczos ground state of the same workload (Ritz vector rebuilt) and its "
                          "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")

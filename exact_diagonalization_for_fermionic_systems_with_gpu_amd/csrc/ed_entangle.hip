@@ -245,13 +245,16 @@ static int make_plan(int N, int Qu, int Qd, uint32_t maskA, Plan &P)
     return ED_OK;
 }
 
+// one rocBLAS handle per host thread (a handle carries its stream, so threads that work on
+// different streams must not share one); kept until the process ends
 static rocblas_handle solver_handle()
 {
-    static rocblas_handle h = nullptr;
-    static std::once_flag once;
-    std::call_once(once, [] {
+    thread_local rocblas_handle h = nullptr;
+    thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
         if (rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
-    });
+    }
     return h;
 }
 

@@ -86,7 +86,9 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
-                 tol=1e-10, maxit=1000, log=None, with_sref=True) -> EnsembleResult:
+                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1) -> EnsembleResult:
+    """workers > 1 solves that many realisations at once after the batched E0 launch, each worker
+    thread on its own stream."""
     eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -98,19 +100,40 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
     evals = np.zeros((n_real, nev))
     S = np.zeros((n_real, 2))
     g = np.zeros((n_real, L_sites))
-    for r, H in enumerate(hs):
-        es = SymEigsSolver(HubbardOp(H), nev, ncv)
-        es.init()
-        es.compute("SmallestAlge", maxit, tol)
-        if es.info() != "Successful":
-            raise RuntimeError(f"realisation {r}: eigensolver did not converge")
-        evals[r] = es.eigenvalues()
-        if log is not None:
-            log(f"realisation {r}: {es.niter} restarts, E0 {evals[r, 0]:.12f}, "
-                f"{time.perf_counter() - t0:.2f} s elapsed")
-        psi = es.eigenvectors()[0]
-        S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, with_sref)
-        g[r] = site_density(H, psi).sum(0)
-        del es, psi
+
+    def solve(r, stream):
+        # the library releases the GIL in every call; each worker thread drives its own stream
+        with torch.cuda.stream(stream):
+            H = hs[r]
+            es = SymEigsSolver(HubbardOp(H), nev, ncv)
+            es.init()
+            es.compute("SmallestAlge", maxit, tol)
+            if es.info() != "Successful":
+                raise RuntimeError(f"realisation {r}: eigensolver did not converge")
+            evals[r] = es.eigenvalues()
+            psi = es.eigenvectors()[0]
+            S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, with_sref)
+            g[r] = site_density(H, psi).sum(0)
+            stream.synchronize()
+            del es, psi
+            if log is not None:
+                log(f"realisation {r}: E0 {evals[r, 0]:.12f}, {time.perf_counter() - t0:.2f} s elapsed")
+
+    if workers <= 1:
+        for r in range(n_real):
+            solve(r, torch.cuda.current_stream())
+    else:
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+        local = threading.local()
+
+        def task(r):
+            if not hasattr(local, "stream"):
+                local.stream = torch.cuda.Stream()  # one stream per worker thread
+            solve(r, local.stream)
+
+        with ThreadPoolExecutor(workers) as ex:
+            for f in [ex.submit(task, r) for r in range(n_real)]:
+                f.result()
     torch.cuda.synchronize()
     return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb)

@@ -159,3 +159,14 @@ def test_chain14_against_fixture():
         assert abs(res.entropy[r, 0] - rec["S_vN"]) < 1e-8
         assert abs(res.entropy[r, 1] - rec["S_ref"]) < 1e-8
         assert np.allclose(res.grad_eps[r], rec["dE0_deps"], atol=1e-8)
+
+
+def test_disorder_ensemble_concurrent_workers_match_serial():
+    """Several realisations solved at once on separate streams (one rocBLAS handle and scratch per
+    host thread) give the serial results."""
+    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, ncv=12, tol=1e-12)
+    a = DS.run_ensemble(**kw, workers=1)
+    b = DS.run_ensemble(**kw, workers=3)
+    assert np.allclose(a.evals, b.evals, rtol=1e-12, atol=0)
+    assert np.allclose(a.entropy, b.entropy, rtol=0, atol=1e-10)
+    assert np.allclose(a.grad_eps, b.grad_eps, rtol=0, atol=1e-10)
