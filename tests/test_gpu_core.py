@@ -115,7 +115,8 @@ def _hv_cases():
 
 _OTF_ENV = {"otf": {}, "otf_w16": {"ED_HV_W": "16"}, "otf_w128": {"ED_HV_W": "128"},
             "otf_w256": {"ED_HV_W": "256"}, "otf_w512": {"ED_HV_W": "512"},
-            "otf_nomask_r4": {"ED_HV_DNMASK": "0", "ED_HV_RMAX": "4"}, "otf_r1": {"ED_HV_RMAX": "1"}}
+            "otf_nomask_r4": {"ED_HV_DNMASK": "0", "ED_HV_RMAX": "4"}, "otf_r1": {"ED_HV_RMAX": "1"},
+            "otf_dn1": {"ED_HV_DN2": "0"}, "otf_k4": {"ED_HV_DNK": "4"}}
 
 
 @pytest.mark.parametrize("path", list(_OTF_ENV) + ["tables", "rowkernel"])
