@@ -93,7 +93,7 @@ __device__ __forceinline__ uint32_t gdesc(const OtfOp &op, int o, uint32_t s, ui
 template <typename V, typename TV, int GU>
 __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
                                               const int *__restrict__ active, const double *__restrict__ bdev,
-                                              int ldb, const int *__restrict__ dstep, const double *x, double *y)
+                                              int ldb, const int *__restrict__ dstep, const double *x, double *y, int kp)
 {
     extern __shared__ double s_upv[]; // value table, TV [nops_up][G][2]
     __shared__ uint32_t s_desc[GUMAX][64];
@@ -106,47 +106,54 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
     TV *s_vt = reinterpret_cast<TV *>(s_upv);
     stage_table<TV>(g.vt_up, g.a.up.nops, g.G, s_vt);
     const int64_t du = g.a.dim_up, dd = g.a.dim_dn;
-    const int64_t nrp = (du + GU - 1) / GU;
-    const int64_t tl = T - g.t1_off, cb = tl / nrp, u0 = (tl - cb * nrp) * GU;
-    if ((threadIdx.x >> 6) < GU) { // wave r evaluates the calls of row u0 + r
-        const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int64_t u = u0 + r;
-        uint32_t dsc = 0xffffffffu;
-        if (u < du && lane < g.a.up.nops)
-            dsc = gdesc(g.a.up.ops[lane], lane, (uint32_t)g.a.state_up[u], (uint32_t)u, g.a.up.addr);
-        const bool has = dsc != 0xffffffffu;
-        const uint64_t bal = __ballot(has);
-        if (has) s_desc[r][__popcll(bal & ((1ull << lane) - 1ull))] = dsc;
-        if (lane == 0) s_cnt[r] = __popcll(bal);
-    }
-    __syncthreads();
+    const int64_t nrg = (du + (int64_t)GU * kp - 1) / ((int64_t)GU * kp);
+    const int64_t tl = T - g.t1_off, cb = tl / nrg, rg = tl - cb * nrg;
     const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
     const int64_t d = cb * g.CW + c;
-    if (k >= g.ncopy || d >= dd) return;
     const int gi = g.cbase + k;
-    if (!active[gi]) return;
-    const double sc = 1.0 / bdev[(int64_t)gi * ldb + *dstep];
+    const bool live = k < g.ncopy && d < dd && active[gi];
+    const double sc = live ? 1.0 / bdev[(int64_t)gi * ldb + *dstep] : 0.0;
     const V *xv = reinterpret_cast<const V *>(x + g.voff);
     V *yv = reinterpret_cast<V *>(y + g.voff);
     const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k, du32 = (uint32_t)d;
-    for (int r = 0; r < GU; ++r) {
-        const int64_t u = u0 + r;
-        if (u >= du) break;
-        const int n = s_cnt[r];
-        V acc = VO<V>::zero();
-        for (int j0 = 0; j0 < n; j0 += 8) {
-            V xl[8];
-            uint32_t ds[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                ds[q] = (j0 + q < n) ? s_desc[r][j0 + q] : 0u;
-                if (j0 + q < n) xl[q] = xv[(((ds[q] >> 8) * ddu + du32) << lg) + ku];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (j0 + q < n) acc = omac(gval<TV>(s_vt, g.G, ds[q] & 0xffu, k), VO<V>::scale(sc, xl[q]), acc);
+    // kp panels of GU rows; the value table is staged once per workgroup
+    for (int pn = 0; pn < kp; ++pn) {
+        const int64_t u0 = (rg * kp + pn) * GU;
+        if (u0 >= du) break; // uniform
+        if (pn > 0) __syncthreads(); // the previous panel's hop lists are no longer read
+        if ((threadIdx.x >> 6) < GU) { // wave r evaluates the calls of row u0 + r
+            const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const int64_t u = u0 + r;
+            uint32_t dsc = 0xffffffffu;
+            if (u < du && lane < g.a.up.nops)
+                dsc = gdesc(g.a.up.ops[lane], lane, (uint32_t)g.a.state_up[u], (uint32_t)u, g.a.up.addr);
+            const bool has = dsc != 0xffffffffu;
+            const uint64_t bal = __ballot(has);
+            if (has) s_desc[r][__popcll(bal & ((1ull << lane) - 1ull))] = dsc;
+            if (lane == 0) s_cnt[r] = __popcll(bal);
         }
-        yv[(((uint32_t)u * ddu + du32) << lg) + ku] = acc;
+        __syncthreads();
+        if (!live) continue;
+        for (int r = 0; r < GU; ++r) {
+            const int64_t u = u0 + r;
+            if (u >= du) break;
+            const int n = s_cnt[r];
+            V acc = VO<V>::zero();
+            for (int j0 = 0; j0 < n; j0 += 8) {
+                V xl[8];
+                uint32_t ds[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    ds[q] = (j0 + q < n) ? s_desc[r][j0 + q] : 0u;
+                    if (j0 + q < n) xl[q] = xv[(((ds[q] >> 8) * ddu + du32) << lg) + ku];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (j0 + q < n)
+                        acc = omac(gval<TV>(s_vt, g.G, ds[q] & 0xffu, k), VO<V>::scale(sc, xl[q]), acc);
+            }
+            yv[(((uint32_t)u * ddu + du32) << lg) + ku] = acc;
+        }
     }
 }
 
@@ -160,7 +167,7 @@ template <typename V, typename TV, bool AXPY, int GR>
 __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
                                               const int *__restrict__ active, const double *__restrict__ bdev,
                                               int ldb, const int *__restrict__ dstep, const double *x,
-                                              const double *xp, double *y, double *partial)
+                                              const double *xp, double *y, double *partial, int kp)
 {
     extern __shared__ double s_dnv[]; // value table TV [nops_dn][G][2], then descriptors [nops_dn][CW]
     __shared__ double s_red[256];
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
     uint32_t *s_gd = reinterpret_cast<uint32_t *>(s_vt + g.a.dn.nops * g.G * 2);
     stage_table<TV>(g.vt_dn, g.a.dn.nops, g.G, s_vt);
     const int64_t tl = T - g.t2_off, rp = tl / g.ncb, cb = tl - rp * g.ncb;
-    const int64_t du = g.a.dim_up, dd = g.a.dim_dn, u0 = rp * GR;
+    const int64_t du = g.a.dim_up, dd = g.a.dim_dn;
     const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
     const int64_t d = cb * g.CW + c;
     const int nops = g.a.dn.nops;
@@ -183,7 +190,10 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
     const int gi = g.cbase + k;
     const bool live = k < g.ncopy && d < dd && active[gi];
     double dot = 0.0;
-    if (live) {
+    // kp panels of GR rows share the column descriptors and the value table
+    for (int pn = 0; pn < kp && live; ++pn) {
+        const int64_t u0 = (rp * kp + pn) * GR;
+        if (u0 >= du) break;
         const int step = *dstep;
         const double bN = bdev[(int64_t)gi * ldb + step], sc = 1.0 / bN;
         const double scp = AXPY ? 1.0 / bdev[(int64_t)gi * ldb + step - 1] : 0.0;
@@ -333,7 +343,8 @@ struct Batch {
     int64_t total_real = 0, t1 = 0, t2 = 0, tc = 0, pcap = 0;
     int maxnops_dn = 1;
     size_t lds_up = 0, lds_dn = 0; // dynamic LDS of the UP / DN kernels (largest group)
-    int gr = 4, gu = 1;            // DN / UP rows per tile (ED_BATCH_GR, ED_BATCH_GU)
+    int gr = 4, gu = 1;            // DN / UP rows per panel (ED_BATCH_GR, ED_BATCH_GU)
+    int kp = 4, kpu = 4;           // DN / UP panels per workgroup (ED_BATCH_KP, ED_BATCH_KPU)
     ~Batch()
     {
         dfree(dgr);
@@ -374,6 +385,8 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
     B->n = ninst;
     if (const char *e = getenv("ED_BATCH_GR")) B->gr = atoi(e) == 8 ? 8 : 4;
     if (const char *e = getenv("ED_BATCH_GU")) B->gu = atoi(e) == 4 ? 4 : 1;
+    if (const char *e = getenv("ED_BATCH_KP")) B->kp = std::max(1, std::min(16, atoi(e)));
+    if (const char *e = getenv("ED_BATCH_KPU")) B->kpu = std::max(1, std::min(64, atoi(e)));
     B->cvec = complex_vectors != 0;
     std::vector<const Hubbard *> hs(ninst);
     std::vector<std::vector<OtfOp>> opu(ninst), opd(ninst);
@@ -467,10 +480,10 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
         voff += (nreal + 31) / 32 * 32;
         g.ncb = (dd + g.CW - 1) / g.CW;
         g.t1_off = t1;
-        g.t1_n = ((du + B->gu - 1) / B->gu) * g.ncb;
+        g.t1_n = ((du + (int64_t)B->gu * B->kpu - 1) / ((int64_t)B->gu * B->kpu)) * g.ncb;
         t1 += g.t1_n;
         g.t2_off = t2;
-        g.t2_n = ((du + B->gr - 1) / B->gr) * g.ncb;
+        g.t2_n = ((du + (int64_t)B->gr * B->kp - 1) / ((int64_t)B->gr * B->kp)) * g.ncb;
         t2 += g.t2_n;
         g.c_off = tc;
         g.c_n = (g.nelem + (int64_t)GCH * g.CW - 1) / ((int64_t)GCH * g.CW);
@@ -555,14 +568,14 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
     const int64_t per1 = (B.t1 + 7) / 8, per2 = (B.t2 + 7) / 8;
     if (B.gu == 4)
         hipLaunchKernelGGL((k_g_up<V, TV, 4>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, bdev, ld, dstep, x, y);
+                           B.t1, per1, active, bdev, ld, dstep, x, y, B.kpu);
     else
         hipLaunchKernelGGL((k_g_up<V, TV, 1>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, bdev, ld, dstep, x, y);
+                           B.t1, per1, active, bdev, ld, dstep, x, y, B.kpu);
     const size_t sh = B.lds_dn;
 #define ED_GDN(AX, R_)                                                                                        \
     hipLaunchKernelGGL((k_g_dn<V, TV, AX, R_>), dim3((unsigned)(per2 * 8)), dim3(256), sh, st, B.dgr, B.dtg2, B.t2, \
-                       per2, active, bdev, ld, dstep, x, xp, y, partial)
+                       per2, active, bdev, ld, dstep, x, xp, y, partial, B.kp)
     if (B.gr == 8) {
         if (first) ED_GDN(false, 8);
         else ED_GDN(true, 8);
