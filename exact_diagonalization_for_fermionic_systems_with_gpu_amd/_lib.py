@@ -34,7 +34,7 @@ class LanczosCfg(C.Structure):
 
 class HvTuning(C.Structure):
     _fields_ = [("slab_width", C.c_int), ("dn_cached", C.c_int), ("dn_panels", C.c_int), ("dn_rows", C.c_int),
-                ("ms_per_apply", C.c_double)]
+                ("row_order", C.c_int), ("up_table", C.c_int), ("ms_per_apply", C.c_double)]
 
 
 class EdError(RuntimeError):

@@ -659,6 +659,12 @@ void Hubbard::release()
     dfree(d_uvec);
     dfree(d_addr_up);
     dfree(d_addr_dn);
+    dfree(d_up_order);
+    dfree(d_up_tab);
+    dfree(d_up_cnt);
+    d_up_order = nullptr;
+    d_up_tab = nullptr;
+    d_up_cnt = nullptr;
     dfree(otf.up.ops_own);
     dfree(otf.dn.ops_own);
     d_state_up = d_state_dn = nullptr;
@@ -739,6 +745,7 @@ extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *
     if (const char *e = getenv("ED_HV_DN2")) h->hv_dn2 = atoi(e) != 0;
     if (const char *e = getenv("ED_HV_DNK")) h->hv_dnk = atoi(e);
     if (const char *e = getenv("ED_HV_UPXCD")) h->hv_upxcd = atoi(e) != 0;
+    if (const char *e = getenv("ED_HV_UPTAB")) h->hv_uptab = atoi(e) != 0;
     // pass 1 keeps 256/W rows of UP lists in LDS: rows with very many slots use the
     // single-pass kernel instead (its LDS holds one row's list)
     if ((size_t)32 * h->up.max_nnz * (2 * sizeof(double) + sizeof(int32_t)) + 256 > 65536) h->algo = 1;
@@ -778,10 +785,86 @@ extern "C" int ed_hubbard_apply(ed_hubbard hh, int complex_vectors, const void *
     return hv_plain(*h, complex_vectors != 0, d_x, d_y, stream_of(stream));
 }
 
-// Tile-shape autotuner for the on-the-fly H.v: times every candidate (pass-1 slab width,
-// pass-2 kernel and row panel) on the caller's vectors and keeps the fastest in the handle.
-// All candidates add the same terms in the same order per element, so the result of H.v does
-// not depend on the choice.
+// Reverse Cuthill-McKee order of the hopping graph of one sector: vertices are the colex
+// ranks, edges the hopping calls (a hop links s and s ^ (1<<i | 1<<j) when exactly one of the
+// two sites is occupied).  Pass 1 of the on-the-fly H.v walks its rows in this order, so the
+// rows read for one row were mostly read for rows visited shortly before and are still in L2
+// (breadth-first level sets instead of colex ranks, whose hops on high sites jump far).
+static int up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> &order)
+{
+    int64_t dim = 0;
+    ED_TRY(ed_basis_count(N, Q, &dim));
+    if (dim > (int64_t)1 << 30) return fail(ED_EUNSUPPORTED, "row order: sector too large");
+    int64_t C[33][33] = {};
+    for (int n = 0; n <= 32; ++n) {
+        C[n][0] = 1;
+        for (int k = 1; k <= n; ++k) C[n][k] = C[n - 1][k - 1] + (k <= n - 1 ? C[n - 1][k] : 0);
+    }
+    auto rank = [&](uint32_t s) {
+        int64_t r = 0;
+        for (int k = 1; s; ++k, s &= s - 1) r += C[__builtin_ctz(s)][k];
+        return r;
+    };
+    std::vector<uint32_t> st((size_t)dim);
+    uint64_t v = Q ? (((uint64_t)1 << Q) - 1) : 0;
+    for (int64_t r = 0; r < dim; ++r) {
+        st[(size_t)r] = (uint32_t)v;
+        if (r + 1 < dim) { // next integer with Q bits set
+            const uint64_t c = v & (~v + 1), t = v + c;
+            v = (((t ^ v) >> 2) / c) | t;
+        }
+    }
+    std::vector<uint32_t> hm;
+    for (int o = 0; o < nops; ++o)
+        if (ops[o].kind != ED_OP_DENSITY && ops[o].kind != ED_OP_DENSITY2 && ops[o].i != ops[o].j)
+            hm.push_back((1u << ops[o].i) | (1u << ops[o].j));
+    auto for_nbrs = [&](int64_t r, auto &&f) {
+        const uint32_t s = st[(size_t)r];
+        for (uint32_t m : hm) {
+            const uint32_t b = s & m;
+            if (b && b != m) f(rank(s ^ m));
+        }
+    };
+    std::vector<int32_t> deg((size_t)dim);
+    for (int64_t r = 0; r < dim; ++r) {
+        int n = 0;
+        for_nbrs(r, [&](int64_t) { ++n; });
+        deg[(size_t)r] = n;
+    }
+    std::vector<int32_t> starts((size_t)dim);
+    for (int64_t r = 0; r < dim; ++r) starts[(size_t)r] = (int32_t)r;
+    std::stable_sort(starts.begin(), starts.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+    std::vector<char> seen((size_t)dim, 0);
+    order.clear();
+    order.reserve((size_t)dim);
+    std::vector<int32_t> nb;
+    for (int32_t s0 : starts) {
+        if (seen[s0]) continue;
+        seen[s0] = 1;
+        size_t head = order.size();
+        order.push_back(s0);
+        while (head < order.size()) {
+            nb.clear();
+            for_nbrs(order[head++], [&](int64_t q) {
+                if (!seen[q]) {
+                    seen[q] = 1;
+                    nb.push_back((int32_t)q);
+                }
+            });
+            std::sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return ED_OK;
+}
+
+// Tile-shape autotuner for the on-the-fly H.v: times the candidates on the caller's vectors
+// and keeps the fastest in the handle.  Pass 1 (slab width, row order) and pass 2 (kernel, row
+// panels) are separate kernels whose times add, so the search runs in two stages: the pass-1
+// candidates with the default pass 2, then the pass-2 candidates with the best pass 1.  All
+// candidates add the same terms in the same order per element, so the result of H.v does not
+// depend on the choice.
 extern "C" int ed_hubbard_autotune(ed_hubbard hh, int complex_vectors, const void *d_x, void *d_y, int reps,
                                    ed_hv_tuning *best, void *stream)
 {
@@ -790,12 +873,8 @@ extern "C" int ed_hubbard_autotune(ed_hubbard hh, int complex_vectors, const voi
     if (h->algo != 3) return fail(ED_EUNSUPPORTED, "ed_hubbard_autotune: needs an operator from ed_hubbard_create_ops");
     hipStream_t st = stream_of(stream);
     struct Cand {
-        int w, dn2, dnk, rmax;
+        int w, ord, tab, dn2, dnk, rmax;
     };
-    std::vector<Cand> cands;
-    for (int w : {64, 128, 256})
-        for (Cand c : {Cand{w, 1, 2, 8}, Cand{w, 1, 4, 8}, Cand{w, 1, 2, 4}, Cand{w, 0, 0, 8}, Cand{w, 0, 0, 4}})
-            cands.push_back(c);
     const Hubbard saved = *h;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ED_HIP(hipEventCreate(&e0));
@@ -803,47 +882,74 @@ extern "C" int ed_hubbard_autotune(ed_hubbard hh, int complex_vectors, const voi
         (void)hipEventDestroy(e0);
         return fail(ED_EHIP, "ed_hubbard_autotune: event creation failed");
     }
-    float best_ms = 0.0f;
-    int bi = -1, s = ED_OK;
-    for (size_t ci = 0; ci < cands.size() && !s; ++ci) {
-        h->hv_w = cands[ci].w;
-        h->hv_dn2 = cands[ci].dn2;
-        h->hv_dnk = cands[ci].dnk ? cands[ci].dnk : h->hv_dnk;
-        h->hv_rmax = cands[ci].rmax;
+    auto set = [&](const Cand &c) {
+        h->hv_w = c.w;
+        h->hv_uporder = c.ord;
+        h->hv_uptab = c.tab;
+        h->hv_dn2 = c.dn2;
+        h->hv_dnk = c.dnk ? c.dnk : saved.hv_dnk;
+        h->hv_rmax = c.rmax;
         h->hv_dnmask = 1;
-        if ((s = hv_plain(*h, complex_vectors != 0, d_x, d_y, st))) break; // warm-up
-        if (hipEventRecord(e0, st) != hipSuccess) {
-            s = fail(ED_EHIP, "ed_hubbard_autotune: event record failed");
-            break;
-        }
+    };
+    auto timeit = [&](const Cand &c, float &ms) {
+        set(c);
+        int s = hv_plain(*h, complex_vectors != 0, d_x, d_y, st); // warm-up
+        if (!s && hipEventRecord(e0, st) != hipSuccess) s = fail(ED_EHIP, "ed_hubbard_autotune: event record failed");
         for (int r = 0; r < reps && !s; ++r) s = hv_plain(*h, complex_vectors != 0, d_x, d_y, st);
-        float ms = 0.0f;
         if (!s && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             s = fail(ED_EHIP, "ed_hubbard_autotune: timing failed");
-        if (!s && (bi < 0 || ms < best_ms)) {
+        return s;
+    };
+    Cand bc{128, h->d_up_order ? 1 : 0, h->d_up_tab ? 1 : 0, 1, 2, 8};
+    float best_ms = 0.0f;
+    int s = ED_OK;
+    bool first = true;
+    // stage 1: pass-1 shapes
+    for (int w : {32, 64, 128, 256})
+        for (int ord = h->d_up_order ? 1 : 0; ord >= 0; --ord)
+            for (int tab = h->d_up_tab ? 1 : 0; tab >= 0 && !s; --tab) {
+                const Cand c{w, ord, tab, 1, 2, 8};
+                float ms = 0.0f;
+                if ((s = timeit(c, ms))) break;
+                if (first || ms < best_ms) {
+                    best_ms = ms;
+                    bc = c;
+                    first = false;
+                }
+            }
+    // stage 2: pass-2 shapes with the chosen pass 1
+    const Cand dns[] = {{0, 0, 0, 1, 4, 8}, {0, 0, 0, 1, 2, 4}, {0, 0, 0, 1, 1, 8}, {0, 0, 0, 0, 0, 8},
+                        {0, 0, 0, 0, 0, 4}};
+    const Cand b1 = bc;
+    for (const Cand &d : dns) {
+        if (s) break;
+        const Cand c{b1.w, b1.ord, b1.tab, d.dn2, d.dnk, d.rmax};
+        float ms = 0.0f;
+        if ((s = timeit(c, ms))) break;
+        if (ms < best_ms) {
             best_ms = ms;
-            bi = (int)ci;
+            bc = c;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     h->hv_w = saved.hv_w;
+    h->hv_uporder = saved.hv_uporder;
+    h->hv_uptab = saved.hv_uptab;
     h->hv_dn2 = saved.hv_dn2;
     h->hv_dnk = saved.hv_dnk;
     h->hv_rmax = saved.hv_rmax;
     h->hv_dnmask = saved.hv_dnmask;
     if (s) return s;
-    const Cand &c = cands[(size_t)bi];
-    h->hv_w = c.w;
-    h->hv_dn2 = c.dn2;
-    if (c.dnk) h->hv_dnk = c.dnk;
-    h->hv_rmax = c.rmax;
+    set(bc);
     if (best) {
-        best->slab_width = c.w;
-        best->dn_cached = c.dn2;
-        best->dn_panels = c.dn2 ? c.dnk : 1;
-        best->dn_rows = c.rmax;
+        best->slab_width = bc.w;
+        best->row_order = bc.ord;
+        best->up_table = bc.tab;
+        best->dn_cached = bc.dn2;
+        best->dn_panels = bc.dn2 ? h->hv_dnk : 1;
+        best->dn_rows = bc.rmax;
         best->ms_per_apply = best_ms / reps;
     }
     return ED_OK;
@@ -917,6 +1023,21 @@ extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int n
         // 16 B per lane); narrower slabs re-walk every row of x once per slab (18-site H.v, UP
         // pass: W=8 82 ms, W=16 ~55 ms, W=64 37 ms, W=128 ~33 ms)
         if (algo == 3 && !getenv("ED_HV_W")) h->hv_w = 128;
+        // pass-1 row order for vectors far beyond L2 + MALL (1 GiB of FP64 and up);
+        // ED_HV_UPORDER=1 builds it at any size, 0 never
+        const char *eo = getenv("ED_HV_UPORDER");
+        const bool want = eo ? atoi(eo) != 0 : (double)cu * (double)cd >= (double)(1 << 27);
+        if (algo == 3 && want && cu > 1) {
+            std::vector<int32_t> ord;
+            if ((s = up_row_order(N, Q_up, ops_up, nops_up, ord))) break;
+            if ((s = dmalloc((void **)&h->d_up_order, sizeof(int32_t) * ord.size()))) break;
+            if (hipMemcpy(h->d_up_order, ord.data(), sizeof(int32_t) * ord.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                s = fail(ED_EHIP, "ed_hubbard_create_ops: row order upload failed");
+                break;
+            }
+        }
+        if (algo == 3 && (s = otf_build_up_table(*h, nullptr))) break;
     } while (0);
     cleanup();
     if (s) {

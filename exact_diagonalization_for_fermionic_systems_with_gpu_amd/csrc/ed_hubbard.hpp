@@ -67,6 +67,9 @@ struct OtfArgs {
     int64_t mask;
     const double *Uvec;
     int has_diag;
+    const int32_t *up_order; // pass-1 row visiting order (rank list), nullptr: ascending rank
+    const uint32_t *up_tab;  // pass-1 entries per row position (column << 8 | value code), or nullptr
+    const int32_t *up_cnt;   // entries per row position
 };
 
 struct Hubbard {
@@ -87,6 +90,11 @@ struct Hubbard {
     int hv_dn2 = 1;    // pass 2 with per-column descriptors cached in LDS over K row panels
     int hv_dnk = 2;
     int hv_upxcd = 1;  // pass 1 tiles partitioned over the 8 XCDs by slab
+    int32_t *d_up_order = nullptr; // reverse Cuthill-McKee order of the UP hopping graph
+    int hv_uporder = 1;            // pass 1 visits rows in d_up_order (when it exists)
+    uint32_t *d_up_tab = nullptr;  // pass-1 entry table in visiting order (dim_up x nops words)
+    int32_t *d_up_cnt = nullptr;
+    int hv_uptab = 1;              // pass 1 reads d_up_tab (when it exists) instead of generating
     void release();
 };
 
@@ -96,6 +104,7 @@ int hv_lanczos(const Hubbard &h, bool cvec, const void *f, void *y, void *v, con
 int hv_lanczos_nodot(const Hubbard &h, bool cvec, const void *f, void *y, void *v, const void *w,
                      const double *bdev, int step, hipStream_t st);
 int64_t hv_num_blocks(const Hubbard &h);
+int otf_build_up_table(Hubbard &h, hipStream_t st);
 template <int FLAGS>
 int otf_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout, const void *w, const double *bdev,
                  int step, double *partial, hipStream_t st, int64_t *nblocks);

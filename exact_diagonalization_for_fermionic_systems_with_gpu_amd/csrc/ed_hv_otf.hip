@@ -15,10 +15,13 @@
 // Generating them costs a few integer ops and one address-table lookup (the 2^N-entry
 // rank table, L2-resident) per hop, amortised over W columns (UP) or R rows (DN).
 //
-// Pass 1 (UP term) -- tiles of 32 rows x 8 columns, XCD-aware slab-major order.  Each
-// wavefront owns 8 rows; for each row the 64 lanes evaluate up to 64 operator calls in
-// parallel, a ballot orders the hopping ones, and the (column, value) list lands in LDS.
-// Then lane (row, col) streams x[neighbour row, col] over the list.
+// Pass 1 (UP term) -- tiles of 256/TPR rows x W = TPR*VEC columns (a column slab), slabs
+// split over the 8 XCDs, each XCD walking its slabs one at a time.  For each row the 64
+// lanes of a wavefront evaluate up to 64 operator calls in parallel, a ballot orders the
+// hopping ones, and the (column, value) list lands in LDS.  Then lane (row, col) streams
+// x[neighbour row, col] over the list.  Large sectors visit the rows of a slab in reverse
+// Cuthill-McKee order of the hopping graph, so a row's neighbour rows were mostly read
+// shortly before (L2 hits) instead of being fetched again.
 // Pass 2 (DN term + diagonal + Lanczos epilogue) -- tiles of R rows x 256 columns,
 // XCD-aware row-panel order; each lane walks the call list for its column once and
 // applies every hop to its R rows, starting from the pass-1 partial.
@@ -40,14 +43,16 @@ __global__ __launch_bounds__(256) void k_otf_up(OtfArgs p, int64_t ntiles, int64
     constexpr int ORB = 256 / TPR;
     extern __shared__ double smem_o[];
     const int nops = p.up.nops;
-    TV *s_val = reinterpret_cast<TV *>(smem_o);                       // [ORB][nops]
+    TV *s_tab = reinterpret_cast<TV *>(smem_o);                       // [4 nops] (entry table only)
+    TV *s_val = s_tab + 4 * nops;                                     // [ORB][nops]
     int32_t *s_col = reinterpret_cast<int32_t *>(s_val + ORB * nops); // [ORB][nops]
     __shared__ int s_cnt[ORB];
+    __shared__ int32_t s_row[ORB];
     const int64_t b = blockIdx.x;
     const int64_t t = per_xcd ? (b & 7) * per_xcd + (b >> 3) : b; // per_xcd = 0: plain order
     if (t >= ntiles) return; // uniform over the workgroup
     const double sc = SCALE ? 1.0 / bdev[step] : 1.0;
-    otf_up_body<V, TV, SCALE, TPR, VEC>(p, t, nrp, x, out, sc, s_val, s_col, s_cnt);
+    otf_up_body<V, TV, SCALE, TPR, VEC>(p, t, nrp, x, out, sc, s_val, s_col, s_cnt, s_row, s_tab);
 }
 
 template <typename V, typename TV, int R, int FLAGS, bool MASKED>
@@ -179,6 +184,47 @@ __global__ __launch_bounds__(256) void k_otf_dn2(OtfArgs p, int64_t ntiles, int6
     }
 }
 
+// Pass-1 entry table, one thread per up row u: the row's (column << 8 | value code) words in
+// call order (the order op_entry visits them), then 0xffffffff padding up to nops words.
+__global__ __launch_bounds__(256) void k_up_entries(OtfArgs p, uint32_t *__restrict__ tab, int32_t *__restrict__ cnt)
+{
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (u >= p.dim_up) return;
+    const int nops = p.up.nops;
+    const uint32_t s = (uint32_t)p.state_up[u];
+    uint32_t *row = tab + u * nops;
+    int n = 0;
+    for (int o = 0; o < nops; ++o) {
+        const OtfOp op = p.up.ops[o];
+        const uint32_t oi = (s >> op.i) & 1u, oj = (s >> op.j) & 1u;
+        uint32_t e = 0xffffffffu;
+        if (op.kind == ED_OP_DENSITY) {
+            if (oi) e = ((uint32_t)u << 8) | (uint32_t)(4 * o);
+        } else if (op.kind == ED_OP_DENSITY2) {
+            if (oi & oj) e = ((uint32_t)u << 8) | (uint32_t)(4 * o);
+        } else if (oi != oj) {
+            const uint32_t code = (oi ? 0u : 2u) | (uint32_t)(__popc(s & op.between) & 1);
+            e = ((uint32_t)p.up.addr[s ^ op.m] << 8) | (uint32_t)(4 * o) | code;
+        }
+        if (e != 0xffffffffu) row[n++] = e;
+    }
+    for (int j = n; j < nops; ++j) row[j] = 0xffffffffu;
+    cnt[u] = n;
+}
+
+int otf_build_up_table(Hubbard &h, hipStream_t st)
+{
+    const OtfArgs &p = h.otf;
+    if (p.up.nops == 0 || p.up.nops > 64 || p.dim_up >= ((int64_t)1 << 24)) return ED_OK; // generated on the fly
+    ED_TRY(dmalloc((void **)&h.d_up_tab, sizeof(uint32_t) * (size_t)p.dim_up * p.up.nops));
+    ED_TRY(dmalloc((void **)&h.d_up_cnt, sizeof(int32_t) * (size_t)p.dim_up));
+    hipLaunchKernelGGL(k_up_entries, dim3((unsigned)((p.dim_up + 255) / 256)), dim3(256), 0, st, p, h.d_up_tab,
+                       h.d_up_cnt);
+    ED_CHECK_LAUNCH();
+    ED_HIP(hipStreamSynchronize(st));
+    return ED_OK;
+}
+
 struct Plan3 {
     int R, W, TPR, VEC, K;
     bool dn2;
@@ -191,9 +237,11 @@ static Plan3 plan3(const Hubbard &h)
     const int64_t du = h.args.dim_up, dd = h.args.dim_dn;
     // slab width W = TPR threads per row x VEC columns per thread; VEC = 2 (16 B loads per lane)
     // needs an even dim_dn so that every row chunk stays 16 B aligned
-    const int w = h.hv_w;
-    P.TPR = (w == 16 || w == 32 || w == 64) ? w : (w == 128 ? 64 : (w == 256 ? 128 : (w == 512 ? 256 : 8)));
-    P.VEC = (w >= 128 && dd % 2 == 0) ? 2 : 1;
+    int w = 8;
+    for (int c = 16; c <= 512; c *= 2)
+        if (h.hv_w >= c) w = c;
+    P.VEC = (w >= 16 && dd % 2 == 0) ? 2 : 1;
+    P.TPR = std::min(w / P.VEC, 256);
     P.W = P.TPR * P.VEC;
     P.nslab = (dd + P.W - 1) / P.W;
     P.nrp1 = (du + 256 / P.TPR - 1) / (256 / P.TPR);
@@ -214,7 +262,7 @@ static Plan3 plan3(const Hubbard &h)
     P.K = 1;
     if (P.dn2) {
         P.R = (h.hv_rmax >= 8) ? 8 : 4;
-        P.K = (h.hv_dnk == 2 || h.hv_dnk == 4 || h.hv_dnk == 16) ? h.hv_dnk : 8;
+        P.K = (h.hv_dnk == 1 || h.hv_dnk == 2 || h.hv_dnk == 4 || h.hv_dnk == 16) ? h.hv_dnk : 8;
     }
     P.nrp2 = P.dn2 ? (du + P.R * P.K - 1) / (P.R * P.K) : (du + P.R - 1) / P.R;
     P.tiles2 = P.nrp2 * P.ncb;
@@ -234,18 +282,28 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
     V *upbuf = (FLAGS & HV_WRITEV) ? vout : y;
     constexpr bool SC = (FLAGS & HV_SCALE) != 0;
     if (has_up) {
-        const size_t sh = (size_t)(256 / P.TPR) * h.otf.up.nops * (sizeof(TV) + sizeof(int32_t));
+        const size_t sh = (size_t)(256 / P.TPR + 4) * h.otf.up.nops * sizeof(TV) +
+                          (size_t)(256 / P.TPR) * h.otf.up.nops * sizeof(int32_t);
         // XCD-partitioned slabs (each XCD walks its own range of slabs) or, with hv_upxcd = 0,
         // all XCDs on the same slabs at once (one slab's rows in MALL instead of eight)
         const int64_t pper = h.hv_upxcd ? P.per1 : 0;
         const dim3 g1((unsigned)(h.hv_upxcd ? P.per1 * 8 : P.tiles1));
-#define ED_UP(TPR_, VEC_)                                                                                   \
-    hipLaunchKernelGGL((k_otf_up<V, TV, SC, TPR_, VEC_>), g1, dim3(256), sh, st, h.otf, P.tiles1, pper, P.nrp1, \
-                       x, upbuf, bdev, step)
+        OtfArgs oa = h.otf;
+        oa.up_order = h.hv_uporder ? h.d_up_order : nullptr;
+        oa.up_tab = h.hv_uptab ? h.d_up_tab : nullptr;
+        oa.up_cnt = h.d_up_cnt;
+#define ED_UP(TPR_, VEC_)                                                                                \
+    hipLaunchKernelGGL((k_otf_up<V, TV, SC, TPR_, VEC_>), g1, dim3(256), sh, st, oa, P.tiles1, pper, P.nrp1, x, \
+                       upbuf, bdev, step)
         if (P.VEC == 2) {
-            if (P.TPR == 256) ED_UP(256, 2);
-            else if (P.TPR == 128) ED_UP(128, 2);
-            else ED_UP(64, 2);
+            switch (P.TPR) {
+            case 256: ED_UP(256, 2); break;
+            case 128: ED_UP(128, 2); break;
+            case 64: ED_UP(64, 2); break;
+            case 32: ED_UP(32, 2); break;
+            case 16: ED_UP(16, 2); break;
+            default: ED_UP(8, 2); break;
+            }
         } else {
             switch (P.TPR) {
             case 256: ED_UP(256, 1); break;
@@ -267,6 +325,7 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
                        upbuf, y, vout, w, bdev, step, partial)
         if (P.R == 8) {
             switch (P.K) {
+            case 1: ED_DN2(8, 1); break;
             case 2: ED_DN2(8, 2); break;
             case 4: ED_DN2(8, 4); break;
             case 16: ED_DN2(8, 16); break;
@@ -274,6 +333,7 @@ static int launch_otf(const Hubbard &h, const V *x, V *y, V *vout, const V *w, c
             }
         } else {
             switch (P.K) {
+            case 1: ED_DN2(4, 1); break;
             case 2: ED_DN2(4, 2); break;
             case 4: ED_DN2(4, 4); break;
             case 16: ED_DN2(4, 16); break;

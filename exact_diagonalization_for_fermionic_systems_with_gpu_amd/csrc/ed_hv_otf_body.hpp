@@ -83,8 +83,11 @@ __device__ __forceinline__ bool op_entry(const OtfOp &op, uint32_t s, int32_t se
 // ---------------------------------------------------------------------------------
 // pass 1 body: UP term of logical tile t (slab-major).  A tile is 256/TPR rows x TPR*VEC
 // columns: TPR threads per row, each owning VEC consecutive columns (VEC = 2 loads 16 B per
-// lane for FP64).  Rows are evaluated one per wave (rows r = wave, wave + 4, ...): the lanes
-// take one operator call each, a ballot orders the hopping ones into LDS.
+// lane for FP64).  Tile row r is up row up_order[r0 + r] (or r0 + r without an order).  The
+// (column, value) list of each row comes from the precomputed entry table when there is one
+// (a coalesced copy into LDS); else rows are evaluated one per wave (rows r = wave, wave + 4,
+// ...): the lanes take one operator call each, a ballot orders the hopping ones into LDS, and
+// each wave loads the states of all its rows before it looks up any target address.
 // ---------------------------------------------------------------------------------
 template <typename V, int VEC>
 struct VecLoad;
@@ -122,34 +125,75 @@ struct VecLoad<cplx, 2> {
 
 template <typename V, typename TV, bool SCALE, int TPR, int VEC = 1>
 __device__ __forceinline__ void otf_up_body(const OtfArgs &p, int64_t t, int64_t nrp, const V *__restrict__ x, V *out,
-                                            double sc, TV *s_val, int32_t *s_col, int *s_cnt)
+                                            double sc, TV *s_val, int32_t *s_col, int *s_cnt, int32_t *s_row,
+                                            TV *s_tab)
 {
-    constexpr int ORB = 256 / TPR, OW = TPR * VEC;
+    constexpr int ORB = 256 / TPR, OW = TPR * VEC, RPW = (ORB + 3) / 4;
     const int nops = p.up.nops;
     const int64_t slab = t / nrp, rp = t - slab * nrp;
     const int64_t u0 = rp * ORB;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int r = wave; r < ORB; r += 4) {
-        const int64_t u = u0 + r;
-        bool has = false;
-        int32_t col = 0;
-        TV val{};
-        if (u < p.dim_up && lane < nops) {
-            const OtfOp op = p.up.ops[lane];
-            has = op_entry<TV>(op, (uint32_t)p.state_up[u], (int32_t)u, p.up.addr, col, val);
+    if (p.up_tab) {
+        // precomputed entries: up row u holds up_cnt[u] words (column << 8 | value code) at
+        // up_tab[u * nops ...], value code 4 o + k -> k = 0: vf, 1: -vf, 2: vb, 3: -vb
+        for (int q = threadIdx.x; q < 4 * nops; q += 256) {
+            const OtfOp op = p.up.ops[q >> 2];
+            const int k = q & 3;
+            const TV v = (k < 2) ? tv_make<TV>(op.vf_re, op.vf_im) : tv_make<TV>(op.vb_re, op.vb_im);
+            s_tab[q] = (k & 1) ? tv_neg<TV>(v) : v;
         }
-        const uint64_t bal = __ballot(has);
-        if (has) {
-            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-            s_col[r * nops + pos] = col;
-            s_val[r * nops + pos] = val;
+        if (threadIdx.x < ORB) {
+            const int64_t q = u0 + threadIdx.x;
+            const int32_t row = q < p.dim_up ? (p.up_order ? p.up_order[q] : (int32_t)q) : -1;
+            s_row[threadIdx.x] = row;
+            s_cnt[threadIdx.x] = row >= 0 ? p.up_cnt[row] : 0;
         }
-        if (lane == 0) s_cnt[r] = __popcll(bal);
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int r = wave; r < ORB; r += 4) {
+            const int32_t row = s_row[r];
+            if (row >= 0 && lane < s_cnt[r]) {
+                const uint32_t e = p.up_tab[(int64_t)row * nops + lane];
+                s_col[r * nops + lane] = (int32_t)(e >> 8);
+                s_val[r * nops + lane] = s_tab[e & 0xffu];
+            }
+        }
+    } else {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        OtfOp op{};
+        if (lane < nops) op = p.up.ops[lane];
+        int32_t ur[RPW];
+        uint32_t st[RPW];
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            const int64_t q = u0 + wave + 4 * k;
+            ur[k] = -1;
+            if (wave + 4 * k < ORB && q < p.dim_up) ur[k] = p.up_order ? p.up_order[q] : (int32_t)q;
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) st[k] = ur[k] >= 0 ? (uint32_t)p.state_up[ur[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            const int r = wave + 4 * k;
+            bool has = false;
+            int32_t col = 0;
+            TV val{};
+            if (ur[k] >= 0 && lane < nops) has = op_entry<TV>(op, st[k], ur[k], p.up.addr, col, val);
+            const uint64_t bal = __ballot(has);
+            if (has) {
+                const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+                s_col[r * nops + pos] = col;
+                s_val[r * nops + pos] = val;
+            }
+            if (lane == 0 && r < ORB) {
+                s_cnt[r] = __popcll(bal);
+                s_row[r] = ur[k];
+            }
+        }
     }
     __syncthreads();
     const int r = threadIdx.x / TPR, c = (threadIdx.x % TPR) * VEC;
-    const int64_t u = u0 + r, d = slab * OW + c;
-    if (u >= p.dim_up || d >= p.dim_dn) return;
+    const int64_t u = s_row[r], d = slab * OW + c;
+    if (u < 0 || d >= p.dim_dn) return;
     const int n = s_cnt[r];
     const int32_t *cl = s_col + r * nops;
     const TV *vl = s_val + r * nops;

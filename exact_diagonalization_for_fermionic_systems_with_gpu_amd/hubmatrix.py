@@ -96,7 +96,8 @@ class Hubbard:
         call("ed_hubbard_autotune", self.handle, _is_complex(x), _raw(x), _raw(y), int(reps), C.byref(t),
              stream_ptr())
         return {"slab_width": t.slab_width, "dn_cached": t.dn_cached, "dn_panels": t.dn_panels,
-                "dn_rows": t.dn_rows, "ms_per_apply": t.ms_per_apply}
+                "dn_rows": t.dn_rows, "row_order": t.row_order, "up_table": t.up_table,
+                "ms_per_apply": t.ms_per_apply}
 
     def apply(self, x: torch.Tensor) -> torch.Tensor:
         y = torch.empty_like(x)

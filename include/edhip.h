@@ -106,11 +106,15 @@ int ed_hubbard_info(ed_hubbard h, int64_t *dim_up, int64_t *dim_dn, int *complex
 int ed_hubbard_apply(ed_hubbard h, int complex_vectors, const void *d_x, void *d_y, void *stream);
 
 /* Tile shapes of the on-the-fly H.v (no counterpart in the reference, whose kernel has one
- * launch shape, HubMatrix_GPU.cuh:54-61).  slab_width: columns per pass-1 tile (64, 128, 256;
- * 128 and up load 16 B per lane); dn_cached: pass 2 keeps per-column call descriptors in LDS
- * over dn_panels panels of dn_rows rows (else it regenerates them per panel). */
+ * launch shape, HubMatrix_GPU.cuh:54-61).  slab_width: columns per pass-1 tile (32 to 256;
+ * 16 B per lane when dim_dn is even); row_order: pass 1 visits the up rows in reverse
+ * Cuthill-McKee order of the up hopping graph (built at creation for vectors of 1 GiB and more)
+ * instead of ascending rank; up_table: pass 1 reads each row's hops from a table built at
+ * creation (dim_up x calls words) instead of generating them per slab; dn_cached: pass 2 keeps
+ * per-column call descriptors in LDS over dn_panels panels of dn_rows rows (else it regenerates
+ * them per panel). */
 typedef struct {
-    int slab_width, dn_cached, dn_panels, dn_rows;
+    int slab_width, dn_cached, dn_panels, dn_rows, row_order, up_table;
     double ms_per_apply;
 } ed_hv_tuning;
 /* Autotuner: times each candidate shape with `reps` applications of H to d_x (into d_y, which is

@@ -113,18 +113,24 @@ def _hv_cases():
     }
 
 
-_OTF_ENV = {"otf": {}, "otf_w16": {"ED_HV_W": "16"}, "otf_w128": {"ED_HV_W": "128"},
-            "otf_w256": {"ED_HV_W": "256"}, "otf_w512": {"ED_HV_W": "512"},
+_OTF_ENV = {"otf": {}, "otf_w8": {"ED_HV_W": "8"}, "otf_w16": {"ED_HV_W": "16"}, "otf_w32": {"ED_HV_W": "32"},
+            "otf_w64": {"ED_HV_W": "64"}, "otf_w256": {"ED_HV_W": "256"}, "otf_w512": {"ED_HV_W": "512"},
+            "otf_rcm": {"ED_HV_UPORDER": "1"}, "otf_rcm_w16": {"ED_HV_UPORDER": "1", "ED_HV_W": "16"},
+            "otf_rcm_w32": {"ED_HV_UPORDER": "1", "ED_HV_W": "32"},
+            "otf_notab": {"ED_HV_UPTAB": "0"}, "otf_notab_rcm_w16": {"ED_HV_UPTAB": "0", "ED_HV_UPORDER": "1",
+                                                                  "ED_HV_W": "16"},
             "otf_nomask_r4": {"ED_HV_DNMASK": "0", "ED_HV_RMAX": "4"}, "otf_r1": {"ED_HV_RMAX": "1"},
-            "otf_dn1": {"ED_HV_DN2": "0"}, "otf_k4": {"ED_HV_DNK": "4"}}
+            "otf_dn1": {"ED_HV_DN2": "0"}, "otf_k4": {"ED_HV_DNK": "4"}, "otf_k1": {"ED_HV_DNK": "1"},
+            "otf_k1_r4": {"ED_HV_DNK": "1", "ED_HV_RMAX": "4"}}
 
 
 @pytest.mark.parametrize("path", list(_OTF_ENV) + ["tables", "rowkernel"])
 @pytest.mark.parametrize("name", list(_hv_cases().keys()))
 def test_hv_parity(name, path, monkeypatch):
-    """otf*: descriptors generated from the call list (default W = 64 columns per UP slab, masked
-    DN loads; the variants change the slab width -- 128 and more use 16 B loads per lane -- the
-    DN row panel and the masking); tables: two-pass kernels on the ELL tables; rowkernel: the
+    """otf*: descriptors generated from the call list (default W = 128 columns per UP slab, DN
+    descriptors cached over 2 panels of 8 rows; the variants change the slab width -- 16 B loads
+    per lane from W = 16 when dim_dn is even -- the UP row order (reverse Cuthill-McKee), the DN
+    panels and the masking); tables: two-pass kernels on the ELL tables; rowkernel: the
     single-pass row kernel (ED_HV_ALGO=1)."""
     N, Qu, Qd, ops, U, mask, Uvec, cplx = _hv_cases()[name]
     print(f"hv {name} {path}", flush=True)
@@ -263,9 +269,12 @@ def test_s2_parity_and_singlet():
     assert np.max(np.abs(y - yo)) <= 1e-14 * np.max(np.abs(yo))
 
 
-def test_autotune_keeps_results_bitwise():
+@pytest.mark.parametrize("order", ["0", "1"])
+def test_autotune_keeps_results_bitwise(order, monkeypatch):
     """The tile-shape autotuner picks among kernels that add the same terms in the same order:
-    H.v after tuning equals H.v before, bit for bit, for real and complex vectors."""
+    H.v after tuning equals H.v before, bit for bit, for real and complex vectors (order 1: the
+    UP row-order candidates are in the search)."""
+    monkeypatch.setenv("ED_HV_UPORDER", order)
     ops = L.hubbard_ops(L.KAGOME12, t=-1.0)
     gu, gd = gpu_species(12, 6, ops), gpu_species(12, 6, ops)
     H = Hubbard(gu, gd, 4.0)
@@ -273,7 +282,7 @@ def test_autotune_keeps_results_bitwise():
     y0 = H.apply(x)
     tmp = torch.empty_like(x)
     best = H.autotune(x, tmp, reps=2)
-    assert best["slab_width"] in (64, 128, 256) and best["ms_per_apply"] > 0
+    assert best["slab_width"] in (32, 64, 128, 256) and best["ms_per_apply"] > 0
     y1 = H.apply(x)
     assert torch.equal(y0, y1)
     xc = x.to(torch.complex128) * (1 + 0.5j)
