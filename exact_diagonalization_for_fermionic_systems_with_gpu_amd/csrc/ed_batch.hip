@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
     const int64_t d = cb * g.CW + c;
     const int gi = g.cbase + k;
     const bool live = k < g.ncopy && d < dd && active[gi];
-    const double sc = live ? 1.0 / bdev[(int64_t)gi * ldb + *dstep] : 0.0;
+    const double sc = live ? 1.0 / bdev[(int64_t)*dstep * ldb + gi] : 0.0;
     const V *xv = reinterpret_cast<const V *>(x + g.voff);
     V *yv = reinterpret_cast<V *>(y + g.voff);
     const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k, du32 = (uint32_t)d;
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
         const int64_t u0 = (rp * kp + pn) * GR;
         if (u0 >= du) break;
         const int step = *dstep;
-        const double bN = bdev[(int64_t)gi * ldb + step], sc = 1.0 / bN;
-        const double scp = AXPY ? 1.0 / bdev[(int64_t)gi * ldb + step - 1] : 0.0;
+        const double bN = bdev[(int64_t)step * ldb + gi], sc = 1.0 / bN;
+        const double scp = AXPY ? 1.0 / bdev[(int64_t)(step - 1) * ldb + gi] : 0.0;
         const V *xv = reinterpret_cast<const V *>(x + g.voff);
         const V *xpv = reinterpret_cast<const V *>(xp + g.voff);
         V *yy = reinterpret_cast<V *>(y + g.voff);
@@ -271,8 +271,8 @@ __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr,
     const int gi = g.cbase + k;
     double s = 0.0;
     if (k < g.ncopy && active[gi]) {
-        const double a = -adev[(int64_t)gi * lda + *dstep];
-        const double sc = 1.0 / bdev[(int64_t)gi * lda + *dstep];
+        const double a = -adev[(int64_t)*dstep * lda + gi];
+        const double sc = 1.0 / bdev[(int64_t)*dstep * lda + gi];
         const int NC = cvec ? 2 : 1;
 #pragma unroll
         for (int j = 0; j < GCH; ++j) {
@@ -296,7 +296,8 @@ __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr,
     }
 }
 
-// one block per global copy: dst[copy*ld + step(+1)] = sum (which 0) or sqrt(sum) (which 1)
+// one block per global copy: dst[step(+1)*ld + copy] = sum (which 0) or sqrt(sum) (which 1);
+// a and b are stored step-major, so the entries added since a check are one contiguous range
 __global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ gr, const int *__restrict__ cgroup,
                                                      const int *__restrict__ active, const double *__restrict__ partial,
                                                      int which, double *dst, int ld, const int *dstep)
@@ -312,8 +313,8 @@ __global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ 
     s = block_sum<1024>(s, red);
     if (threadIdx.x == 0) {
         const int step = *dstep;
-        if (which == 0) dst[(int64_t)gi * ld + step] = s;
-        else dst[(int64_t)gi * ld + step + 1] = sqrt(s);
+        if (which == 0) dst[(int64_t)step * ld + gi] = s;
+        else dst[(int64_t)(step + 1) * ld + gi] = sqrt(s);
     }
 }
 
@@ -621,6 +622,9 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
     double *buf[3] = {nullptr, nullptr, nullptr}, *partial = nullptr, *adev = nullptr, *bdev = nullptr,
            *tmp = nullptr;
     int *active = nullptr, *dstep = nullptr;
+    // pinned host staging for every transfer of the solve (a, b readback; b_0, flags, step upload)
+    double *pa = nullptr, *pb = nullptr;
+    int *pact = nullptr;
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     int s = ED_OK;
@@ -637,6 +641,9 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         dfree(active);
         dfree(dstep);
         dfree(tmp);
+        if (pa) (void)hipHostFree(pa);
+        if (pb) (void)hipHostFree(pb);
+        if (pact) (void)hipHostFree(pact);
         (void)hipStreamDestroy(st);
         (void)hipEventDestroy(ev);
     };
@@ -651,6 +658,12 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             (s = dmalloc((void **)&active, sizeof(int) * nc)) || (s = dmalloc((void **)&dstep, sizeof(int) * 2)) ||
             (s = dmalloc((void **)&tmp, sizeof(double) * (size_t)maxe * (B->cvec ? 2 : 1))))
             break;
+        if (hipHostMalloc((void **)&pa, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&pb, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(nc + 2), hipHostMallocDefault) != hipSuccess) {
+            s = fail(ED_ENOMEM, "ed_batch_lowest: pinned staging allocation failed");
+            break;
+        }
         // start vectors: each copy gets the normalised GenRand vector of its sector length (the
         // padding copies stay zero and inactive); b_0 = |f| as in the single-instance driver
         const int NC = B->cvec ? 2 : 1;
@@ -671,20 +684,28 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             }
         }
         if (s) break;
+        // host copies per copy (row c: a_0.., b_0..); on the device a and b are step-major
         std::vector<double> hb(nc * (size_t)ld, 0.0), ha(nc * (size_t)ld, 0.0);
         for (int c = 0; c < nc; ++c) hb[(size_t)c * ld] = hb0[c] > 0.0 ? hb0[c] : 1.0;
-        std::vector<int> hact(nc, 0), hstep2 = {0, 0};
+        std::vector<int> hact(nc, 0);
         for (int c = 0; c < nc; ++c) hact[c] = B->inst_of[c] >= 0 ? 1 : 0;
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
             s = fail(ED_EHIP, "ed_batch_lowest: start vectors failed");
             break;
         }
-        if (hipMemcpyAsync(bdev, hb.data(), sizeof(double) * nc * ld, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(active, hact.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(dstep, hstep2.data(), sizeof(int) * 2, hipMemcpyHostToDevice, st)) {
+        for (int c = 0; c < nc; ++c) {
+            pb[c] = hb[(size_t)c * ld];
+            pact[c] = hact[c];
+        }
+        pact[nc] = pact[nc + 1] = 0;
+        if (hipMemcpyAsync(bdev, pb, sizeof(double) * nc, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(dstep, pact + nc, sizeof(int) * 2, hipMemcpyHostToDevice, st) ||
+            hipStreamSynchronize(st)) {
             s = fail(ED_EHIP, "ed_batch_lowest: upload failed");
             break;
         }
+        int got_a = 0, got_b = 1; // step rows of a and b already on the host
         std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0);
         std::vector<double> anorm(nc, 0.0), z, ev_c(nc, 0.0);
         int remaining = 0;
@@ -700,11 +721,11 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                 const double *X = buf[(ph + k) % 3], *XP = buf[(ph + k + 2) % 3];
                 double *Y = buf[(ph + k + 1) % 3];
                 const bool first = (n0 + k) == 0;
-                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, adev, bdev, ld,
+                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, adev, bdev, nc,
                                                                      dstep, first, st)
                                            : enqueue_step<cplx, double>(*B, X, XP, Y, partial, active, adev, bdev,
-                                                                       ld, dstep, first, st))
-                                : enqueue_step<double, double>(*B, X, XP, Y, partial, active, adev, bdev, ld, dstep,
+                                                                       nc, dstep, first, st))
+                                : enqueue_step<double, double>(*B, X, XP, Y, partial, active, adev, bdev, nc, dstep,
                                                                first, st);
                 if (r) return r;
             }
@@ -737,16 +758,22 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             }
             N += chunk;
             phase = (phase + chunk) % 3;
-            // convergence check of every active copy (same test as ed_lanczos_eigen)
-            // only the first N (a) and N + 1 (b) entries of each copy's row are new
-            const size_t pitch = sizeof(double) * ld;
-            if (hipMemcpy2DAsync(ha.data(), pitch, adev, pitch, sizeof(double) * N, nc, hipMemcpyDeviceToHost, st) ||
-                hipMemcpy2DAsync(hb.data(), pitch, bdev, pitch, sizeof(double) * (N + 1), nc, hipMemcpyDeviceToHost,
-                                 st) ||
+            // convergence check of every active copy (same test as ed_lanczos_eigen); the
+            // a rows got_a..N-1 and b rows got_b..N are new, one contiguous range each
+            if (hipMemcpyAsync(pa + (size_t)got_a * nc, adev + (size_t)got_a * nc, sizeof(double) * nc * (N - got_a),
+                               hipMemcpyDeviceToHost, st) ||
+                hipMemcpyAsync(pb + (size_t)got_b * nc, bdev + (size_t)got_b * nc,
+                               sizeof(double) * nc * (N + 1 - got_b), hipMemcpyDeviceToHost, st) ||
                 hipStreamSynchronize(st)) {
                 s = fail(ED_EHIP, "ed_batch_lowest: readback failed");
                 break;
             }
+            for (int k = got_a; k < N; ++k)
+                for (int c = 0; c < nc; ++c) ha[(size_t)c * ld + k] = pa[(size_t)k * nc + c];
+            for (int k = got_b; k <= N; ++k)
+                for (int c = 0; c < nc; ++c) hb[(size_t)c * ld + k] = pb[(size_t)k * nc + c];
+            got_a = N;
+            got_b = N + 1;
             bool changed = false;
             for (int c = 0; c < nc; ++c) {
                 if (done[c]) continue;
@@ -778,9 +805,12 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     changed = true;
                 }
             }
-            if (changed && hipMemcpyAsync(active, hact.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
-                s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
-                break;
+            if (changed) { // st is idle here (synchronised above), so pact is free to rewrite
+                for (int c = 0; c < nc; ++c) pact[c] = hact[c];
+                if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
+                    s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
+                    break;
+                }
             }
         }
         if (s) break;
