@@ -320,13 +320,24 @@ __global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ 
 
 __global__ void k_g_bump(int *dstep) { dstep[0] += 1; }
 
-// start vectors: copy k of a group <- src (its instance's GenRand vector); NC doubles per element
-__global__ __launch_bounds__(256) void k_g_scatter(int64_t nelem, int G, int k, int NC, const double *__restrict__ src,
-                                                   double *__restrict__ dst)
+// start vectors: copies 0..ncopy-1 of a group <- src (their sector's GenRand vector); NC
+// doubles per element
+__global__ __launch_bounds__(256) void k_g_scatter(int64_t nelem, int G, int ncopy, int NC,
+                                                   const double *__restrict__ src, double *__restrict__ dst)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= nelem) return;
-    for (int q = 0; q < NC; ++q) dst[(e * G + k) * NC + q] = src[e * NC + q];
+    for (int k = 0; k < ncopy; ++k)
+        for (int q = 0; q < NC; ++q) dst[(e * G + k) * NC + q] = src[e * NC + q];
+}
+
+// b_0 row (step 0 of the step-major b array) of one group: |f| for its copies, 1 for padding
+__global__ void k_g_b0(int cbase, int ncopy, int G, const double *__restrict__ nrm, double *__restrict__ bdev)
+{
+    const int k = threadIdx.x;
+    if (k >= G) return;
+    const double v = *nrm;
+    bdev[cbase + k] = (k < ncopy && v > 0.0) ? v : 1.0;
 }
 
 struct Batch {
@@ -623,7 +634,8 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
            *tmp = nullptr;
     int *active = nullptr, *dstep = nullptr;
     // pinned host staging for every transfer of the solve (a, b readback; b_0, flags, step upload)
-    double *pa = nullptr, *pb = nullptr;
+    double *pa = nullptr, *pb = nullptr, *dpart = nullptr; // dpart: start-vector norm partials + result
+    int64_t npart = 0;
     int *pact = nullptr;
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
@@ -641,6 +653,7 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         dfree(active);
         dfree(dstep);
         dfree(tmp);
+        dfree(dpart);
         if (pa) (void)hipHostFree(pa);
         if (pb) (void)hipHostFree(pb);
         if (pact) (void)hipHostFree(pact);
@@ -652,22 +665,24 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         if (s) break;
         int64_t maxe = 1;
         for (const GGroup &g : B->host) maxe = std::max(maxe, g.nelem);
+        npart = blas_num_blocks(maxe * (B->cvec ? 2 : 1));
         if ((s = dmalloc((void **)&partial, sizeof(double) * (size_t)std::max<int64_t>(B->pcap, 1))) ||
             (s = dmalloc((void **)&adev, sizeof(double) * (size_t)nc * ld)) ||
             (s = dmalloc((void **)&bdev, sizeof(double) * (size_t)nc * ld)) ||
             (s = dmalloc((void **)&active, sizeof(int) * nc)) || (s = dmalloc((void **)&dstep, sizeof(int) * 2)) ||
-            (s = dmalloc((void **)&tmp, sizeof(double) * (size_t)maxe * (B->cvec ? 2 : 1))))
+            (s = dmalloc((void **)&tmp, sizeof(double) * (size_t)maxe * (B->cvec ? 2 : 1))) ||
+            (s = dmalloc((void **)&dpart, sizeof(double) * (size_t)(npart + 1))))
             break;
         if (hipHostMalloc((void **)&pa, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&pb, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(nc + 2), hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(2 * nc + 2), hipHostMallocDefault) != hipSuccess) {
             s = fail(ED_ENOMEM, "ed_batch_lowest: pinned staging allocation failed");
             break;
         }
         // start vectors: each copy gets the normalised GenRand vector of its sector length (the
-        // padding copies stay zero and inactive); b_0 = |f| as in the single-instance driver
+        // padding copies stay zero and inactive); b_0 = |f| as in the single-instance driver,
+        // reduced and written into the b_0 row on the device (no host round trip per group)
         const int NC = B->cvec ? 2 : 1;
-        std::vector<double> hb0(nc, 0.0);
         for (const GGroup &g : B->host) {
             if (s) break;
             if (hipMemsetAsync(buf[0] + g.voff, 0, sizeof(double) * (size_t)g.nelem * g.G * NC, st) != hipSuccess) {
@@ -675,37 +690,33 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                 break;
             }
             if ((s = ed_random_vector(g.nelem, seed, B->cvec, tmp, 1, st))) break;
-            double out2[2];
-            if ((s = ed_dot(g.nelem, B->cvec, tmp, tmp, out2, st))) break;
-            for (int k = 0; k < g.ncopy; ++k) {
-                hipLaunchKernelGGL(k_g_scatter, dim3((unsigned)((g.nelem + 255) / 256)), dim3(256), 0, st, g.nelem,
-                                   g.G, k, NC, (const double *)tmp, buf[0] + g.voff);
-                hb0[g.cbase + k] = std::sqrt(out2[0]);
-            }
+            int64_t nb = 0;
+            if ((s = dot_partials(g.nelem * NC, tmp, tmp, dpart, &nb, st)) ||
+                (s = finalize_sum(dpart, nb, dpart + npart, 1, st)))
+                break;
+            hipLaunchKernelGGL(k_g_b0, dim3(1), dim3(64), 0, st, g.cbase, g.ncopy, g.G, (const double *)(dpart + npart),
+                               bdev);
+            hipLaunchKernelGGL(k_g_scatter, dim3((unsigned)((g.nelem + 255) / 256)), dim3(256), 0, st, g.nelem, g.G,
+                               g.ncopy, NC, (const double *)tmp, buf[0] + g.voff);
         }
         if (s) break;
         // host copies per copy (row c: a_0.., b_0..); on the device a and b are step-major
         std::vector<double> hb(nc * (size_t)ld, 0.0), ha(nc * (size_t)ld, 0.0);
-        for (int c = 0; c < nc; ++c) hb[(size_t)c * ld] = hb0[c] > 0.0 ? hb0[c] : 1.0;
         std::vector<int> hact(nc, 0);
         for (int c = 0; c < nc; ++c) hact[c] = B->inst_of[c] >= 0 ? 1 : 0;
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        if (hipGetLastError() != hipSuccess) {
             s = fail(ED_EHIP, "ed_batch_lowest: start vectors failed");
             break;
         }
-        for (int c = 0; c < nc; ++c) {
-            pb[c] = hb[(size_t)c * ld];
-            pact[c] = hact[c];
-        }
-        pact[nc] = pact[nc + 1] = 0;
-        if (hipMemcpyAsync(bdev, pb, sizeof(double) * nc, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(dstep, pact + nc, sizeof(int) * 2, hipMemcpyHostToDevice, st) ||
+        for (int c = 0; c < nc; ++c) pact[c] = hact[c];
+        pact[2 * nc] = pact[2 * nc + 1] = 0;
+        if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(dstep, pact + 2 * nc, sizeof(int) * 2, hipMemcpyHostToDevice, st) ||
             hipStreamSynchronize(st)) {
             s = fail(ED_EHIP, "ed_batch_lowest: upload failed");
             break;
         }
-        int got_a = 0, got_b = 1; // step rows of a and b already on the host
+
         std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0);
         std::vector<double> anorm(nc, 0.0), z, ev_c(nc, 0.0);
         int remaining = 0;
@@ -731,54 +742,72 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             }
             return ED_OK;
         };
-        while (remaining > 0 && N < maxN) {
+        // one chunk runs ahead of the check: the GPU steps chunk k + 1 while the host checks
+        // chunk k, and flags cleared at that check take effect from chunk k + 2 (a copy may
+        // take up to ten steps past its stop; they touch only its own vectors and rows)
+        hipEvent_t evr[2] = {nullptr, nullptr};
+        for (int k = 0; k < 2 && !s; ++k)
+            if (hipEventCreateWithFlags(&evr[k], hipEventDisableTiming) != hipSuccess)
+                s = fail(ED_EHIP, "ed_batch_lowest: event creation failed");
+        int qN[2] = {0, 0}, qslot[2] = {0, 0}, nq = 0, slot = 0, flip = 0, Nc = 0;
+        auto launch_chunk = [&]() -> int {
             const int chunk = std::min(every, maxN - N);
             // the first ten steps contain the step-0 kernel: run them eagerly; later phases
             // are captured once and replayed
             if (use_graph && chunk == every && N > 0) {
                 if (!gexec[phase]) {
-                    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                        s = fail(ED_EHIP, "ed_batch_lowest: capture begin failed");
-                        break;
-                    }
+                    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                        return fail(ED_EHIP, "ed_batch_lowest: capture begin failed");
                     int r = enqueue(phase, N, chunk);
                     hipError_t e = hipStreamEndCapture(st, &graph[phase]);
                     if (r || e != hipSuccess ||
-                        hipGraphInstantiate(&gexec[phase], graph[phase], nullptr, nullptr, 0) != hipSuccess) {
-                        s = r ? r : fail(ED_EHIP, "ed_batch_lowest: graph capture failed");
-                        break;
-                    }
+                        hipGraphInstantiate(&gexec[phase], graph[phase], nullptr, nullptr, 0) != hipSuccess)
+                        return r ? r : fail(ED_EHIP, "ed_batch_lowest: graph capture failed");
                 }
-                if (hipGraphLaunch(gexec[phase], st) != hipSuccess) {
-                    s = fail(ED_EHIP, "ed_batch_lowest: graph launch failed");
-                    break;
-                }
-            } else if ((s = enqueue(phase, N, chunk))) {
-                break;
+                if (hipGraphLaunch(gexec[phase], st) != hipSuccess)
+                    return fail(ED_EHIP, "ed_batch_lowest: graph launch failed");
+            } else if (int r = enqueue(phase, N, chunk)) {
+                return r;
             }
+            const int N0 = N;
             N += chunk;
             phase = (phase + chunk) % 3;
-            // convergence check of every active copy (same test as ed_lanczos_eigen); the
-            // a rows got_a..N-1 and b rows got_b..N are new, one contiguous range each
-            if (hipMemcpyAsync(pa + (size_t)got_a * nc, adev + (size_t)got_a * nc, sizeof(double) * nc * (N - got_a),
+            // the a rows N0..N-1 and b rows N0+1..N this chunk wrote, one contiguous range each
+            if (hipMemcpyAsync(pa + (size_t)N0 * nc, adev + (size_t)N0 * nc, sizeof(double) * nc * chunk,
                                hipMemcpyDeviceToHost, st) ||
-                hipMemcpyAsync(pb + (size_t)got_b * nc, bdev + (size_t)got_b * nc,
-                               sizeof(double) * nc * (N + 1 - got_b), hipMemcpyDeviceToHost, st) ||
-                hipStreamSynchronize(st)) {
-                s = fail(ED_EHIP, "ed_batch_lowest: readback failed");
+                hipMemcpyAsync(pb + (size_t)(N0 + 1) * nc, bdev + (size_t)(N0 + 1) * nc,
+                               sizeof(double) * nc * chunk, hipMemcpyDeviceToHost, st) ||
+                hipEventRecord(evr[slot], st) != hipSuccess)
+                return fail(ED_EHIP, "ed_batch_lowest: readback failed");
+            qN[nq] = N;
+            qslot[nq] = slot;
+            ++nq;
+            slot ^= 1;
+            return ED_OK;
+        };
+        while (!s) {
+            if (remaining > 0 && N < maxN && nq < 2 && (s = launch_chunk())) break;
+            if (nq == 0) break;
+            const int Ne = qN[0];
+            if (hipEventSynchronize(evr[qslot[0]]) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_batch_lowest: readback wait failed");
                 break;
             }
-            for (int k = got_a; k < N; ++k)
-                for (int c = 0; c < nc; ++c) ha[(size_t)c * ld + k] = pa[(size_t)k * nc + c];
-            for (int k = got_b; k <= N; ++k)
-                for (int c = 0; c < nc; ++c) hb[(size_t)c * ld + k] = pb[(size_t)k * nc + c];
-            got_a = N;
-            got_b = N + 1;
+            qN[0] = qN[1];
+            qslot[0] = qslot[1];
+            --nq;
+            for (int k = Nc; k < Ne; ++k)
+                for (int c = 0; c < nc; ++c) {
+                    ha[(size_t)c * ld + k] = pa[(size_t)k * nc + c];
+                    hb[(size_t)c * ld + k + 1] = pb[(size_t)(k + 1) * nc + c];
+                }
+            Nc = Ne;
+            // convergence check of every active copy (same test as ed_lanczos_eigen)
             bool changed = false;
             for (int c = 0; c < nc; ++c) {
                 if (done[c]) continue;
                 const double *a = &ha[(size_t)c * ld], *b = &hb[(size_t)c * ld];
-                const int Nl = N - 1;
+                const int Nl = Ne - 1;
                 int M = Nl;
                 for (int k = processed[c]; k <= Nl; ++k)
                     if (!(b[k + 1] > 0.0) || !std::isfinite(a[k])) {
@@ -805,14 +834,24 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     changed = true;
                 }
             }
-            if (changed) { // st is idle here (synchronised above), so pact is free to rewrite
-                for (int c = 0; c < nc; ++c) pact[c] = hact[c];
-                if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
+            if (changed && remaining > 0) {
+                // two staging halves: the upload from two checks ago has completed (it was
+                // queued before the chunk whose readback event was just waited on)
+                int *pf = pact + (size_t)flip * nc;
+                flip ^= 1;
+                for (int c = 0; c < nc; ++c) pf[c] = hact[c];
+                if (hipMemcpyAsync(active, pf, sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
                     s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
                     break;
                 }
             }
+            if (remaining == 0) break;
         }
+        for (int k = 0; k < 2; ++k)
+            if (evr[k]) {
+                (void)hipStreamSynchronize(st);
+                (void)hipEventDestroy(evr[k]);
+            }
         if (s) break;
         for (int i = 0; i < B->n; ++i) {
             const int c = B->copy_of[i];
