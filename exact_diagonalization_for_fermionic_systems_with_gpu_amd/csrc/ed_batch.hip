@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -637,11 +638,16 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
     double *pa = nullptr, *pb = nullptr, *dpart = nullptr; // dpart: start-vector norm partials + result
     int64_t npart = 0;
     int *pact = nullptr;
+    hipStream_t fs = nullptr; // flag uploads while a chunk runs
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     int s = ED_OK;
     auto cleanup = [&] {
         (void)hipStreamSynchronize(st);
+        if (fs) {
+            (void)hipStreamSynchronize(fs);
+            (void)hipStreamDestroy(fs);
+        }
         for (int k = 0; k < 3; ++k) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
@@ -677,6 +683,10 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             hipHostMalloc((void **)&pb, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(2 * nc + 2), hipHostMallocDefault) != hipSuccess) {
             s = fail(ED_ENOMEM, "ed_batch_lowest: pinned staging allocation failed");
+            break;
+        }
+        if (hipStreamCreateWithFlags(&fs, hipStreamNonBlocking) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_batch_lowest: stream creation failed");
             break;
         }
         // start vectors: each copy gets the normalised GenRand vector of its sector length (the
@@ -717,7 +727,8 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             break;
         }
 
-        std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0);
+        std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0), todo;
+        std::vector<char> stopnow(nc, 0);
         std::vector<double> anorm(nc, 0.0), z, ev_c(nc, 0.0);
         int remaining = 0;
         for (int c = 0; c < nc; ++c) {
@@ -743,13 +754,12 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             return ED_OK;
         };
         // one chunk runs ahead of the check: the GPU steps chunk k + 1 while the host checks
-        // chunk k, and flags cleared at that check take effect from chunk k + 2 (a copy may
-        // take up to ten steps past its stop; they touch only its own vectors and rows)
+        // chunk k; a copy found converged there is switched off during chunk k + 1
         hipEvent_t evr[2] = {nullptr, nullptr};
         for (int k = 0; k < 2 && !s; ++k)
             if (hipEventCreateWithFlags(&evr[k], hipEventDisableTiming) != hipSuccess)
                 s = fail(ED_EHIP, "ed_batch_lowest: event creation failed");
-        int qN[2] = {0, 0}, qslot[2] = {0, 0}, nq = 0, slot = 0, flip = 0, Nc = 0;
+        int qN[2] = {0, 0}, qslot[2] = {0, 0}, nq = 0, slot = 0, Nc = 0;
         auto launch_chunk = [&]() -> int {
             const int chunk = std::min(every, maxN - N);
             // the first ten steps contain the step-0 kernel: run them eagerly; later phases
@@ -802,10 +812,9 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     hb[(size_t)c * ld + k + 1] = pb[(size_t)(k + 1) * nc + c];
                 }
             Nc = Ne;
-            // convergence check of every active copy (same test as ed_lanczos_eigen)
-            bool changed = false;
-            for (int c = 0; c < nc; ++c) {
-                if (done[c]) continue;
+            // convergence check of every active copy (same test as ed_lanczos_eigen), the
+            // copies split over host threads (the Sturm bisection dominates: ~60 passes of M)
+            auto check_copy = [&](int c, std::vector<double> &zc) {
                 const double *a = &ha[(size_t)c * ld], *b = &hb[(size_t)c * ld];
                 const int Nl = Ne - 1;
                 int M = Nl;
@@ -819,28 +828,51 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     else anorm[c] = std::fmax(anorm[c], b[k] + std::fabs(a[k] + b[k + 1]));
                 }
                 processed[c] = M + 1;
-                z.assign(M + 1, 0.0);
+                zc.assign(M + 1, 0.0);
                 double lam;
-                ed_tridiag_lowest(M + 1, a, b + 1, &lam, z.data());
+                ed_tridiag_lowest(M + 1, a, b + 1, &lam, zc.data());
                 ev_c[c] = lam;
                 last[c] = M;
                 bool stop = false;
                 if (M < Nl || !(b[M + 1] > 0.0)) stop = true;
-                else if (std::fabs(b[M + 1] * z[M]) < tol * anorm[c] && std::fabs(z[0]) > tol * anorm[c]) stop = true;
-                if (stop) {
+                else if (std::fabs(b[M + 1] * zc[M]) < tol * anorm[c] && std::fabs(zc[0]) > tol * anorm[c]) stop = true;
+                stopnow[c] = stop ? 1 : 0;
+            };
+            todo.clear();
+            for (int c = 0; c < nc; ++c)
+                if (!done[c]) todo.push_back(c);
+            const int nth = (int)std::min<size_t>(8, (todo.size() + 15) / 16);
+            if (nth <= 1) {
+                for (int c : todo) check_copy(c, z);
+            } else {
+                std::vector<std::thread> th;
+                for (int t = 0; t < nth; ++t)
+                    th.emplace_back([&, t] {
+                        std::vector<double> zc;
+                        for (size_t i = t; i < todo.size(); i += nth) check_copy(todo[i], zc);
+                    });
+                for (auto &x : th) x.join();
+            }
+            bool changed = false;
+            for (int c : todo)
+                if (stopnow[c]) {
                     done[c] = conv[c] = 1;
                     hact[c] = 0;
                     --remaining;
                     changed = true;
                 }
-            }
             if (changed && remaining > 0) {
-                // two staging halves: the upload from two checks ago has completed (it was
-                // queued before the chunk whose readback event was just waited on)
-                int *pf = pact + (size_t)flip * nc;
-                flip ^= 1;
-                for (int c = 0; c < nc; ++c) pf[c] = hact[c];
-                if (hipMemcpyAsync(active, pf, sizeof(int) * nc, hipMemcpyHostToDevice, st)) {
+                // converged copies are switched off at once, on a second stream, while the
+                // chunk that runs ahead is in flight: the kernels it has not started yet skip
+                // them.  A flag only goes from 1 to 0, and a copy's kernels touch only its own
+                // vectors, partial slots and a/b rows, so a kernel that still sees the old
+                // flag does wasted but harmless work.
+                if (hipStreamSynchronize(fs) != hipSuccess) {
+                    s = fail(ED_EHIP, "ed_batch_lowest: flag stream failed");
+                    break;
+                }
+                for (int c = 0; c < nc; ++c) pact[c] = hact[c];
+                if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, fs)) {
                     s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
                     break;
                 }
