@@ -101,11 +101,18 @@ def batched_drude_line():
     gold = json.load(open(os.path.join(REPO, "tests", "golden", "lieb12_drude_oracle.json")))
     Dg = np.array(gold["D"])
     BT.lieb12_drude_sweep(batched=True)  # warm-up (allocations, graph capture paths)
-    E0b, Db, tb, _ = BT.lieb12_drude_sweep(batched=True)
-    E0p, Dp, tp, _ = BT.lieb12_drude_sweep(batched=False)
+    BT.lieb12_drude_sweep(batched=False)
+    # three timed sweeps of each, alternating; the fastest of each is reported
+    runs_b, runs_p = [], []
+    for _ in range(3):
+        runs_b.append(BT.lieb12_drude_sweep(batched=True))
+        runs_p.append(BT.lieb12_drude_sweep(batched=False))
+    E0b, Db, tb, _ = min(runs_b, key=lambda r: r[2])
+    E0p, Dp, tp, _ = min(runs_p, key=lambda r: r[2])
     nz = np.abs(Dg) > 1e-6
-    err = float(max(np.max(np.abs(Db[nz] - Dg[nz]) / np.abs(Dg[nz])), np.max(np.abs(Dp[nz] - Dg[nz]) / np.abs(Dg[nz]))))
-    log(f"[bench] batched Drude sweep {tb:.3f} s, plain {tp:.3f} s, max rel err vs oracle {err:.2e}")
+    err = float(max(max(np.max(np.abs(r[1][nz] - Dg[nz]) / np.abs(Dg[nz])) for r in runs_b + runs_p), 0.0))
+    log(f"[bench] batched Drude sweep {tb:.3f} s (runs {[round(r[2], 3) for r in runs_b]}), plain {tp:.3f} s "
+        f"(runs {[round(r[2], 3) for r in runs_p]}), max rel err vs oracle {err:.2e}")
     return {"value": round(13 / tb, 3), "unit": "Drude weights/s", "plain": round(13 / tp, 3),
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}

@@ -211,27 +211,32 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
 #pragma unroll
         for (int r = 0; r < GR; ++r)
             acc[r] = (g.has_up && u0 + r < du) ? yy[((rowbase[r] + (uint32_t)d) << lg) + ku] : VO<V>::zero();
+        // no branches around the gathers, so the KB x GR loads of a chunk are all in flight: a
+        // column without an element of a call loads its own amplitude (contiguous with its
+        // neighbours' and needed below anyway) and adds 0 times it, which leaves the sum
+        // unchanged bit for bit (finite x; the sums start at +0 or the pass-1 value, never -0)
         constexpr int KB = 4;
+        const TV tzero = tv_make<TV>(0.0, 0.0);
         for (int o0 = 0; o0 < nops; o0 += KB) {
-            uint32_t dk[KB];
+            uint32_t col[KB];
+            TV av[KB];
             V xl[KB][GR];
 #pragma unroll
             for (int q = 0; q < KB; ++q) {
-                dk[q] = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
-                if (dk[q] != 0xffffffffu) {
-                    const uint32_t col = dk[q] >> 8;
-#pragma unroll
-                    for (int r = 0; r < GR; ++r) xl[q][r] = xv[((rowbase[r] + col) << lg) + ku];
-                }
+                const uint32_t e = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
+                const bool on = e != 0xffffffffu;
+                col[q] = on ? (e >> 8) : (uint32_t)d;
+                const TV a = gval<TV>(s_vt, g.G, on ? (e & 0xffu) : 0u, k);
+                av[q] = on ? a : tzero;
             }
 #pragma unroll
-            for (int q = 0; q < KB; ++q) {
-                if (dk[q] != 0xffffffffu) {
-                    const TV a = gval<TV>(s_vt, g.G, dk[q] & 0xffu, k);
+            for (int q = 0; q < KB; ++q)
 #pragma unroll
-                    for (int r = 0; r < GR; ++r) acc[r] = omac(a, VO<V>::scale(sc, xl[q][r]), acc[r]);
-                }
-            }
+                for (int r = 0; r < GR; ++r) xl[q][r] = xv[((rowbase[r] + col[q]) << lg) + ku];
+#pragma unroll
+            for (int q = 0; q < KB; ++q)
+#pragma unroll
+                for (int r = 0; r < GR; ++r) acc[r] = omac(av[q], VO<V>::scale(sc, xl[q][r]), acc[r]);
         }
 #pragma unroll
         for (int r = 0; r < GR; ++r) {
@@ -421,22 +426,15 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
                         (long long)h->otf.dim_up, (long long)h->otf.dim_dn);
         }
         hs[i] = h;
-        opu[i].resize(h->otf.up.nops);
-        opd[i].resize(h->otf.dn.nops);
-        if ((h->otf.up.nops && hipMemcpy(opu[i].data(), h->otf.up.ops, sizeof(OtfOp) * h->otf.up.nops,
-                                         hipMemcpyDeviceToHost) != hipSuccess) ||
-            (h->otf.dn.nops && hipMemcpy(opd[i].data(), h->otf.dn.ops, sizeof(OtfOp) * h->otf.dn.nops,
-                                         hipMemcpyDeviceToHost) != hipSuccess)) {
+        // host copies kept by ed_hubbard_create_ops (no device round trip per instance)
+        if ((int)h->h_ops_up.size() != h->otf.up.nops || (int)h->h_ops_dn.size() != h->otf.dn.nops ||
+            (h->d_uvec && (int)h->h_uvec.size() != h->N)) {
             delete B;
-            return fail(ED_EHIP, "ed_batch_create: operator readback failed");
+            return fail(ED_EINVAL, "ed_batch_create: instance %d has no host copy of its operator", i);
         }
-        if (h->d_uvec) {
-            uv[i].resize(h->N);
-            if (hipMemcpy(uv[i].data(), h->d_uvec, sizeof(double) * h->N, hipMemcpyDeviceToHost) != hipSuccess) {
-                delete B;
-                return fail(ED_EHIP, "ed_batch_create: per-site U readback failed");
-            }
-        }
+        opu[i] = h->h_ops_up;
+        opd[i] = h->h_ops_dn;
+        if (h->d_uvec) uv[i] = h->h_uvec;
     }
     // greedy grouping, first-seen order
     std::vector<std::vector<int>> groups;
@@ -861,7 +859,7 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                     --remaining;
                     changed = true;
                 }
-            if (changed && remaining > 0) {
+            if (changed) { // also when all have stopped: the chunk in flight then skips them
                 // converged copies are switched off at once, on a second stream, while the
                 // chunk that runs ahead is in flight: the kernels it has not started yet skip
                 // them.  A flag only goes from 1 to 0, and a copy's kernels touch only its own

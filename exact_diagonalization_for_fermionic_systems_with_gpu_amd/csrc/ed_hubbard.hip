@@ -702,6 +702,7 @@ extern "C" int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *
         if (s == ED_OK) {
             double tmp[32] = {0};
             for (int i = 0; i < N; ++i) tmp[i] = Uvec[i];
+            h->h_uvec.assign(tmp, tmp + N);
             if (hipMemcpy(h->d_uvec, tmp, sizeof tmp, hipMemcpyHostToDevice) != hipSuccess)
                 s = fail(ED_EHIP, "ed_hubbard_create: Uvec upload failed");
         }
@@ -1014,6 +1015,15 @@ extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int n
         if ((s = otf_prepare(o.up, ops_up, nops_up, h->d_addr_up, nullptr)) ||
             (s = otf_prepare(o.dn, ops_dn, nops_dn, h->d_addr_dn, nullptr)))
             break;
+        h->h_ops_up.resize((size_t)nops_up);
+        h->h_ops_dn.resize((size_t)nops_dn);
+        if ((nops_up && hipMemcpy(h->h_ops_up.data(), o.up.ops, sizeof(OtfOp) * nops_up, hipMemcpyDeviceToHost) !=
+                            hipSuccess) ||
+            (nops_dn && hipMemcpy(h->h_ops_dn.data(), o.dn.ops, sizeof(OtfOp) * nops_dn, hipMemcpyDeviceToHost) !=
+                            hipSuccess)) {
+            s = fail(ED_EHIP, "ed_hubbard_create_ops: operator readback failed");
+            break;
+        }
         int algo = (nops_up <= 64 && nops_dn <= 64) ? 3 : h->algo;
         if (const char *e = getenv("ED_HV_ALGO")) algo = atoi(e);
         if (algo == 3 && (nops_up > 64 || nops_dn > 64)) algo = 2;

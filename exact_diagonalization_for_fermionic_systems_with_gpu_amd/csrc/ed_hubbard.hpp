@@ -1,6 +1,7 @@
 // ed_hubbard.hpp -- internal types of the Hubbard operator (not part of the C ABI).
 #pragma once
 
+#include <vector>
 #include "ed_common.hpp"
 
 namespace ed {
@@ -95,6 +96,8 @@ struct Hubbard {
     uint32_t *d_up_tab = nullptr;  // pass-1 entry table in visiting order (dim_up x nops words)
     int32_t *d_up_cnt = nullptr;
     int hv_uptab = 1;              // pass 1 reads d_up_tab (when it exists) instead of generating
+    std::vector<OtfOp> h_ops_up, h_ops_dn; // host copies of otf.up/dn.ops (for ed_batch_create)
+    std::vector<double> h_uvec;            // host copy of d_uvec (N entries), empty without one
     void release();
 };
 
