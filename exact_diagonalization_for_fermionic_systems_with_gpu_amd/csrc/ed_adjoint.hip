@@ -164,6 +164,9 @@ __global__ __launch_bounds__(256) void k_colsum(const double *__restrict__ parti
 
 using namespace ed;
 
+// per-thread workspace of the bilinear / site-sum entry points
+static thread_local ed::Arena t_adj_arena;
+
 extern "C" int ed_bond_bilinear(ed_hubbard hh, int species, int complex_vectors, const void *d_a, const void *d_b,
                                 double *out, void *stream)
 {
@@ -176,9 +179,10 @@ extern "C" int ed_bond_bilinear(ed_hubbard hh, int species, int complex_vectors,
     hipStream_t st = stream_of(stream);
     const int64_t D = h->otf.dim_up * h->otf.dim_dn;
     const int64_t nb = (D + 255) / 256;
-    double *dpart = nullptr, *dout = nullptr;
-    ED_TRY(dmalloc((void **)&dpart, sizeof(double) * (size_t)nb * 4 * kMaxBondOps));
-    int s = dmalloc((void **)&dout, sizeof(double) * 4 * kMaxBondOps);
+    Scratch SC(t_adj_arena, st);
+    ED_TRY(SC.reserve(Scratch::round(sizeof(double) * (size_t)nb * 4 * kMaxBondOps) + Scratch::round(sizeof(double) * 4 * kMaxBondOps)));
+    double *dpart = (double *)SC.take(sizeof(double) * (size_t)nb * 4 * kMaxBondOps), *dout = (double *)SC.take(sizeof(double) * 4 * kMaxBondOps);
+    int s = ED_OK;
     std::vector<double> host(4 * kMaxBondOps);
     for (int o0 = 0; o0 < S.nops && !s; o0 += kMaxBondOps) {
         const int no = std::min(kMaxBondOps, S.nops - o0);
@@ -200,8 +204,6 @@ extern "C" int ed_bond_bilinear(ed_hubbard hh, int species, int complex_vectors,
         for (int k = 0; k < no; ++k)
             for (int q = 0; q < 4; ++q) out[(size_t)(o0 + k) * 4 + q] = host[(size_t)k * 4 + q];
     }
-    dfree(dpart);
-    dfree(dout);
     return s;
 }
 
@@ -218,9 +220,10 @@ extern "C" int ed_site_double_occupancy(ed_hubbard hh, int complex_vectors, cons
     p.state_up = h->d_state_up;
     p.state_dn = h->d_state_dn;
     const int64_t D = p.dim_up * p.dim_dn, nb = (D + 255) / 256;
-    double *dpart = nullptr, *dout = nullptr;
-    ED_TRY(dmalloc((void **)&dpart, sizeof(double) * (size_t)nb * N));
-    int s = dmalloc((void **)&dout, sizeof(double) * N);
+    Scratch SC(t_adj_arena, st);
+    ED_TRY(SC.reserve(Scratch::round(sizeof(double) * (size_t)nb * N) + Scratch::round(sizeof(double) * N)));
+    double *dpart = (double *)SC.take(sizeof(double) * (size_t)nb * N), *dout = (double *)SC.take(sizeof(double) * N);
+    int s = ED_OK;
     if (!s) {
         if (complex_vectors)
             hipLaunchKernelGGL(k_site_docc<true>, dim3((unsigned)nb), dim3(256), 0, st, p, N, (const double *)d_a, (const double *)d_b, dpart);
@@ -232,8 +235,6 @@ extern "C" int ed_site_double_occupancy(ed_hubbard hh, int complex_vectors, cons
             hipStreamSynchronize(st) != hipSuccess)
             s = fail(ED_EHIP, "ed_site_double_occupancy: kernel failed");
     }
-    dfree(dpart);
-    dfree(dout);
     return s;
 }
 
@@ -250,9 +251,10 @@ extern "C" int ed_site_density(ed_hubbard hh, int complex_vectors, const void *d
     p.state_up = h->d_state_up;
     p.state_dn = h->d_state_dn;
     const int64_t D = p.dim_up * p.dim_dn, nb = (D + 255) / 256;
-    double *dpart = nullptr, *dout = nullptr;
-    ED_TRY(dmalloc((void **)&dpart, sizeof(double) * (size_t)nb * 2 * N));
-    int s = dmalloc((void **)&dout, sizeof(double) * 2 * N);
+    Scratch SC(t_adj_arena, st);
+    ED_TRY(SC.reserve(Scratch::round(sizeof(double) * (size_t)nb * 2 * N) + Scratch::round(sizeof(double) * 2 * N)));
+    double *dpart = (double *)SC.take(sizeof(double) * (size_t)nb * 2 * N), *dout = (double *)SC.take(sizeof(double) * 2 * N);
+    int s = ED_OK;
     if (!s) {
         if (complex_vectors)
             hipLaunchKernelGGL(k_site_dens<true>, dim3((unsigned)nb), dim3(256), 0, st, p, N, (const double *)d_a, (const double *)d_b, dpart);
@@ -264,7 +266,5 @@ extern "C" int ed_site_density(ed_hubbard hh, int complex_vectors, const void *d
             hipStreamSynchronize(st) != hipSuccess)
             s = fail(ED_EHIP, "ed_site_density: kernel failed");
     }
-    dfree(dpart);
-    dfree(dout);
     return s;
 }

@@ -9,6 +9,7 @@
 //     so one thread owns one row, replays the op list and drops each element into the
 //     first slot whose value is exactly 0.0 -- the same slot the reference picks.
 // Both are integer/byte work, bit-exact against the reference's own fermi_operator.h.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -77,6 +78,44 @@ int dmalloc(void **p, size_t bytes)
 void dfree(void *p)
 {
     if (p) (void)hipFree(p);
+}
+
+Arena::~Arena()
+{
+    if (base) (void)hipFree(base);
+}
+
+Scratch::~Scratch()
+{
+    if (cur_) (void)hipStreamSynchronize(st_);
+    dfree(own_);
+}
+
+int Scratch::reserve(size_t total)
+{
+    total = std::max<size_t>(total, 256);
+    if (total <= kArenaMax) {
+        if (a_.cap < total) {
+            dfree(a_.base);
+            a_.base = nullptr;
+            a_.cap = 0;
+            ED_TRY(dmalloc((void **)&a_.base, total));
+            a_.cap = total;
+        }
+        cur_ = a_.base;
+    } else {
+        ED_TRY(dmalloc((void **)&own_, total));
+        cur_ = own_;
+    }
+    end_ = cur_ + total;
+    return ED_OK;
+}
+
+void *Scratch::take(size_t bytes)
+{
+    char *p = cur_;
+    cur_ += round(bytes);
+    return cur_ <= end_ ? p : nullptr;
 }
 
 // ---------------------------------------------------------------------------------

@@ -75,4 +75,35 @@ int64_t binom(int n, int k);
 int dmalloc(void **p, size_t bytes);
 void dfree(void *p);
 
+// Grow-only device workspace of one host thread.  hipMalloc and hipFree wait for the whole
+// device, so entry points that run many times on concurrent streams (the disorder-ensemble
+// solves, eight host threads at once) take their per-call buffers from it instead of allocating
+// them; the memory is kept until the thread ends.  Each source file has its own arena, so an
+// entry point never shares one with an entry point it calls.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0;
+    ~Arena();
+};
+// Per-call view: buffers carved (256 B aligned) from the thread's arena when the call needs at
+// most kArenaMax bytes in total, else from one private allocation freed with the view.  The
+// view synchronises its stream when it goes out of scope, so no work of this call still uses
+// the arena when the thread's next call (maybe on another stream) reuses it.
+class Scratch {
+  public:
+    static constexpr size_t kArenaMax = (size_t)4 << 30;
+    Scratch(Arena &a, hipStream_t st) : a_(a), st_(st) {}
+    ~Scratch();
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    static size_t round(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+    int reserve(size_t total); // total = sum of round(bytes) of the buffers to take
+    void *take(size_t bytes);
+
+  private:
+    Arena &a_;
+    hipStream_t st_;
+    char *own_ = nullptr, *cur_ = nullptr, *end_ = nullptr;
+};
+
 } // namespace ed

@@ -29,37 +29,6 @@ constexpr int kEB = 256;
 constexpr int kEChunk = kEB * 8;
 constexpr int kMaxNcv = 64;
 
-// partial[j * nb + b] = (Re, Im) of sum over chunk b of conj(V_j) w
-template <bool CPLX>
-__global__ __launch_bounds__(kEB) void k_proj(int64_t n, int64_t ld, const double *__restrict__ V,
-                                              const double *__restrict__ w, double *__restrict__ partial, int64_t nb)
-{
-    __shared__ double red[kEB / 64];
-    const int j = blockIdx.y;
-    const double *v = V + (int64_t)j * ld;
-    const int64_t base = (int64_t)blockIdx.x * kEChunk + threadIdx.x;
-    double sr = 0.0, si = 0.0;
-#pragma unroll
-    for (int k = 0; k < kEChunk / kEB; ++k) {
-        const int64_t i = base + (int64_t)k * kEB;
-        if (i < n) {
-            if (CPLX) {
-                const double vr = v[2 * i], vi = v[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
-                sr = fma(vr, wr, fma(vi, wi, sr));
-                si = fma(vr, wi, fma(-vi, wr, si));
-            } else {
-                sr = fma(v[i], w[i], sr);
-            }
-        }
-    }
-    sr = block_sum<kEB>(sr, red);
-    if (CPLX) si = block_sum<kEB>(si, red);
-    if (threadIdx.x == 0) {
-        partial[2 * ((int64_t)j * nb + blockIdx.x)] = sr;
-        partial[2 * ((int64_t)j * nb + blockIdx.x) + 1] = si;
-    }
-}
-
 // c[2j], c[2j+1] = sum_b partial[j][b] (one block per j, fixed order)
 __global__ __launch_bounds__(256) void k_proj_fin(const double *__restrict__ partial, int64_t nb,
                                                   double *__restrict__ c)
@@ -79,27 +48,112 @@ __global__ __launch_bounds__(256) void k_proj_fin(const double *__restrict__ par
     }
 }
 
-// w -= sum_{j<m} V_j c_j
+// c = V^T w over V_0..V_{m-1}: per block of kEChunk elements, w is read once into registers
+// and each basis vector streamed against it; partial[j][block] = (Re, Im) of that block's sum
+// of conj(V_j) w (the layout k_proj_fin reduces in a fixed order)
 template <bool CPLX>
-__global__ __launch_bounds__(kEB) void k_subtract(int64_t n, int64_t ld, int m, const double *__restrict__ V,
-                                                  const double *__restrict__ c, double *__restrict__ w)
+__global__ __launch_bounds__(kEB) void k_proj_all(int64_t n, int64_t ld, int m, const double *__restrict__ V,
+                                                  const double *__restrict__ w, double *__restrict__ partial,
+                                                  int64_t nb)
 {
-    const int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x;
-    if (i >= n) return;
-    if (CPLX) {
-        double wr = w[2 * i], wi = w[2 * i + 1];
-        for (int j = 0; j < m; ++j) {
-            const double vr = V[(int64_t)j * ld + 2 * i], vi = V[(int64_t)j * ld + 2 * i + 1];
-            const double cr = c[2 * j], ci = c[2 * j + 1];
-            wr -= vr * cr - vi * ci;
-            wi -= vr * ci + vi * cr;
+    constexpr int NC = CPLX ? 2 : 1, E = kEChunk / kEB;
+    __shared__ double red[kEB / 64][kMaxNcv][2];
+    const int64_t base = (int64_t)blockIdx.x * kEChunk + threadIdx.x;
+    double wr[E], wi[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = base + (int64_t)e * kEB;
+        wr[e] = i < n ? w[NC * i] : 0.0;
+        wi[e] = (CPLX && i < n) ? w[NC * i + 1] : 0.0;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = 0; j < m; ++j) {
+        const double *v = V + (int64_t)j * ld;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int64_t i = base + (int64_t)e * kEB;
+            if (i < n) {
+                if (CPLX) {
+                    const double vr = v[2 * i], vi = v[2 * i + 1];
+                    sr = fma(vr, wr[e], fma(vi, wi[e], sr));
+                    si = fma(vr, wi[e], fma(-vi, wr[e], si));
+                } else {
+                    sr = fma(v[i], wr[e], sr);
+                }
+            }
         }
-        w[2 * i] = wr;
-        w[2 * i + 1] = wi;
-    } else {
-        double wr = w[i];
-        for (int j = 0; j < m; ++j) wr = fma(-c[2 * j], V[(int64_t)j * ld + i], wr);
-        w[i] = wr;
+        sr = wave_sum(sr);
+        if (CPLX) si = wave_sum(si);
+        if (lane == 0) {
+            red[wid][j][0] = sr;
+            red[wid][j][1] = si;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += kEB) {
+        double sr = 0.0, si = 0.0;
+        for (int q = 0; q < kEB / 64; ++q) {
+            sr += red[q][j][0];
+            si += red[q][j][1];
+        }
+        partial[2 * ((int64_t)j * nb + blockIdx.x)] = sr;
+        partial[2 * ((int64_t)j * nb + blockIdx.x) + 1] = si;
+    }
+}
+
+// w -= sum_{j<m} V_j c_j; NRM: also nrm[block] = that block's sum of |w|^2 after the update
+template <bool CPLX, bool NRM>
+__global__ __launch_bounds__(kEB) void k_sub_all(int64_t n, int64_t ld, int m, const double *__restrict__ V,
+                                                 const double *__restrict__ c, double *__restrict__ w,
+                                                 double *__restrict__ nrm)
+{
+    constexpr int NC = CPLX ? 2 : 1, E = kEChunk / kEB;
+    __shared__ double nred[kEB / 64];
+    const int64_t base = (int64_t)blockIdx.x * kEChunk + threadIdx.x;
+    double wr[E], wi[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = base + (int64_t)e * kEB;
+        wr[e] = i < n ? w[NC * i] : 0.0;
+        wi[e] = (CPLX && i < n) ? w[NC * i + 1] : 0.0;
+    }
+    for (int j = 0; j < m; ++j) {
+        const double *v = V + (int64_t)j * ld;
+        const double cr = c[2 * j], ci = c[2 * j + 1];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int64_t i = base + (int64_t)e * kEB;
+            if (i < n) {
+                if (CPLX) {
+                    const double vr = v[2 * i], vi = v[2 * i + 1];
+                    wr[e] -= vr * cr - vi * ci;
+                    wi[e] -= vr * ci + vi * cr;
+                } else {
+                    wr[e] = fma(-cr, v[i], wr[e]);
+                }
+            }
+        }
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = base + (int64_t)e * kEB;
+        if (i < n) {
+            w[NC * i] = wr[e];
+            if (CPLX) w[NC * i + 1] = wi[e];
+            if (NRM) t = fma(wr[e], wr[e], CPLX ? fma(wi[e], wi[e], t) : t);
+        }
+    }
+    if (NRM) {
+        t = wave_sum(t);
+        if ((threadIdx.x & 63) == 0) nred[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int q = 0; q < kEB / 64; ++q) r += nred[q];
+            nrm[blockIdx.x] = r;
+        }
     }
 }
 
@@ -184,43 +238,41 @@ static void jacobi_eig(int m, std::vector<double> A, std::vector<double> &w, std
     Z.swap(Zs);
 }
 
+// buffers of one solve, carved from the calling thread's arena (see Scratch)
 struct EigWork {
     double *V = nullptr;   // (ncv + 1) vectors of ld doubles
     double *part = nullptr; // 2 * ncv * nb
     double *c = nullptr;    // 2 * ncv
+    double *c2 = nullptr;   // 2 * ncv
     double *Y = nullptr;    // ncv * ncv
     double *nrm = nullptr;  // 1
-    ~EigWork()
-    {
-        dfree(V);
-        dfree(part);
-        dfree(c);
-        dfree(Y);
-        dfree(nrm);
-    }
 };
+static thread_local Arena t_eigs_arena;
 
+// Classical Gram-Schmidt twice against V_0..V_{m-1}: c1 = V^T w, w -= V c1, c2 = V^T w,
+// w -= V c2 with |w|^2 in the same pass.  c1, c2 land in c1h, c2h and |w| in *b (the call
+// synchronises the stream).
 template <bool CPLX>
-static int proj_subtract(int64_t n, int64_t ld, int m, const double *V, double *w, EigWork &W, int64_t nb,
-                         double *c_host, hipStream_t st)
+static int cgs2(int64_t n, int64_t ld, int m, const double *V, double *w, EigWork &W, int64_t nb, double *c1h,
+                double *c2h, double *b, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_proj<CPLX>), dim3((unsigned)nb, (unsigned)m), dim3(kEB), 0, st, n, ld, V, (const double *)w,
-                       W.part, nb);
-    ED_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_proj_fin, dim3((unsigned)m), dim3(256), 0, st, (const double *)W.part, nb, W.c);
-    ED_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_subtract<CPLX>), dim3((unsigned)((n + kEB - 1) / kEB)), dim3(kEB), 0, st, n, ld, m, V,
-                       (const double *)W.c, w);
-    ED_CHECK_LAUNCH();
-    ED_HIP(hipMemcpyAsync(c_host, W.c, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
-    return ED_OK;
-}
-
-static int norm2(int64_t nreal, const double *x, EigWork &W, int64_t *nbv, double *out, hipStream_t st)
-{
-    ED_TRY(dot_partials(nreal, x, x, W.part, nbv, st));
-    ED_TRY(finalize_sum(W.part, *nbv, W.nrm, 1, st));
-    ED_HIP(hipMemcpyAsync(out, W.nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+    const dim3 g((unsigned)nb), t(kEB);
+    double *cc[2] = {W.c, W.c2}, *ch[2] = {c1h, c2h};
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL((k_proj_all<CPLX>), g, t, 0, st, n, ld, m, V, (const double *)w, W.part, nb);
+        hipLaunchKernelGGL(k_proj_fin, dim3((unsigned)m), dim3(256), 0, st, (const double *)W.part, nb, cc[pass]);
+        ED_CHECK_LAUNCH();
+        ED_HIP(hipMemcpyAsync(ch[pass], cc[pass], sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
+        if (pass == 0)
+            hipLaunchKernelGGL((k_sub_all<CPLX, false>), g, t, 0, st, n, ld, m, V, (const double *)cc[pass], w,
+                               nullptr);
+        else
+            hipLaunchKernelGGL((k_sub_all<CPLX, true>), g, t, 0, st, n, ld, m, V, (const double *)cc[pass], w,
+                               W.part);
+        ED_CHECK_LAUNCH();
+    }
+    ED_TRY(finalize_sum(W.part, nb, W.nrm, 1, st));
+    ED_HIP(hipMemcpyAsync(b, W.nrm, sizeof(double), hipMemcpyDeviceToHost, st));
     ED_HIP(hipStreamSynchronize(st));
     return ED_OK;
 }
@@ -236,11 +288,18 @@ static int eigs_run(const Hubbard *h, int nev, int ncv, int maxit, double tol, u
     const int64_t nb = (n + kEChunk - 1) / kEChunk;
     const int64_t nbblas = blas_num_blocks(nreal);
     EigWork W;
-    ED_TRY(dmalloc((void **)&W.V, sizeof(double) * (size_t)ld * (m + 1)));
-    ED_TRY(dmalloc((void **)&W.part, sizeof(double) * 2 * (size_t)std::max<int64_t>(m * nb, nbblas)));
-    ED_TRY(dmalloc((void **)&W.c, sizeof(double) * 2 * m));
-    ED_TRY(dmalloc((void **)&W.Y, sizeof(double) * m * m));
-    ED_TRY(dmalloc((void **)&W.nrm, sizeof(double)));
+    Scratch S(t_eigs_arena, st);
+    const size_t bV = sizeof(double) * (size_t)ld * (m + 1),
+                 bP = sizeof(double) * 2 * (size_t)std::max<int64_t>(m * nb, nbblas), bC = sizeof(double) * 2 * m,
+                 bY = sizeof(double) * m * m, bN = sizeof(double);
+    ED_TRY(S.reserve(Scratch::round(bV) + Scratch::round(bP) + 2 * Scratch::round(bC) + Scratch::round(bY) +
+                     Scratch::round(bN)));
+    W.V = (double *)S.take(bV);
+    W.part = (double *)S.take(bP);
+    W.c = (double *)S.take(bC);
+    W.c2 = (double *)S.take(bC);
+    W.Y = (double *)S.take(bY);
+    W.nrm = (double *)S.take(bN);
     auto vec = [&](int j) { return W.V + (int64_t)j * ld; };
     ED_TRY(ed_random_vector(n, seed, CPLX ? 1 : 0, vec(0), 1, st));
 
@@ -256,11 +315,8 @@ static int eigs_run(const Hubbard *h, int nev, int ncv, int maxit, double tol, u
         for (int j = k; j < m; ++j) {
             double *w = vec(j + 1);
             ED_TRY(hv_plain(*h, CPLX, vec(j), w, st));
-            int64_t nbv;
             double b = 0.0;
-            ED_TRY(proj_subtract<CPLX>(n, ld, j + 1, W.V, w, W, nb, c1.data(), st));
-            ED_TRY(proj_subtract<CPLX>(n, ld, j + 1, W.V, w, W, nb, c2.data(), st));
-            ED_TRY(norm2(nreal, w, W, &nbv, &b, st)); // synchronizes: c1, c2 are on the host now
+            ED_TRY(cgs2<CPLX>(n, ld, j + 1, W.V, w, W, nb, c1.data(), c2.data(), &b, st));
             if (j == k) {
                 for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = c1[2 * i] + c2[2 * i];
             } else {
@@ -349,17 +405,17 @@ extern "C" int ed_hubbard_apply_host(ed_hubbard hh, int complex_vectors, const v
     const int64_t n = h->args.dim_up * h->args.dim_dn;
     const size_t bytes = sizeof(double) * (size_t)n * (complex_vectors ? 2 : 1);
     hipStream_t st = stream_of(stream);
-    void *dx = nullptr, *dy = nullptr;
-    ED_TRY(dmalloc(&dx, bytes));
-    int s = dmalloc(&dy, bytes);
-    if (!s && hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+    static thread_local Arena t_apply_arena;
+    Scratch S(t_apply_arena, st);
+    ED_TRY(S.reserve(2 * Scratch::round(bytes)));
+    void *dx = S.take(bytes), *dy = S.take(bytes);
+    int s = ED_OK;
+    if (hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
         s = fail(ED_EHIP, "ed_hubbard_apply_host: upload failed");
     if (!s) s = ed_hubbard_apply(hh, complex_vectors, dx, dy, stream);
     if (!s && (hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
                hipStreamSynchronize(st) != hipSuccess))
         s = fail(ED_EHIP, "ed_hubbard_apply_host: download failed");
     if (st == nullptr || s) (void)hipDeviceSynchronize();
-    dfree(dx);
-    dfree(dy);
     return s;
 }

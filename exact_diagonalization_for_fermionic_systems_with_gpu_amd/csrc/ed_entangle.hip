@@ -285,25 +285,24 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
         maxK = std::max(maxK, G.cols);
     }
     int s = ED_OK;
-    auto cleanup = [&] {
-        dfree(dM);
-        dfree(drho);
-        dfree(dW);
-        dfree(dE);
-        dfree(dcu);
-        dfree(dcd);
-        dfree(dgeo);
-        dfree(dinfo);
-    };
+    static thread_local Arena t_ee_arena;
+    Scratch S(t_ee_arena, st);
     do {
-        if ((s = dmalloc((void **)&dM, sizeof(double) * NC * (size_t)P.total))) break;
-        if ((s = dmalloc((void **)&drho, sizeof(double) * NC * (size_t)(maxk * maxk)))) break;
-        if ((s = dmalloc((void **)&dW, sizeof(double) * (size_t)maxk))) break;
-        if ((s = dmalloc((void **)&dE, sizeof(double) * (size_t)std::max(maxk, maxK)))) break;
-        if ((s = dmalloc((void **)&dcu, sizeof(int32_t) * P.cu.size()))) break;
-        if ((s = dmalloc((void **)&dcd, sizeof(int32_t) * P.cd.size()))) break;
-        if ((s = dmalloc((void **)&dgeo, sizeof(BlockGeo) * P.geo.size()))) break;
-        if ((s = dmalloc((void **)&dinfo, sizeof(rocblas_int)))) break;
+        const size_t bM = sizeof(double) * NC * (size_t)P.total, bR = sizeof(double) * NC * (size_t)(maxk * maxk),
+                     bW = sizeof(double) * (size_t)maxk, bE = sizeof(double) * (size_t)std::max(maxk, maxK),
+                     bU = sizeof(int32_t) * P.cu.size(), bD = sizeof(int32_t) * P.cd.size(),
+                     bG = sizeof(BlockGeo) * P.geo.size(), bI = sizeof(rocblas_int);
+        if ((s = S.reserve(Scratch::round(bM) + Scratch::round(bR) + Scratch::round(bW) + Scratch::round(bE) +
+                           Scratch::round(bU) + Scratch::round(bD) + Scratch::round(bG) + Scratch::round(bI))))
+            break;
+        dM = (double *)S.take(bM);
+        drho = (double *)S.take(bR);
+        dW = (double *)S.take(bW);
+        dE = (double *)S.take(bE);
+        dcu = (int32_t *)S.take(bU);
+        dcd = (int32_t *)S.take(bD);
+        dgeo = (BlockGeo *)S.take(bG);
+        dinfo = (rocblas_int *)S.take(bI);
         if (hipMemcpyAsync(dcu, P.cu.data(), sizeof(int32_t) * P.cu.size(), hipMemcpyHostToDevice, st) ||
             hipMemcpyAsync(dcd, P.cd.data(), sizeof(int32_t) * P.cd.size(), hipMemcpyHostToDevice, st) ||
             hipMemcpyAsync(dgeo, P.geo.data(), sizeof(BlockGeo) * P.geo.size(), hipMemcpyHostToDevice, st)) {
@@ -385,7 +384,6 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
             for (double v : w) sigma.push_back(v);
         }
     } while (0);
-    cleanup();
     return s;
 }
 
