@@ -18,6 +18,7 @@
 // in place, one element per thread with the m old values in registers).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -249,31 +250,33 @@ struct EigWork {
 };
 static thread_local Arena t_eigs_arena;
 
-// Classical Gram-Schmidt twice against V_0..V_{m-1}: c1 = V^T w, w -= V c1, c2 = V^T w,
-// w -= V c2 with |w|^2 in the same pass.  c1, c2 land in c1h, c2h and |w| in *b (the call
-// synchronises the stream).
+// Classical Gram-Schmidt against V_0..V_{m-1}: c1 = V^T w, w -= V c1 with |w|^2 in the same
+// pass; then, unless the first pass kept most of w (the DGKS test |w_new|^2 >= |w_old|^2 / 2,
+// with |w_old|^2 = |c1|^2 + |w_new|^2; always when force2), c2 = V^T w, w -= V c2.  c1, c2 land
+// in c1h, c2h (c2 = 0 after one pass) and |w| in *b; the call synchronises the stream.
 template <bool CPLX>
-static int cgs2(int64_t n, int64_t ld, int m, const double *V, double *w, EigWork &W, int64_t nb, double *c1h,
-                double *c2h, double *b, hipStream_t st)
+static int cgs(bool force2, int64_t n, int64_t ld, int m, const double *V, double *w, EigWork &W, int64_t nb,
+               double *c1h, double *c2h, double *b, hipStream_t st)
 {
     const dim3 g((unsigned)nb), t(kEB);
     double *cc[2] = {W.c, W.c2}, *ch[2] = {c1h, c2h};
+    std::fill(c2h, c2h + 2 * m, 0.0);
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL((k_proj_all<CPLX>), g, t, 0, st, n, ld, m, V, (const double *)w, W.part, nb);
         hipLaunchKernelGGL(k_proj_fin, dim3((unsigned)m), dim3(256), 0, st, (const double *)W.part, nb, cc[pass]);
         ED_CHECK_LAUNCH();
         ED_HIP(hipMemcpyAsync(ch[pass], cc[pass], sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
-        if (pass == 0)
-            hipLaunchKernelGGL((k_sub_all<CPLX, false>), g, t, 0, st, n, ld, m, V, (const double *)cc[pass], w,
-                               nullptr);
-        else
-            hipLaunchKernelGGL((k_sub_all<CPLX, true>), g, t, 0, st, n, ld, m, V, (const double *)cc[pass], w,
-                               W.part);
+        hipLaunchKernelGGL((k_sub_all<CPLX, true>), g, t, 0, st, n, ld, m, V, (const double *)cc[pass], w, W.part);
         ED_CHECK_LAUNCH();
+        ED_TRY(finalize_sum(W.part, nb, W.nrm, 1, st));
+        ED_HIP(hipMemcpyAsync(b, W.nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        if (pass == 0 && !force2) {
+            double c2sum = 0.0;
+            for (int i = 0; i < 2 * m; ++i) c2sum += c1h[i] * c1h[i];
+            if ((*b) * (*b) >= 0.5 * (c2sum + (*b) * (*b))) break;
+        }
     }
-    ED_TRY(finalize_sum(W.part, nb, W.nrm, 1, st));
-    ED_HIP(hipMemcpyAsync(b, W.nrm, sizeof(double), hipMemcpyDeviceToHost, st));
-    ED_HIP(hipStreamSynchronize(st));
     return ED_OK;
 }
 
@@ -304,6 +307,8 @@ static int eigs_run(const Hubbard *h, int nev, int ncv, int maxit, double tol, u
     ED_TRY(ed_random_vector(n, seed, CPLX ? 1 : 0, vec(0), 1, st));
 
     const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+    const char *ep = getenv("ED_EIGS_PASSES");
+    const bool two_pass = ep && atoi(ep) == 2;
     std::vector<double> T((size_t)m * m, 0.0), theta, Z, c1(2 * m), c2(2 * m);
     std::vector<double> beta(m, 0.0);
     int k = 0, iter = 0, nconv = 0, m_eff = m;
@@ -316,7 +321,8 @@ static int eigs_run(const Hubbard *h, int nev, int ncv, int maxit, double tol, u
             double *w = vec(j + 1);
             ED_TRY(hv_plain(*h, CPLX, vec(j), w, st));
             double b = 0.0;
-            ED_TRY(cgs2<CPLX>(n, ld, j + 1, W.V, w, W, nb, c1.data(), c2.data(), &b, st));
+            // second pass by the DGKS test (ED_EIGS_PASSES=2: always)
+            ED_TRY(cgs<CPLX>(two_pass, n, ld, j + 1, W.V, w, W, nb, c1.data(), c2.data(), &b, st));
             if (j == k) {
                 for (int i = 0; i < k; ++i) T[(size_t)i * m + k] = T[(size_t)k * m + i] = c1[2 * i] + c2[2 * i];
             } else {
