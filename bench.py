@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--ground-state", action="store_true",
                     help="also run the Lanczos ground state of the same workload (Ritz vector rebuilt) and its "
                          "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")
+    ap.add_argument("--drude-only", action="store_true",
+                    help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run one device copy of a vector (16 D bytes) as the byte-count reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -154,6 +156,10 @@ def main():
         print("bench.py measures one GPU: the H.v path here is single-GPU (multi-GPU sharding is a design "
               "note only), so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.drude_only:
+        print(json.dumps({"batched_drude": batched_drude_line()}), flush=True)
+        return
 
     import torch
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard, Lanczos
