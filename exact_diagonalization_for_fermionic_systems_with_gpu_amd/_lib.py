@@ -65,6 +65,9 @@ SIGNATURES = {
     "ed_normalize": (_i32, [_i64, _i32, _p, _pd, _p]),
     "ed_lanczos_eigen": (_i32, [_p, _i32, C.POINTER(LanczosCfg), _p, _pd, _p, _pd, _pd, _pi, _p]),
     "ed_lanczos_tridiagonal": (_i32, [_p, _i32, _i32, _p, _pd, _pd, _p]),
+    "ed_tridiag_eigen": (_i32, [_i32, _pd, _pd, _pd, _pd]),
+    "ed_krylov_tridiagonal": (_i32, [_p, _p, _i32, _d, _pd, _pd, _pi, _p]),
+    "ed_krylov_time_evol": (_i32, [_p, _p, _i32, _pd, _pd, _pd, _pd, _d, _p, _p]),
     "ed_tridiag_lowest": (_i32, [_i32, _pd, _pd, _pd, _pd]),
     "ed_tridiag_eigvals": (_i32, [_i32, _pd, _pd, _pd]),
     "ed_eigs_lowest": (_i32, [_p, _i32, _i32, _i32, _i32, _d, C.c_uint32, _pd, _p, _pi, _pi, _p]),

@@ -125,8 +125,9 @@ static void inverse_iteration(int n, const double *d, const double *e, double la
     for (int i = 0; i < n; ++i) z[i] = sg * x[i];
 }
 
-// implicit QL with Wilkinson-type shifts (eigenvalues only), Golub & Van Loan sec. 8.3
-static void ql_eigvals(int n, double *d, double *e /* n entries, e[n-1] scratch */)
+// implicit QL with Wilkinson-type shifts, Golub & Van Loan sec. 8.3; with z (n x n, column j =
+// eigenvector j, z[j * n + i], identity on entry) the plane rotations are accumulated into it
+static void ql_eig(int n, double *d, double *e /* n entries, e[n-1] scratch */, double *z)
 {
     e[n - 1] = 0.0;
     for (int l = 0; l < n; ++l) {
@@ -159,6 +160,14 @@ static void ql_eigvals(int n, double *d, double *e /* n entries, e[n-1] scratch 
                     p = s * r;
                     d[i + 1] = g + p;
                     g = c * r - b;
+                    if (z) {
+                        double *zi = z + (size_t)i * n, *zj = z + (size_t)(i + 1) * n;
+                        for (int k = 0; k < n; ++k) {
+                            const double t = zj[k];
+                            zj[k] = s * zi[k] + c * t;
+                            zi[k] = c * zi[k] - s * t;
+                        }
+                    }
                 }
                 if (r == 0.0 && i >= l) continue;
                 d[l] -= p;
@@ -236,9 +245,26 @@ extern "C" int ed_tridiag_eigvals(int n, const double *d, const double *e, doubl
     if (n < 1 || !d || (n > 1 && !e) || !w) return fail(ED_EINVAL, "ed_tridiag_eigvals: bad arguments");
     std::vector<double> dd(d, d + n), ee(n, 0.0);
     for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
-    ql_eigvals(n, dd.data(), ee.data());
+    ql_eig(n, dd.data(), ee.data(), nullptr);
     std::sort(dd.begin(), dd.end());
     for (int i = 0; i < n; ++i) w[i] = dd[i];
+    return ED_OK;
+}
+
+extern "C" int ed_tridiag_eigen(int n, const double *d, const double *e, double *w, double *z)
+{
+    if (n < 1 || !d || (n > 1 && !e) || !w || !z) return fail(ED_EINVAL, "ed_tridiag_eigen: bad arguments");
+    std::vector<double> dd(d, d + n), ee(n, 0.0), zz((size_t)n * n, 0.0);
+    for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
+    for (int i = 0; i < n; ++i) zz[(size_t)i * n + i] = 1.0;
+    ql_eig(n, dd.data(), ee.data(), zz.data());
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return dd[x] < dd[y]; });
+    for (int j = 0; j < n; ++j) {
+        w[j] = dd[idx[j]];
+        std::copy(zz.begin() + (size_t)idx[j] * n, zz.begin() + (size_t)(idx[j] + 1) * n, z + (size_t)j * n);
+    }
     return ED_OK;
 }
 
@@ -418,6 +444,132 @@ extern "C" int ed_continued_fraction(int n, const double *a, const double *b, do
         }
         g_ri[2 * k] = norm * gr;
         g_ri[2 * k + 1] = norm * gi;
+    }
+    return ED_OK;
+}
+
+// ---- real-time evolution in the Krylov space (Krylov<C>, HubMatrix_GPU.cuh:619-728) ---------
+
+namespace ed {
+
+// One step of the propagation: phi += c q_i (complex c); then, unless this is the last term,
+// q_{i+1} = ((y - b_i q_{i-1}) - a_i q_i) / b_{i+1} with y = H q_i, written over q_{i-1}.
+__global__ __launch_bounds__(256) void k_krylov_step(int64_t n, double cr, double ci, double a, double b, double inv,
+                                                     int last, const double *__restrict__ y, double *qprev,
+                                                     const double *__restrict__ q, double *__restrict__ phi)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double qr = q[2 * i], qi = q[2 * i + 1];
+    phi[2 * i] += cr * qr - ci * qi;
+    phi[2 * i + 1] += cr * qi + ci * qr;
+    if (last) return;
+    const double tr = (y[2 * i] - b * qprev[2 * i]) - a * qr;
+    const double ti = (y[2 * i + 1] - b * qprev[2 * i + 1]) - a * qi;
+    qprev[2 * i] = tr * inv;
+    qprev[2 * i + 1] = ti * inv;
+}
+
+static thread_local Arena t_krylov_arena;
+
+} // namespace ed
+
+extern "C" int ed_krylov_tridiagonal(ed_hubbard hh, const void *d_phi0, int max_steps, double btol, double *a_out,
+                                     double *b_out, int *n_out, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_phi0 || max_steps < 1 || !(btol >= 0.0) || !a_out || !b_out || !n_out)
+        return fail(ED_EINVAL, "ed_krylov_tridiagonal: bad arguments");
+    hipStream_t st = stream_of(stream);
+    const int64_t n = h->args.dim_up * h->args.dim_dn, nreal = 2 * n;
+    Work w;
+    ED_TRY(alloc_work(w, *h, nreal, max_steps, true));
+    double *f = nullptr;
+    ED_TRY(dmalloc((void **)&f, sizeof(double) * (size_t)nreal));
+    int s = ED_OK;
+    std::vector<double> hb((size_t)max_steps + 1);
+    int N = 0, nt = -1;
+    do {
+        // the input is used as it is (b_0 = 1: no normalisation), as Krylov::tridiagonal does
+        const double one = 1.0;
+        if (hipMemcpyAsync(f, d_phi0, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess ||
+            hipMemcpy(w.bdev, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_krylov_tridiagonal: setup failed");
+            break;
+        }
+        double *A = f, *B = w.buf[0], *C = w.buf[1];
+        while (N < max_steps && nt < 0 && s == ED_OK) {
+            const int chunk = std::min(10, max_steps - N);
+            for (int k = 0; k < chunk && s == ED_OK; ++k, ++N) {
+                s = lanczos_step(*h, true, nreal, A, B, C, w, N, false, st);
+                double *t = A;
+                A = B;
+                B = C;
+                C = t;
+            }
+            if (s) break;
+            // stop at the first b_k < btol (Krylov::tridiagonal breaks after appending it)
+            if (hipStreamSynchronize(st) != hipSuccess ||
+                hipMemcpy(hb.data(), w.bdev, sizeof(double) * (size_t)(N + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_krylov_tridiagonal: readback failed");
+                break;
+            }
+            for (int k = N - chunk + 1; k <= N; ++k)
+                if (hb[(size_t)k] < btol) {
+                    nt = k;
+                    break;
+                }
+        }
+        if (s) break;
+        if (nt < 0) nt = N;
+        if (hipMemcpy(a_out, w.adev, sizeof(double) * (size_t)nt, hipMemcpyDeviceToHost) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_krylov_tridiagonal: readback failed");
+            break;
+        }
+        b_out[0] = 0.0;
+        for (int k = 1; k <= nt; ++k) b_out[k] = hb[(size_t)k];
+        *n_out = nt;
+    } while (0);
+    (void)hipStreamSynchronize(st);
+    dfree(f);
+    return s;
+}
+
+extern "C" int ed_krylov_time_evol(ed_hubbard hh, const void *d_phi0, int n, const double *a, const double *b,
+                                   const double *m, const double *z, double dt, void *d_phi, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !d_phi0 || !d_phi || n < 1 || !a || !b || !m || !z)
+        return fail(ED_EINVAL, "ed_krylov_time_evol: bad arguments");
+    for (int i = 1; i < n; ++i)
+        if (!(b[i] != 0.0)) return fail(ED_EINVAL, "ed_krylov_time_evol: b[%d] is zero", i);
+    hipStream_t st = stream_of(stream);
+    const int64_t D = h->args.dim_up * h->args.dim_dn;
+    const size_t bytes = sizeof(double) * 2 * (size_t)D;
+    // coefficients of the Krylov vectors: c_i = sum_j exp(-i m_j dt) z_0j z_ij
+    std::vector<double> cr((size_t)n, 0.0), ci((size_t)n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const double zz = z[(size_t)j * n] * z[(size_t)j * n + i];
+            cr[(size_t)i] += std::cos(m[j] * dt) * zz;
+            ci[(size_t)i] += -std::sin(m[j] * dt) * zz;
+        }
+    Scratch S(t_krylov_arena, st);
+    ED_TRY(S.reserve(3 * Scratch::round(bytes)));
+    double *q = (double *)S.take(bytes), *qp = (double *)S.take(bytes), *y = (double *)S.take(bytes);
+    if (hipMemcpyAsync(q, d_phi0, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemsetAsync(qp, 0, bytes, st) != hipSuccess)
+        return fail(ED_EHIP, "ed_krylov_time_evol: setup failed");
+    const unsigned grid = (unsigned)((D + 255) / 256);
+    for (int i = 0; i < n; ++i) {
+        const bool last = i + 1 == n;
+        if (!last) ED_TRY(hv_plain(*h, true, q, y, st));
+        hipLaunchKernelGGL(k_krylov_step, dim3(grid), dim3(256), 0, st, D, cr[(size_t)i], ci[(size_t)i], a[i], b[i],
+                           last ? 0.0 : 1.0 / b[i + 1], last ? 1 : 0, (const double *)y, qp, (const double *)q,
+                           (double *)d_phi);
+        ED_CHECK_LAUNCH();
+        std::swap(q, qp);
     }
     return ED_OK;
 }

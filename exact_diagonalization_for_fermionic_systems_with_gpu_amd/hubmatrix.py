@@ -215,6 +215,75 @@ class Lanczos(Thrust_Lin):
         self.b += [b[k] for k in range(1, maxN + 1)]
 
 
+def tridiag_eigen(a, b):
+    """All eigenpairs of the tridiagonal T (diagonal a[n], off-diagonal b[1..n-1] of the
+    reference's b list), as LAPACKf77::dsted(compz='I') returns them (MAGMA_LAPACK.h:296-326):
+    (m ascending [n], z [n*n] column-major, z[j*n + i] = component i of eigenvector j)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = len(a)
+    e = np.ascontiguousarray(np.asarray(b, dtype=np.float64)[1:n], dtype=np.float64)
+    if len(e) == 0:
+        e = np.zeros(1)
+    m = np.empty(n)
+    z = np.empty(n * n)
+    call("ed_tridiag_eigen", n, a.ctypes.data_as(C.POINTER(C.c_double)), e.ctypes.data_as(C.POINTER(C.c_double)),
+         m.ctypes.data_as(C.POINTER(C.c_double)), z.ctypes.data_as(C.POINTER(C.c_double)))
+    return m, z
+
+
+class Krylov(Thrust_Lin):
+    """Krylov<C> (HubMatrix_GPU.cuh:619-728): real-time evolution psi(dt) = exp(-i H dt) psi0 in
+    the Lanczos space of psi0 (complex128 vectors).  a, b (b starts as [0.0]), m, z are host
+    arrays as in the reference."""
+
+    def __init__(self, dimUP: int, dimDN: int, max_steps: int = 1000, btol: float = 1e-10):
+        super().__init__(dimUP, dimDN)
+        self.max_steps = int(max_steps)
+        self.btol = float(btol)
+        self.clear()
+
+    def clear(self):
+        """HubMatrix_GPU.cuh:730-739"""
+        self.a: list[float] = []
+        self.b: list[float] = [0.0]
+        self.m = np.zeros(0)
+        self.z = np.zeros(0)
+
+    def tridiagonal(self, Hub: Hubbard, f: torch.Tensor):
+        """Krylov<C>::tridiagonal (HubMatrix_GPU.cuh:660-680): recurrence from f as it is (no
+        normalisation), at most max_steps steps, stopping after the first b < btol."""
+        Hub._check(f)
+        if not f.is_complex():
+            raise TypeError("Krylov works on complex128 vectors (thrust::complex<C> in the reference)")
+        a = (C.c_double * self.max_steps)()
+        b = (C.c_double * (self.max_steps + 1))()
+        n = C.c_int()
+        call("ed_krylov_tridiagonal", Hub.handle, _raw(f), self.max_steps, self.btol, a, b, C.byref(n),
+             stream_ptr())
+        self.a = [a[k] for k in range(n.value)]
+        self.b = [0.0] + [b[k] for k in range(1, n.value + 1)]
+
+    def Make_tridiagonal_eigen(self, Hub: Hubbard, phi0: torch.Tensor):
+        """HubMatrix_GPU.cuh:683-692: tridiagonal, then all eigenpairs of T (m, z)."""
+        self.tridiagonal(Hub, phi0)
+        self.m, self.z = tridiag_eigen(self.a, self.b)
+
+    def Krylov_time_evol(self, Hub: Hubbard, phi0: torch.Tensor, phi: torch.Tensor, dt: float):
+        """HubMatrix_GPU.cuh:694-724: phi += sum_i c_i q_i with c_i = sum_j exp(-i m_j dt) z_0j z_ij
+        (phi accumulates; pass zeros for exp(-i H dt) phi0).  phi0 is not modified."""
+        Hub._check(phi0, phi)
+        n = len(self.a)
+        a = np.ascontiguousarray(self.a, dtype=np.float64)
+        b = np.ascontiguousarray(self.b, dtype=np.float64)
+        m = np.ascontiguousarray(self.m, dtype=np.float64)
+        z = np.ascontiguousarray(self.z, dtype=np.float64)
+        if len(m) != n or len(z) != n * n:
+            raise ValueError("Krylov_time_evol needs Make_tridiagonal_eigen first")
+        P = C.POINTER(C.c_double)
+        call("ed_krylov_time_evol", Hub.handle, _raw(phi0), n, a.ctypes.data_as(P), b.ctypes.data_as(P),
+             m.ctypes.data_as(P), z.ctypes.data_as(P), float(dt), _raw(phi), stream_ptr())
+
+
 def correlator_continued_fraction(lanctri: Lanczos, H: Hubbard, nw: int, w, energy: float, phi: torch.Tensor,
                                   maxN: int = 200):
     """HubMatrix_GPU.cuh:820-835: g(w) = |phi|^2/(w+E-a0-b1^2/(w+E-a1-...)).  phi is

@@ -154,6 +154,57 @@ class Lanczos : public Thrust_Lin<T> {
     }
 };
 
+// Krylov<C> (HubMatrix_GPU.cuh:619-728): real-time evolution exp(-i H dt) phi0 in the Lanczos
+// space of phi0 (complex vectors).  a, b (b starts as {0}), m, z are host vectors as in the
+// reference; the recurrence stops after the first b < btol or maxN steps.
+template <typename Cr>
+class Krylov : public Thrust_Lin<std::complex<Cr>> {
+  public:
+    using V = device_vector<std::complex<Cr>>;
+    Krylov(int64_t dimUP_, int64_t dimDN_) : Thrust_Lin<std::complex<Cr>>(dimUP_, dimDN_) { clear(); }
+
+    std::vector<double> a, b, m, z;
+    int maxN = 1000;
+    double btol = 1e-10;
+
+    void clear() // HubMatrix_GPU.cuh:730-739
+    {
+        a.clear();
+        b.assign(1, 0.0);
+        m.clear();
+        z.clear();
+    }
+    // Krylov<C>::tridiagonal (HubMatrix_GPU.cuh:660-680): f is used as given
+    void tridiagonal(const Hubbard &Hub, const V &f)
+    {
+        std::vector<double> ha(maxN), hb(maxN + 1);
+        int n = 0;
+        check(ed_krylov_tridiagonal(Hub.handle(), f.data(), maxN, btol, ha.data(), hb.data(), &n, nullptr),
+              "Krylov::tridiagonal");
+        a.assign(ha.begin(), ha.begin() + n);
+        b.assign(hb.begin(), hb.begin() + n + 1);
+    }
+    // HubMatrix_GPU.cuh:683-692
+    void Make_tridiagonal_eigen(const Hubbard &Hub, const V &phi0)
+    {
+        tridiagonal(Hub, phi0);
+        const int n = (int)a.size();
+        m.resize(n);
+        z.resize((size_t)n * n);
+        std::vector<double> e(b.begin() + 1, b.begin() + std::max(1, n));
+        if (e.empty()) e.push_back(0.0);
+        check(ed_tridiag_eigen(n, a.data(), e.data(), m.data(), z.data()), "Krylov::Make_tridiagonal_eigen");
+    }
+    // Krylov<C>::Krylov_time_evol (HubMatrix_GPU.cuh:694-724): phi accumulates
+    void Krylov_time_evol(const Hubbard &Hub, const V &phi0, V &phi, double dt)
+    {
+        check(ed_krylov_time_evol(Hub.handle(), phi0.data(), (int)a.size(), a.data(), b.data(), m.data(), z.data(),
+                                  dt, phi.data(), nullptr),
+              "Krylov::Krylov_time_evol");
+        hip_check(hipDeviceSynchronize(), "Krylov::Krylov_time_evol");
+    }
+};
+
 // correlator_continued_fraction (HubMatrix_GPU.cuh:820-835): phi is normalised in place
 template <typename T>
 void correlator_continued_fraction(Lanczos<T> &lanctri, const Hubbard &H, int nw,

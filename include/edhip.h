@@ -167,6 +167,23 @@ int ed_lanczos_tridiagonal(ed_hubbard h, int complex_vectors, int nsteps, const 
 int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z);
 /* all eigenvalues ascending */
 int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
+/* all eigenpairs of a symmetric tridiagonal (LAPACKf77::dsted with compz = 'I', as
+ * Krylov::Make_tridiagonal_eigen calls it, HubMatrix_GPU.cuh:690-699): d[n], e[n-1]; w[n]
+ * ascending; z[n*n] column-major, z[j*n + i] = component i of the unit eigenvector of w[j]. */
+int ed_tridiag_eigen(int n, const double *d, const double *e, double *w, double *z);
+
+/* Krylov<C>::tridiagonal (HubMatrix_GPU.cuh:660-680): Lanczos recurrence from the complex128
+ * vector d_phi0 (used as it is, not normalised, not modified) without reorthogonalisation, at
+ * most max_steps steps, stopping after the first b_k < btol (the reference uses 1000, 1e-10).
+ * a_out[max_steps], b_out[max_steps + 1] with b_out[0] = 0; *n_out = number of a's. */
+int ed_krylov_tridiagonal(ed_hubbard h, const void *d_phi0, int max_steps, double btol, double *a_out,
+                          double *b_out, int *n_out, void *stream);
+/* Krylov<C>::Krylov_time_evol (HubMatrix_GPU.cuh:702-728): d_phi += sum_i c_i q_i with
+ * c_i = sum_j exp(-i m_j dt) z[j*n] z[j*n+i], the Krylov vectors q_i of d_phi0 rebuilt by the
+ * same recurrence from a[n], b[n+1] (from ed_krylov_tridiagonal) and m, z (from
+ * ed_tridiag_eigen).  Complex128; d_phi accumulates (zero it for exp(-iH dt) phi0). */
+int ed_krylov_time_evol(ed_hubbard h, const void *d_phi0, int n, const double *a, const double *b, const double *m,
+                        const double *z, double dt, void *d_phi, void *stream);
 
 /* ---------------- several lowest eigenpairs (cuSpectra) ------------------------- */
 

@@ -264,6 +264,58 @@ def tridiag_eigvals(d, e):
     return w
 
 
+def krylov_tridiagonal(H: Hubbard, f, max_steps=1000, btol=1e-10):
+    """Krylov<C>::tridiagonal (HubMatrix_GPU.cuh:660-680), restated with numpy on the oracle H.v:
+    f is used as given (complex128), cq = H f - b.back() cf_, a = Re<cq, f> (conj on the first
+    argument, complex_prod at :325-329), cq -= a f, b = |cq|, stop after appending b < btol.
+    Returns (a[n], b[n+1]) with b[0] = 0."""
+    f = np.array(f, dtype=np.complex128)
+    cf = np.zeros_like(f)
+    a, b = [], [0.0]
+    for _ in range(max_steps):
+        q = -b[-1] * cf + H.apply(f)
+        a.append(float(np.vdot(q, f).real))
+        q = -a[-1] * f + q
+        b.append(float(np.sqrt(np.vdot(q, q).real)))
+        if b[-1] < btol:
+            break
+        cf = f
+        f = (1.0 / b[-1]) * q
+    return np.array(a), np.array(b)
+
+
+def tridiag_eigen(a, b):
+    """LAPACKf77::dsted(compz='I') as Krylov::Make_tridiagonal_eigen uses it
+    (HubMatrix_GPU.cuh:683-692, MAGMA_LAPACK.h:296-326), here LAPACK's own tridiagonal solver
+    through scipy: (m ascending, z column-major with z[j*n + i] = component i of vector j)."""
+    from scipy.linalg import eigh_tridiagonal
+    a = np.asarray(a, dtype=np.float64)
+    n = len(a)
+    m, V = eigh_tridiagonal(a, np.asarray(b, dtype=np.float64)[1:n])
+    return m, np.ascontiguousarray(V.T).reshape(-1)
+
+
+def krylov_time_evol(H: Hubbard, phi0, a, b, m, z, dt, phi=None):
+    """Krylov<C>::Krylov_time_evol (HubMatrix_GPU.cuh:694-724): phi += sum_i Tri_i q_i with
+    Tri_i = sum_j (cos(m_j dt) - i sin(m_j dt)) z[j n] z[j n + i], the q_i rebuilt by the same
+    recurrence (q_{i+1} = ((H q_i - b_i q_{i-1}) - a_i q_i) / b_{i+1}).  Returns phi."""
+    n = len(a)
+    q = np.array(phi0, dtype=np.complex128)
+    phi = np.zeros_like(q) if phi is None else np.array(phi, dtype=np.complex128)
+    cf = np.zeros_like(q)
+    zz = np.asarray(z).reshape(n, n)  # zz[j, i] = z[j n + i]
+    for i in range(n):
+        tri = np.sum((np.cos(m * dt) - 1j * np.sin(m * dt)) * zz[:, 0] * zz[:, i])
+        phi = tri * q + phi
+        if i + 1 == n:
+            break
+        cq = -b[i] * cf + H.apply(q)
+        cq = -a[i] * q + cq
+        cf = q
+        q = (1.0 / b[i + 1]) * cq
+    return phi
+
+
 def continued_fraction(a, b, energy, norm, w):
     """correlator_continued_fraction loop (HubMatrix_GPU.cuh:829-834)."""
     a = np.ascontiguousarray(a, dtype=np.float64)

@@ -89,12 +89,27 @@ int main()
         double rq = 0.0;
         for (int64_t i = 0; i < D; ++i) rq += xh[i] * yh[i];
 
+        // quench as in the MBL drivers (Hubbard_EE_test.cu:44-50, 76-83) with Krylov<C>
+        // (HubMatrix_GPU.cuh:619-728): the product state up = sites 0-5, dn = sites 6-11, evolved
+        // by dt = 0.2 in an 80-step Krylov space, then its 6|6 entanglement entropy
+        std::vector<std::complex<double>> ph((size_t)D, 0.0);
+        ph[(size_t)HqUP.address[0x03f] * Hub.dimDN + HqDN.address[0xfc0]] = 1.0;
+        device_vector<std::complex<double>> psi0(ph), psi((size_t)D);
+        Krylov<double> kr(Hub.dimUP, Hub.dimDN);
+        kr.maxN = 80;
+        kr.Make_tridiagonal_eigen(Hub, psi0);
+        kr.Krylov_time_evol(Hub, psi0, psi, 0.2);
+        EEmatHubAB_GPU<std::complex<double>> EEq(Asite, 6, HqUP, HqDN);
+        const auto entq = EEq.entropy(psi);
+        const double qnorm = kr.norm(psi);
+
         const auto &ev = eigs.eigenvalues();
         std::printf("{\"E0\": %.17g, \"nsteps\": %d, \"SS\": %.17g, \"D\": %.17g, \"kinetic\": %.17g, "
                     "\"S_vN\": %.17g, \"S_ref\": %.17g, \"n_schmidt\": %lld, \"nconv\": %d, "
-                    "\"evals\": [%.17g, %.17g, %.17g], \"rayleigh_host\": %.17g}\n",
+                    "\"evals\": [%.17g, %.17g, %.17g], \"rayleigh_host\": %.17g, \"krylov_steps\": %d, "
+                    "\"quench_norm2\": %.17g, \"quench_S_vN\": %.17g}\n",
                     eval, lanc.nsteps, SS, Dw, kinetic, ent.S_vN, ent.S_ref, (long long)ent.n_schmidt, nconv, ev[0],
-                    ev[1], ev[2], rq);
+                    ev[1], ev[2], rq, (int)kr.a.size(), qnorm, entq.S_vN);
     } catch (const Error &e) {
         std::fprintf(stderr, "kagome12_driver: %s\n", e.what());
         return 1;

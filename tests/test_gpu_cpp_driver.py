@@ -56,3 +56,16 @@ def test_kagome12_cpp_driver_against_oracle():
     assert out["nconv"] == 3
     assert np.allclose(out["evals"], wa, rtol=1e-10, atol=0)
     assert abs(out["rayleigh_host"] - e0) < 1e-10 * abs(e0)
+    # Krylov quench of the product state (up on sites 0-5, dn on 6-11), dt = 0.2, 80 steps
+    assert out["krylov_steps"] == 80
+    assert abs(out["quench_norm2"] - 1.0) < 1e-9
+    sq = O.Species(12, 6, ops, True)
+    Hq = O.Hubbard(sq, sq, 4.0)
+    phi0 = np.zeros(Hq.dim, dtype=np.complex128)
+    adr = {int(s): k for k, s in enumerate(sq.state)}
+    phi0[adr[0x03f] * Hq.dim_dn + adr[0xfc0]] = 1.0
+    ak, bk = O.krylov_tridiagonal(Hq, phi0, 80, 1e-10)
+    mk, zk = O.tridiag_eigen(ak, bk)
+    phi = O.krylov_time_evol(Hq, phi0, ak, bk, mk, zk, 0.2)
+    lamq, _ = EE.block_spectra(12, 6, 6, 0x3f, phi)
+    assert abs(out["quench_S_vN"] - EE.entropy_from_lambda(lamq)) < 1e-8
