@@ -133,6 +133,41 @@ def ensemble_line(n_real, ncv=32):
             "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
 
 
+def krylov_line(max_steps=1000, dt=0.02):
+    """Krylov<C> quench step (HubMatrix_GPU.cuh:619-728) on the 14-site open chain, 7/7, U = 4,
+    W = 8 (realisation 0 of the ensemble): from the product state up = sites 0-6, dn = sites
+    7-13, the Lanczos recurrence to max_steps (b stays above 1e-10 here), all eigenpairs of T,
+    and the propagation exp(-i H dt) phi0 by the recomputed recurrence; complex128."""
+    import numpy as np
+    import torch
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Krylov, tridiag_eigen
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
+    H = DS.build_realisation(DS.realisation_eps(0, 14, 8.0))
+    hq = H_TBq(14, 7)
+    i0 = int(hq.address[0x007f].item()) * H.dimDN + int(hq.address[0x3f80].item())
+    phi0 = torch.zeros(H.dim, dtype=torch.complex128, device="cuda")
+    phi0[i0] = 1.0
+    phi = torch.zeros_like(phi0)
+    kr = Krylov(H.dimUP, H.dimDN, max_steps=max_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kr.tridiagonal(H, phi0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kr.m, kr.z = tridiag_eigen(kr.a, kr.b)
+    t2 = time.perf_counter()
+    kr.Krylov_time_evol(H, phi0, phi, dt)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    n = len(kr.a)
+    nrm = float(torch.linalg.vector_norm(phi).item())
+    log(f"[bench] Krylov quench step: {n} Lanczos steps {t1 - t0:.2f} s, eigenpairs of T {t2 - t1:.2f} s, "
+        f"propagation {t3 - t2:.2f} s, |phi| - 1 = {nrm - 1.0:.1e}")
+    return {"value": round(1.0 / (t3 - t0), 4), "unit": "Krylov time steps/s", "krylov_dim": n, "dt": dt,
+            "tridiagonal_s": round(t1 - t0, 3), "eigenpairs_s": round(t2 - t1, 3), "propagation_s": round(t3 - t2, 3),
+            "norm_error": abs(nrm - 1.0), "workload": "chain14_open_nup7_ndn7_U4_W8_krylov1000_complex128"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +286,7 @@ def main():
         torch.cuda.empty_cache()
         out["batched_drude"] = batched_drude_line()
         out["disorder_ensemble"] = ensemble_line(args.ensemble)
+        out["krylov_quench"] = krylov_line()
     print(json.dumps(out), flush=True)
 
 
