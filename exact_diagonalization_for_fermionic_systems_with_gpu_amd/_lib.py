@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libedhip.so")
 
 ED_OK, ED_EINVAL, ED_ENOMEM, ED_EBREAKDOWN, ED_ENOCONV, ED_EHIP, ED_EUNSUPPORTED, ED_ESLOT = range(8)
-ED_OP_HOP, ED_OP_CURRENT, ED_OP_DENSITY, ED_OP_DENSITY2, ED_OP_HOP_PHASE = range(5)
+ED_OP_HOP, ED_OP_CURRENT, ED_OP_DENSITY, ED_OP_DENSITY2, ED_OP_HOP_PHASE, ED_OP_FERMI1, ED_OP_FERMI2 = range(7)
 
 _p = C.c_void_p
 _i64 = C.c_int64
@@ -24,7 +24,7 @@ _pi = C.POINTER(C.c_int)
 
 
 class EdOp(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("kind", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("dag", C.c_int32),
                 ("value", C.c_double), ("phase", C.c_double)]
 
 
@@ -50,6 +50,7 @@ SIGNATURES = {
     "ed_device_sync": (_i32, []),
     "ed_basis_count": (_i32, [_i32, _i32, _pi64]),
     "ed_basis_build": (_i32, [_i32, _i32, _p, _p, _p]),
+    "ed_basis_build_lowmem": (_i32, [_i32, _i32, _p, _p]),
     "ed_tables_build": (_i32, [_i32, _i32, _p, _p, _i64, _p, _i32, _i32, _p, _p, _p]),
     "ed_hubbard_create": (_i32, [_i32, _i64, _i32, _p, _p, _p, _i64, _i32, _p, _p, _p, _i32, _d, _i64, _p,
                                  C.POINTER(_p)]),
@@ -58,6 +59,11 @@ SIGNATURES = {
     "ed_hubbard_info": (_i32, [_p, _pi64, _pi64, _pi]),
     "ed_hubbard_apply": (_i32, [_p, _i32, _p, _p, _p]),
     "ed_hubbard_autotune": (_i32, [_p, _i32, _p, _p, _i32, C.POINTER(HvTuning), _p]),
+    "ed_dense_zero": (_i32, [_i64, _i32, _p, _p]),
+    "ed_hubbard_write_dense": (_i32, [_p, _p, _p]),
+    "ed_tables_write_dense": (_i32, [_i64, _i32, _i32, _p, _p, _p, _p]),
+    "ed_dense_eigh": (_i32, [_i64, _i32, _p, _p, _p]),
+    "ed_ted_time_evol": (_i32, [_i64, _i32, _p, _p, _p, _d, _p, _p]),
     "ed_random_vector": (_i32, [_i64, C.c_uint32, _i32, _p, _i32, _p]),
     "ed_dot": (_i32, [_i64, _i32, _p, _p, _pd, _p]),
     "ed_axpy": (_i32, [_i64, _i32, _d, _p, _p, _p]),

@@ -175,6 +175,8 @@ extern "C" int ed_bond_bilinear(ed_hubbard hh, int species, int complex_vectors,
         return fail(ED_EINVAL, "ed_bond_bilinear: bad arguments");
     if (!h->otf.up.ops_own || !h->otf.dn.ops_own)
         return fail(ED_EINVAL, "ed_bond_bilinear: operator needs ed_hubbard_create_ops");
+    if (h->has_fermi)
+        return fail(ED_EUNSUPPORTED, "ed_bond_bilinear: Insert_1Fop / Insert_2Fop calls are not supported");
     const OtfSpecies &S = species ? h->otf.dn : h->otf.up;
     hipStream_t st = stream_of(stream);
     const int64_t D = h->otf.dim_up * h->otf.dim_dn;

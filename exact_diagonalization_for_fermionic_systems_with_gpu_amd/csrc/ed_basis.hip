@@ -4,7 +4,7 @@
 //   * Labeling (44-53): the sector basis is the ascending sequence of N-bit integers
 //     with popcount Q.  That order is the colex order, so state[r] is the colex unrank
 //     of r (one thread per rank, binomial table in LDS) and address[state[r]] = r.
-//   * stacking + Make_H/Make_J/Make_N/Insert_2Nop (64-149, 216-225): every row of the
+//   * stacking + Make_H/Make_J/Make_N/Insert_1Fop/Insert_2Fop/Insert_2Nop (64-225): every row of the
 //     column-major ELL table is independent and sees the operator calls in call order,
 //     so one thread owns one row, replays the op list and drops each element into the
 //     first slot whose value is exactly 0.0 -- the same slot the reference picks.
@@ -147,6 +147,27 @@ __global__ void k_basis_unrank(int N, int Q, int64_t count, const int64_t *__res
     address[s] = (int32_t)r0;
 }
 
+// LOW_MEM::H_TBq::Labeling (TotalSpin/include/fermi_operator_lowmem.h:50-64): prev_permutation
+// over occ[0..N-1], starting from Q ones first, visits the occupation patterns in descending
+// lexicographic order of (occ[0], occ[1], ...), i.e. descending value of the bit-reversed state.
+// Index r is therefore the colex unrank of count-1-r, bit-reversed over N bits.
+__global__ void k_basis_unrank_lowmem(int N, int Q, int64_t count, const int64_t *__restrict__ binom_flat,
+                                      uint64_t *__restrict__ state)
+{
+    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r0 >= count) return;
+    int64_t r = count - 1 - r0;
+    uint64_t s = 0;
+    int p = N - 1;
+    for (int k = Q; k >= 1; --k) {
+        while (binom_flat[p * 64 + k] > r) --p;
+        s |= (uint64_t)1 << (N - 1 - p); // bit p of the reversed value is site N-1-p
+        r -= binom_flat[p * 64 + k];
+        --p;
+    }
+    state[r0] = s;
+}
+
 } // namespace ed
 
 using namespace ed;
@@ -198,6 +219,22 @@ extern "C" int ed_basis_build(int N, int Q, int32_t *d_state, int32_t *d_address
     const int B = 256;
     const int64_t grid = (count + B - 1) / B;
     hipLaunchKernelGGL(k_basis_unrank, dim3((unsigned)grid), dim3(B), 0, st, N, Q, count, db, d_state, d_address);
+    ED_CHECK_LAUNCH();
+    return ED_OK;
+}
+
+extern "C" int ed_basis_build_lowmem(int N, int Q, uint64_t *d_state, void *stream)
+{
+    if (N < 1 || N > 63 || Q < 0 || Q > N || !d_state)
+        return fail(ED_EINVAL, "ed_basis_build_lowmem: bad N=%d Q=%d", N, Q);
+    const int64_t count = binom(N, Q);
+    if (count <= 0 || count > ((int64_t)1 << 31) - 1)
+        return fail(ED_EINVAL, "ed_basis_build_lowmem: C(%d,%d) does not fit an int rank", N, Q);
+    int64_t *db = device_binom();
+    if (!db) return fail(ED_ENOMEM, "ed_basis_build_lowmem: binomial table upload failed");
+    const int B = 256;
+    hipLaunchKernelGGL(k_basis_unrank_lowmem, dim3((unsigned)((count + B - 1) / B)), dim3(B), 0, stream_of(stream), N,
+                       Q, count, db, d_state);
     ED_CHECK_LAUNCH();
     return ED_OK;
 }
@@ -279,6 +316,22 @@ __global__ void k_tables(int Q, int S, int64_t count, const int32_t *__restrict_
                 vr = op.value;
             }
             break;
+        case ED_OP_FERMI1: // Insert_1Fop (fermi_operator.h:151-174): sign from the sites above i
+            if (oi == (op.pad & 1)) {
+                emit = true;
+                sign = (__popcll((uint64_t)n >> (op.i + 1)) & 1) ? -1.0 : 1.0;
+                col = address[n ^ ((int64_t)1 << op.i)];
+                vr = op.value;
+            }
+            break;
+        case ED_OP_FERMI2: // Insert_2Fop (fermi_operator.h:176-214)
+            if (oi == (op.pad & 1) && oj == ((op.pad >> 1) & 1)) {
+                emit = true;
+                sign = (count_between(n, op.i, op.j) & 1) ? -1.0 : 1.0;
+                col = address[n ^ (((int64_t)1 << op.i) | ((int64_t)1 << op.j))];
+                vr = op.value;
+            }
+            break;
         default: break;
         }
         if (!emit) continue;
@@ -317,8 +370,9 @@ extern "C" int ed_tables_build(int N, int Q, const int32_t *d_state, const int32
         return fail(ED_EINVAL, "ed_tables_build: bad arguments");
     for (int o = 0; o < nops; ++o) {
         const ed_op &op = ops[o];
-        if (op.kind < 0 || op.kind > 4 || op.i < 0 || op.i >= N || op.j < 0 || op.j >= N ||
-            (op.kind != ED_OP_DENSITY && op.i == op.j))
+        const bool one_site = op.kind == ED_OP_DENSITY || op.kind == ED_OP_FERMI1;
+        if (op.kind < 0 || op.kind > 6 || op.i < 0 || op.i >= N || (!one_site && (op.j < 0 || op.j >= N)) ||
+            (!one_site && op.i == op.j) || ((op.kind == ED_OP_FERMI1 || op.kind == ED_OP_FERMI2) && (op.dag & ~3)))
             return fail(ED_EINVAL, "ed_tables_build: bad op %d (kind %d, i %d, j %d)", o, op.kind, op.i, op.j);
         if (op.kind == ED_OP_HOP_PHASE && !complex_tables)
             return fail(ED_EINVAL, "ed_tables_build: a phase hop needs complex tables");
@@ -336,7 +390,10 @@ extern "C" int ed_tables_build(int N, int Q, const int32_t *d_state, const int32
         return s;
     }
     std::vector<DevOp> hops(nops);
-    for (int o = 0; o < nops; ++o) hops[o] = DevOp{ops[o].kind, ops[o].i, ops[o].j, 0, ops[o].value, ops[o].phase};
+    for (int o = 0; o < nops; ++o) {
+        const bool one_site = ops[o].kind == ED_OP_DENSITY || ops[o].kind == ED_OP_FERMI1;
+        hops[o] = DevOp{ops[o].kind, ops[o].i, one_site ? ops[o].i : ops[o].j, ops[o].dag, ops[o].value, ops[o].phase};
+    }
     int herr = 0;
     int status = ED_OK;
     do {

@@ -1024,9 +1024,12 @@ extern "C" int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int n
             s = fail(ED_EHIP, "ed_hubbard_create_ops: operator readback failed");
             break;
         }
+        for (int o = 0; o < nops_up; ++o) h->has_fermi |= ops_up[o].kind == ED_OP_FERMI1 || ops_up[o].kind == ED_OP_FERMI2;
+        for (int o = 0; o < nops_dn; ++o) h->has_fermi |= ops_dn[o].kind == ED_OP_FERMI1 || ops_dn[o].kind == ED_OP_FERMI2;
         int algo = (nops_up <= 64 && nops_dn <= 64) ? 3 : h->algo;
         if (const char *e = getenv("ED_HV_ALGO")) algo = atoi(e);
-        if (algo == 3 && (nops_up > 64 || nops_dn > 64)) algo = 2;
+        // the on-the-fly hop generation covers the particle-conserving calls only
+        if (algo == 3 && (nops_up > 64 || nops_dn > 64 || h->has_fermi)) algo = 2;
         if (algo < 1 || algo > 3) algo = 2;
         h->algo = algo;
         // on-the-fly pass 1 reads 128-column slabs (one 1 KB row segment per wavefront load,

@@ -98,6 +98,7 @@ struct Hubbard {
     int hv_uptab = 1;              // pass 1 reads d_up_tab (when it exists) instead of generating
     std::vector<OtfOp> h_ops_up, h_ops_dn; // host copies of otf.up/dn.ops (for ed_batch_create)
     std::vector<double> h_uvec;            // host copy of d_uvec (N entries), empty without one
+    bool has_fermi = false; // an Insert_1Fop / Insert_2Fop call: table kernels only
     void release();
 };
 

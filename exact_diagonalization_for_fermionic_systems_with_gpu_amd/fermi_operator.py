@@ -62,29 +62,44 @@ class H_TBq:
     # --- the operator-insert methods (call order is kept) ------------------------------
     def Make_H(self, i_site, j_site, t):
         """fermi_operator.h:100-115"""
-        self.ops.append((_lib.ED_OP_HOP, int(i_site), int(j_site), float(t), 0.0))
+        self.ops.append((_lib.ED_OP_HOP, int(i_site), int(j_site), float(t), 0.0, 0))
 
     def Make_J(self, i_site, j_site, t):
         """fermi_operator.h:117-138"""
-        self.ops.append((_lib.ED_OP_CURRENT, int(i_site), int(j_site), float(t), 0.0))
+        self.ops.append((_lib.ED_OP_CURRENT, int(i_site), int(j_site), float(t), 0.0, 0))
 
     def Make_N(self, i_site, v):
         """fermi_operator.h:140-149"""
-        self.ops.append((_lib.ED_OP_DENSITY, int(i_site), int(i_site), float(v), 0.0))
+        self.ops.append((_lib.ED_OP_DENSITY, int(i_site), int(i_site), float(v), 0.0, 0))
 
     def Insert_2Nop(self, i_site, j_site, v):
         """fermi_operator.h:216-225"""
-        self.ops.append((_lib.ED_OP_DENSITY2, int(i_site), int(j_site), float(v), 0.0))
+        self.ops.append((_lib.ED_OP_DENSITY2, int(i_site), int(j_site), float(v), 0.0, 0))
 
     def Make_H_phase(self, i_site, j_site, t, phase):
         """t e^{i phase} c+_i c_j + h.c. (complex tables only)."""
         if not self.complex_tables:
             raise ValueError("Make_H_phase needs H_TBq(..., complex_tables=True)")
-        self.ops.append((_lib.ED_OP_HOP_PHASE, int(i_site), int(j_site), float(t), float(phase)))
+        self.ops.append((_lib.ED_OP_HOP_PHASE, int(i_site), int(j_site), float(t), float(phase), 0))
+
+    def Insert_1Fop(self, i_dag, i_site, t):
+        """fermi_operator.h:151-174: t c^{i_dag}_i ('+' creates, '-' annihilates); full Fock
+        space only, as the reference says"""
+        self.ops.append((_lib.ED_OP_FERMI1, int(i_site), int(i_site), float(t), 0.0, _dag_bit(i_dag)))
+
+    def Insert_2Fop(self, i_dag, i_site, j_dag, j_site, t):
+        """fermi_operator.h:176-214: rows whose sites i, j are in the states the two operators
+        act on; sign (-1)^(occupied sites strictly between i and j)"""
+        if int(i_site) == int(j_site):
+            raise ValueError("Insert_2Fop needs two different sites")
+        self.ops.append((_lib.ED_OP_FERMI2, int(i_site), int(j_site), float(t), 0.0,
+                         _dag_bit(i_dag) | (_dag_bit(j_dag) << 1)))
 
     def add_ops(self, ops):
+        """ops: (kind, i, j, value[, phase[, dag]]) tuples"""
         for o in ops:
-            self.ops.append((int(o[0]), int(o[1]), int(o[2]), float(o[3]), float(o[4]) if len(o) > 4 else 0.0))
+            self.ops.append((int(o[0]), int(o[1]), int(o[2]), float(o[3]), float(o[4]) if len(o) > 4 else 0.0,
+                             int(o[5]) if len(o) > 5 else 0))
 
     def clear(self):
         """fermi_operator.h:237-241 (tables reset, basis kept)"""
@@ -94,8 +109,8 @@ class H_TBq:
     def ops_array(self):
         """the recorded calls as a C array of ed_op (for ed_tables_build / ed_hubbard_create_ops)"""
         arr = (_lib.EdOp * max(1, len(self.ops)))()
-        for k, (kind, i, j, v, ph) in enumerate(self.ops):
-            arr[k] = _lib.EdOp(kind, i, j, 0, v, ph)
+        for k, (kind, i, j, v, ph, dg) in enumerate(self.ops):
+            arr[k] = _lib.EdOp(kind, i, j, dg, v, ph)
         return arr, len(self.ops)
 
     # --- tables ---------------------------------------------------------------------------
@@ -128,12 +143,35 @@ class H_TBq:
         self._build()
         return self._Ax, self._Aj
 
-    def Write(self):
-        """Dense count x count matrix of the operator (fermi_operator.h:227-235), on the device."""
-        Ax, Aj = self.Ax, self.Aj
+    def Write(self, Mat: torch.Tensor | None = None):
+        """H_TBq::Write (fermi_operator.h:227-235): Mat[j, Aj] += Ax over the nonzero slots
+        (ed_tables_write_dense).  Mat: a count x count float64 (complex128 for complex tables)
+        device tensor, added into; None allocates a zeroed one.  Returns Mat."""
+        Ax, Aj = self.raw_tables()
         dt = torch.complex128 if self.complex_tables else torch.float64
-        M = torch.zeros(self.count * self.count, dtype=dt, device=self.device)
-        rows = torch.arange(self.count, device=self.device, dtype=torch.int64).repeat(self.slots)
-        nz = Ax != 0
-        M.index_add_(0, (rows * self.count + Aj.to(torch.int64))[nz], Ax[nz])
-        return M.view(self.count, self.count)
+        if Mat is None:
+            Mat = torch.zeros((self.count, self.count), dtype=dt, device=self.device)
+        if Mat.numel() != self.count * self.count or Mat.dtype != dt or not Mat.is_contiguous():
+            raise ValueError(f"Mat must be a contiguous {dt} tensor of count^2 elements")
+        call("ed_tables_write_dense", self.count, self.slots, int(self.complex_tables), dptr(Ax), dptr(Aj),
+             dptr(Mat), stream_ptr())
+        return Mat
+
+
+def _dag_bit(c) -> int:
+    if c == "+":
+        return 0
+    if c == "-":
+        return 1
+    raise ValueError("operator sign must be '+' or '-'")  # the reference prints "wrong sign!"
+
+
+def lowmem_basis(N: int, Q: int, device="cuda") -> torch.Tensor:
+    """LOW_MEM::H_TBq basis (TotalSpin/include/fermi_operator_lowmem.h:50-64): the C(N,Q) states
+    of N <= 63 sites as uint64 values (in an int64 tensor), in std::prev_permutation order
+    (descending bit-reversed value).  The rank of a state is its index."""
+    import math
+    n = math.comb(int(N), int(Q))
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    call("ed_basis_build_lowmem", int(N), int(Q), dptr(out), stream_ptr())
+    return out

@@ -104,6 +104,24 @@ class Hubbard:
         self.Hubbard_mv(x, y)
         return y
 
+    def Write(self, Hmat: torch.Tensor):
+        """Hubbard::Write (HubMatrix_GPU.cuh:219-230): ADDS H into the dense D x D matrix Hmat
+        (row-major, Hmat[i_up*dimDN + i_dn, col]; float64, complex128 for complex tables)."""
+        self._check_dense(Hmat)
+        call("ed_hubbard_write_dense", self._h, _raw(Hmat), stream_ptr())
+
+    Write_pinned = Write  # HubMatrix_GPU.cuh:232-248 (the target is any device-visible buffer)
+
+    def initialize(self, Hmat: torch.Tensor):
+        """Hubbard::initialize (HubMatrix_GPU.cuh:257-277): zero the dense matrix."""
+        self._check_dense(Hmat)
+        call("ed_dense_zero", self.dim, 1 if Hmat.is_complex() else 0, _raw(Hmat), stream_ptr())
+
+    def _check_dense(self, Hmat):
+        dt = torch.complex128 if self.complex_tables else torch.float64
+        if Hmat.numel() != self.dim * self.dim or Hmat.dtype != dt or not Hmat.is_cuda or not Hmat.is_contiguous():
+            raise ValueError(f"Hmat must be a contiguous CUDA {dt} tensor of (dimUP*dimDN)^2 elements")
+
     def _check(self, *ts):
         for t in ts:
             if t.numel() != self.dim or not t.is_cuda or not t.is_contiguous():
@@ -282,6 +300,58 @@ class Krylov(Thrust_Lin):
         P = C.POINTER(C.c_double)
         call("ed_krylov_time_evol", Hub.handle, _raw(phi0), n, a.ctypes.data_as(P), b.ctypes.data_as(P),
              m.ctypes.data_as(P), z.ctypes.data_as(P), float(dt), _raw(phi), stream_ptr())
+
+
+def cusolverDsyevd(A: torch.Tensor, W: torch.Tensor):
+    """cusolverDsyevd / cusolverZheevd (EIGEN_cusolver.h:16-63, 65-112) on rocSOLVER: all
+    eigenpairs of the dense Hermitian matrix A (n x n, written by Hubbard.Write or H_TBq.Write).
+    W (float64, n) receives the eigenvalues ascending, A the eigenvectors: A.view(n, n)[k] is the
+    unit eigenvector of W[k] (the reference reads eigenvector k at A + k*n)."""
+    n = int(round(math.sqrt(A.numel())))
+    if n * n != A.numel() or W.numel() != n or W.dtype != torch.float64 or not A.is_contiguous():
+        raise ValueError("A must be a contiguous n x n matrix and W a float64 vector of n")
+    if A.dtype not in (torch.float64, torch.complex128):
+        raise ValueError("A is float64 or complex128")
+    call("ed_dense_eigh", n, 1 if A.is_complex() else 0, _raw(A), _raw(W), stream_ptr())
+
+
+cusolverZheevd = cusolverDsyevd
+
+
+class tED(Thrust_Lin):
+    """tED<C> (HubMatrix_GPU.cuh:730-814): exact propagation in the eigenbasis of the dense
+    Hamiltonian.  W (eigenvalues) and V (eigenvectors, V.view(D, D)[k] = eigenvector k) live on
+    the device."""
+
+    def __init__(self, dimUP: int, dimDN: int):
+        super().__init__(dimUP, dimDN)
+        self.dim = int(dimUP) * int(dimDN)
+        self.W = None
+        self.V = None
+
+    def Make_eigen(self, Hmat: torch.Tensor):
+        """HubMatrix_GPU.cuh:756-767: V = Hmat (copied), then the dense eigensolver."""
+        if Hmat.numel() != self.dim * self.dim:
+            raise ValueError("Hmat must hold (dimUP*dimDN)^2 elements")
+        self.V = Hmat.reshape(-1).clone()
+        self.W = torch.empty(self.dim, dtype=torch.float64, device=Hmat.device)
+        cusolverDsyevd(self.V, self.W)
+
+    def tED_time_evol(self, phi0: torch.Tensor, phi: torch.Tensor, dt: float):
+        """HubMatrix_GPU.cuh:777-790: phi += sum_k exp(-i W_k dt) <V_k|phi0> V_k (phi accumulates,
+        as in the reference; pass zeros for exp(-i H dt) phi0).  complex128 vectors."""
+        if self.V is None:
+            raise ValueError("tED_time_evol needs Make_eigen first")
+        for t in (phi0, phi):
+            if t.numel() != self.dim or t.dtype != torch.complex128 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("phi0 and phi are contiguous complex128 CUDA vectors of dimUP*dimDN")
+        call("ed_ted_time_evol", self.dim, 1 if self.V.is_complex() else 0, _raw(self.V), _raw(self.W),
+             _raw(phi0), float(dt), _raw(phi), stream_ptr())
+
+    def clear(self):
+        """HubMatrix_GPU.cuh:807-812"""
+        self.W = None
+        self.V = None
 
 
 def correlator_continued_fraction(lanctri: Lanczos, H: Hubbard, nw: int, w, energy: float, phi: torch.Tensor,

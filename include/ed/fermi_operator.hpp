@@ -6,6 +6,8 @@
 // replaying the calls in order gives the reference's slot placement exactly.
 #pragma once
 
+#include <map>
+
 #include "ed.hpp"
 
 namespace ed {
@@ -51,7 +53,30 @@ class H_TBq {
         if (!cplx) throw Error(ED_EINVAL, "Make_H_phase needs complex tables");
         ops.push_back(ed_op{ED_OP_HOP_PHASE, i, j, 0, t, phase});
     }
+    // Insert_1Fop(i_dag, i, t) (fermi_operator.h:151-174): t c^{i_dag}_i, full Fock space only
+    void Insert_1Fop(const char i_dag, int i, double t)
+    {
+        ops.push_back(ed_op{ED_OP_FERMI1, i, i, dag_bit(i_dag), t, 0.0});
+    }
+    // Insert_2Fop(i_dag, i, j_dag, j, t) (fermi_operator.h:176-214)
+    void Insert_2Fop(const char i_dag, int i, const char j_dag, int j, double t)
+    {
+        ops.push_back(ed_op{ED_OP_FERMI2, i, j, dag_bit(i_dag) | (dag_bit(j_dag) << 1), t, 0.0});
+    }
     void clear() { ops.clear(); } // fermi_operator.h:237-241 (basis kept)
+
+    // H_TBq::Write (fermi_operator.h:227-235): Mat[j*count + Aj] += Ax over the nonzero slots;
+    // Mat holds count*count doubles (real tables)
+    void Write(device_vector<double> &Mat) const
+    {
+        if (cplx) throw Error(ED_EINVAL, "H_TBq::Write: real tables only");
+        if (Mat.size() != (size_t)(count * count)) throw Error(ED_EINVAL, "H_TBq::Write: Mat must hold count^2");
+        device_vector<double> Ax;
+        device_vector<int32_t> Aj;
+        tables(Ax, Aj);
+        check(ed_tables_write_dense(count, slots(), 0, Ax.data(), Aj.data(), Mat.data(), nullptr), "H_TBq::Write");
+        hip_check(hipDeviceSynchronize(), "H_TBq::Write");
+    }
 
     bool complex_tables() const { return cplx; }
     bool full_fock() const { return full; }
@@ -72,6 +97,12 @@ class H_TBq {
     }
 
   private:
+    static int dag_bit(char c)
+    {
+        if (c == '+') return 0;
+        if (c == '-') return 1;
+        throw Error(ED_EINVAL, "H_TBq: operator sign must be '+' or '-'"); // the reference prints "wrong sign!"
+    }
     void init(int q)
     {
         dim = int64_t(1) << N;
@@ -87,5 +118,29 @@ class H_TBq {
     bool cplx = false, full = false;
     std::vector<ed_op> ops;
 };
+
+namespace LOW_MEM {
+// LOW_MEM::H_TBq basis (TotalSpin/include/fermi_operator_lowmem.h:50-64): 64-bit states in
+// prev_permutation order; address is the rank map the reference keeps in a std::map.
+struct H_TBq_basis {
+    H_TBq_basis(int N_, int Q_) : N(N_), Q(Q_)
+    {
+        int64_t n = 0;
+        if (Q_ < 0 || Q_ > N_ || N_ > 63) throw Error(ED_EINVAL, "LOW_MEM::H_TBq: bad N, Q");
+        long double c = 1;
+        for (int k = 0; k < Q_; ++k) c = c * (N_ - k) / (k + 1);
+        n = (int64_t)(c + 0.5L);
+        device_vector<uint64_t> d((size_t)n);
+        check(ed_basis_build_lowmem(N_, Q_, d.data(), nullptr), "LOW_MEM::H_TBq");
+        hip_check(hipDeviceSynchronize(), "LOW_MEM::H_TBq");
+        state = d.to_host();
+        count = (int)n;
+        for (int a = 0; a < count; ++a) address[state[a]] = a;
+    }
+    int N, Q, count = 0;
+    std::vector<uint64_t> state;
+    std::map<uint64_t, int> address;
+};
+} // namespace LOW_MEM
 
 } // namespace ed

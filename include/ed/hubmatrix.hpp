@@ -57,6 +57,28 @@ class Hubbard {
     ed_hubbard handle() const { return h; }
     bool complex_tables() const { return cplx_tables; }
 
+    // Hubbard::Write (HubMatrix_GPU.cuh:219-230): ADDS H into the (dimUP*dimDN)^2 row-major dense
+    // matrix Hmat (complex tables: interleaved complex entries, 2x the doubles)
+    void Write(device_vector<double> &Hmat) const
+    {
+        const size_t D = (size_t)(dimUP * dimDN);
+        if (Hmat.size() != D * D * (cplx_tables ? 2 : 1)) throw Error(ED_EINVAL, "Hubbard::Write: Hmat size");
+        check(ed_hubbard_write_dense(h, Hmat.data(), nullptr), "Hubbard::Write");
+        hip_check(hipDeviceSynchronize(), "Hubbard::Write");
+    }
+    // Hubbard::Write_pinned (HubMatrix_GPU.cuh:232-248): Hmat is any device-accessible pointer
+    void Write_pinned(double *Hmat) const
+    {
+        check(ed_hubbard_write_dense(h, Hmat, nullptr), "Hubbard::Write_pinned");
+        hip_check(hipDeviceSynchronize(), "Hubbard::Write_pinned");
+    }
+    // Hubbard::initialize (HubMatrix_GPU.cuh:257-277): zero the dense matrix
+    void initialize(double *Hmat) const
+    {
+        check(ed_dense_zero(dimUP * dimDN, cplx_tables ? 1 : 0, Hmat, nullptr), "Hubbard::initialize");
+        hip_check(hipDeviceSynchronize(), "Hubbard::initialize");
+    }
+
     int64_t dimUP, dimDN;
     int N;
     double U;
@@ -202,6 +224,48 @@ class Krylov : public Thrust_Lin<std::complex<Cr>> {
                                   dt, phi.data(), nullptr),
               "Krylov::Krylov_time_evol");
         hip_check(hipDeviceSynchronize(), "Krylov::Krylov_time_evol");
+    }
+};
+
+// cusolverDsyevd (EIGEN_cusolver.h:16-63): eigenvalues ascending into d_W, eigenvectors over
+// d_A (vector k contiguous at d_A + k*m); rocSOLVER syevd
+inline void cusolverDsyevd(const int m, double *d_A, double *d_W)
+{
+    check(ed_dense_eigh(m, 0, d_A, d_W, nullptr), "cusolverDsyevd");
+}
+// cusolverZheevd (EIGEN_cusolver.h:65-112), complex128 Hermitian
+inline void cusolverZheevd(const int m, std::complex<double> *d_A, double *d_W)
+{
+    check(ed_dense_eigh(m, 1, d_A, d_W, nullptr), "cusolverZheevd");
+}
+
+// tED<C> (HubMatrix_GPU.cuh:730-814): propagation in the eigenbasis of the dense Hamiltonian.
+// W holds the eigenvalues, V the eigenvectors (V[k*D ...] is eigenvector k), both on the device.
+template <typename Cr>
+class tED : public Thrust_Lin<Cr> {
+  public:
+    tED(int64_t dimUP_, int64_t dimDN_) : Thrust_Lin<Cr>(dimUP_, dimDN_), W((size_t)(dimUP_ * dimDN_)) {}
+    device_vector<double> W;
+    device_vector<Cr> V;
+
+    // Make_eigen (HubMatrix_GPU.cuh:756-767): V = Hmat, then syevd
+    void Make_eigen(const device_vector<Cr> &Hmat)
+    {
+        V = Hmat;
+        const int64_t D = this->dimUP * this->dimDN;
+        if (W.size() != (size_t)D) W.resize((size_t)D);
+        check(ed_dense_eigh(D, 0, V.data(), W.data(), nullptr), "tED::Make_eigen");
+    }
+    // tED_time_evol (HubMatrix_GPU.cuh:777-790): phi += sum_k e^{-i W_k dt} <V_k|phi0> V_k
+    void tED_time_evol(const device_vector<std::complex<Cr>> &phi0, device_vector<std::complex<Cr>> &phi, double dt)
+    {
+        const int64_t D = this->dimUP * this->dimDN;
+        check(ed_ted_time_evol(D, 0, V.data(), W.data(), phi0.data(), dt, phi.data(), nullptr), "tED::tED_time_evol");
+    }
+    void clear() // HubMatrix_GPU.cuh:807-812
+    {
+        W.resize((size_t)(this->dimUP * this->dimDN));
+        V.clear();
     }
 };
 

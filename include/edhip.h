@@ -45,14 +45,21 @@ typedef enum {
     ED_OP_CURRENT = 1,   /* H_TBq::Make_J(i,j,t)        fermi_operator.h:117-138 */
     ED_OP_DENSITY = 2,   /* H_TBq::Make_N(i,v)          fermi_operator.h:140-149 */
     ED_OP_DENSITY2 = 3,  /* H_TBq::Insert_2Nop(i,j,v)   fermi_operator.h:216-225 */
-    ED_OP_HOP_PHASE = 4  /* t e^{i phase} c+_i c_j + h.c. (complex tables; a Peierls twist --
+    ED_OP_HOP_PHASE = 4, /* t e^{i phase} c+_i c_j + h.c. (complex tables; a Peierls twist --
                             no counterpart in the reference, whose tables are real) */
+    ED_OP_FERMI1 = 5,    /* H_TBq::Insert_1Fop(dag_i, i, t)          fermi_operator.h:151-174:
+                            t c^{dag_i}_i, sign (-1)^(occupied sites above i); full Fock space */
+    ED_OP_FERMI2 = 6     /* H_TBq::Insert_2Fop(dag_i, i, dag_j, j, t) fermi_operator.h:176-214:
+                            rows with occ_i, occ_j matching the two operators, sign
+                            (-1)^(occupied sites strictly between i and j) */
 } ed_op_kind;
 
 typedef struct {
     int32_t kind; /* ed_op_kind */
-    int32_t i, j; /* sites (j unused for ED_OP_DENSITY) */
-    int32_t pad;
+    int32_t i, j; /* sites (j unused for ED_OP_DENSITY and ED_OP_FERMI1) */
+    int32_t dag;  /* ED_OP_FERMI1/2 only: bit 0 = 1 if the operator on site i is an annihilator
+                     ('-' in the reference, which acts on rows with site i occupied), 0 for a
+                     creator ('+'); bit 1 the same for site j.  0 for the other kinds. */
     double value; /* t or v */
     double phase; /* ED_OP_HOP_PHASE only */
 } ed_op;
@@ -70,8 +77,15 @@ int ed_basis_count(int N, int Q, int64_t *count);
  * d_state[count], d_address[2^N] (rank inside the sector, 0 elsewhere). */
 int ed_basis_build(int N, int Q, int32_t *d_state, int32_t *d_address, void *stream);
 
-/* H_TBq::Make_H / Make_J / Make_N / Insert_2Nop applied in call order with the
- * stacking rule (fermi_operator.h:64-149,216-225).  ops is a host array.  The
+/* LOW_MEM::H_TBq::Labeling (TotalSpin/include/fermi_operator_lowmem.h:50-64): the sector basis
+ * of 64-bit states in the order std::prev_permutation visits them from the pattern with the Q
+ * ones first (descending value of the bit-reversed state), for N <= 63.  d_state[count]
+ * receives the states; the rank of a state is its index (the reference keeps a std::map).
+ * Asynchronous. */
+int ed_basis_build_lowmem(int N, int Q, uint64_t *d_state, void *stream);
+
+/* H_TBq::Make_H / Make_J / Make_N / Insert_1Fop / Insert_2Fop / Insert_2Nop applied in call
+ * order with the stacking rule (fermi_operator.h:64-225).  ops is a host array.  The
  * tables are cleared first (H_TBq::clear, fermi_operator.h:237-241), so one call
  * builds a whole operator; d_Ax holds count*S doubles (2x for complex tables).
  * Returns ED_ESLOT if a row overflows its S slots. */
@@ -91,10 +105,13 @@ int ed_hubbard_create(int N, int64_t dim_up, int S_up, const double *d_AxUP, con
                       const int32_t *d_AjDN, const int32_t *d_stateDN, int complex_tables, double U, int64_t mask,
                       const double *Uvec, ed_hubbard *out);
 /* Hubbard(HqUP, HqDN, U [, dilution]) built straight from the two H_TBq call lists
- * (the Make_H / Make_J / Make_N / Insert_2Nop calls of fermi_operator.h:100-225, in call
- * order, as ed_tables_build takes them).  Builds the bases and tables itself and, when each
- * list has at most 64 calls, applies H with hop descriptors generated on the fly from the
- * call list (same matrix elements, same per-element summation order as the tables). */
+ * (the Make_H / Make_J / Make_N / Insert_1Fop / Insert_2Fop / Insert_2Nop calls of
+ * fermi_operator.h:100-225, in call order, as ed_tables_build takes them).  Builds the bases
+ * and tables itself and, when each list has at most 64 calls and no Insert_1Fop / Insert_2Fop
+ * call, applies H with hop descriptors generated on the fly from the call list (same matrix
+ * elements, same per-element summation order as the tables); otherwise with the table
+ * kernels.  Operators with Insert_1Fop / Insert_2Fop calls are not accepted by the batch
+ * engine, the autotuner or the bilinear (gradient) entry points (ED_EUNSUPPORTED). */
 int ed_hubbard_create_ops(int N, int Q_up, const ed_op *ops_up, int nops_up, int Q_dn, const ed_op *ops_dn,
                           int nops_dn, int complex_tables, double U, int64_t mask, const double *Uvec,
                           ed_hubbard *out);
@@ -123,6 +140,34 @@ typedef struct {
  * choice and its time. */
 int ed_hubbard_autotune(ed_hubbard h, int complex_vectors, const void *d_x, void *d_y, int reps, ed_hv_tuning *best,
                         void *stream);
+
+/* ---------------- dense full-spectrum path ---------------------------------------- */
+
+/* Hubbard::initialize (HubMatrix_GPU.cuh:250-270): zero the n x n matrix d_A (complex128 when
+ * complex_values).  Asynchronous. */
+int ed_dense_zero(int64_t n, int complex_values, void *d_A, void *stream);
+/* Hubbard::Write (HubMatrix_GPU.cuh:184-230, kernel write_): ADDS H into the dense D x D matrix
+ * d_H, D = dim_up*dim_dn, row-major d_H[row*D + col] with row = i_up*dim_dn + i_dn, the
+ * reference's indexing; per row the nonzero UP slots, then the nonzero DN slots, then the
+ * interaction.  complex128 entries for an operator with complex tables.  D <= 65536.
+ * Asynchronous. */
+int ed_hubbard_write_dense(ed_hubbard h, void *d_H, void *stream);
+/* H_TBq::Write (fermi_operator.h:227-235): d_M[j*count + Aj] += Ax over the nonzero slots of
+ * the ELL tables of ed_tables_build (count <= 65536).  Asynchronous. */
+int ed_tables_write_dense(int64_t count, int S, int complex_tables, const double *d_Ax, const int32_t *d_Aj,
+                          void *d_M, void *stream);
+/* cusolverDsyevd / cusolverZheevd (EIGEN_cusolver.h:16-63, 65-112), as tED<C>::Make_eigen and the
+ * TotalSpinFull driver call them: all eigenpairs of the Hermitian n x n matrix d_A written by
+ * the functions above (n <= 46340).  d_W[n] (device) receives the eigenvalues ascending and d_A
+ * the unit eigenvectors, eigenvector k contiguous at d_A + k*n elements.  rocSOLVER syevd/heevd.
+ * Synchronises; ED_ENOCONV when the solver reports a failure. */
+int ed_dense_eigh(int64_t n, int complex_values, void *d_A, double *d_W, void *stream);
+/* tED<C>::tED_time_evol (HubMatrix_GPU.cuh:784-797): d_phi += sum_k exp(-i W_k dt) <V_k|phi0> V_k
+ * over the n eigenpairs from ed_dense_eigh (d_V real, or complex128 when complex_values); phi0
+ * and phi are complex128 device vectors of n elements (phi accumulates, as in the reference:
+ * zero it for exp(-iH dt) phi0).  Synchronises. */
+int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const double *d_W, const void *d_phi0,
+                     double dt, void *d_phi, void *stream);
 
 /* ---------------- vector BLAS-1 (Thrust_Lin, HubMatrix_GPU.cuh:291-471) -------- */
 

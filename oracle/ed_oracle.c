@@ -43,9 +43,18 @@ enum {
     OR_OP_CURRENT = 1,   /* Make_J(i,j,t)           fermi_operator.h:117-138 */
     OR_OP_DENSITY = 2,   /* Make_N(i,v)             fermi_operator.h:140-149 */
     OR_OP_DENSITY2 = 3,  /* Insert_2Nop(i,j,v)      fermi_operator.h:216-225 */
-    OR_OP_HOP_PHASE = 4  /* Make_H with a Peierls phase (complex tables only; no
+    OR_OP_HOP_PHASE = 4, /* Make_H with a Peierls phase (complex tables only; no
                             counterpart in the reference, which has real tables) */
+    OR_OP_FERMI1 = 5,    /* Insert_1Fop(dag,i,t)    fermi_operator.h:151-174 */
+    OR_OP_FERMI2 = 6     /* Insert_2Fop(dag_i,i,dag_j,j,t) fermi_operator.h:176-214 */
 };
+
+/* Insert_1Fop / Insert_2Fop (fermi_operator.h:151-214).  dag bit 0 is i_bit ('+' -> 0: the
+ * operator acts on rows with site i empty; '-' -> 1: occupied), bit 1 is j_bit.  Returns 1 and
+ * the column and sign when row state n has an element.  Insert_1Fop's sign counts the occupied
+ * sites strictly between i and N (count1_bt_sites(n, i_site, N), fermi_operator.h:166). */
+static int fermi_entry(int N, int kind, int i, int j, int dag, int64_t n, const int32_t *address, int32_t *b,
+                       double *sign);
 
 /* ---- a1: count1, Kernighan popcount (fermi_operator.h:19-24) ---------------- */
 int or_count1(int64_t state)
@@ -68,6 +77,24 @@ int or_count1_bt(int64_t basis, int i_site, int j_site)
         b = b % ((int64_t)1 << (i_site - j_site - 1));
     }
     return or_count1(b);
+}
+
+static int fermi_entry(int N, int kind, int i, int j, int dag, int64_t n, const int32_t *address, int32_t *b,
+                       double *sign)
+{
+    int i_bit = dag & 1, j_bit = (dag >> 1) & 1;
+    int oi = (int)((n >> i) & 1);
+    if (kind == OR_OP_FERMI1) {
+        if (oi != i_bit) return 0;
+        *sign = pow(-1.0, or_count1_bt(n, i, N));
+        *b = address[n ^ ((int64_t)1 << i)];
+        return 1;
+    }
+    int oj = (int)((n >> j) & 1);
+    if (oi != i_bit || oj != j_bit) return 0;
+    *sign = pow(-1.0, or_count1_bt(n, i, j));
+    *b = address[n ^ (((int64_t)1 << i) | ((int64_t)1 << j))];
+    return 1;
 }
 
 /* ---- a3: H_TBq::Labeling (fermi_operator.h:44-53) and Full_Labeling (55-62).
@@ -120,10 +147,10 @@ static int stack_cplx(double complex *Ax, int32_t *Aj, int64_t count, int S, int
  * (fermi_operator.h:100-149, 216-225).  Real tables.  Ax/Aj hold S*count
  * zero-initialised entries, S = 10*Q+1. --------------------------------------- */
 int or_tables(int N, int Q, const int32_t *state, const int32_t *address, int64_t count, int nops,
-              const int32_t *kind, const int32_t *si, const int32_t *sj, const double *val, double *Ax, int32_t *Aj)
+              const int32_t *kind, const int32_t *si, const int32_t *sj, const double *val, double *Ax, int32_t *Aj,
+              const int32_t *dag)
 {
     int S = OR_Z * (Q < 0 ? N : Q) + 1;
-    (void)N;
     for (int o = 0; o < nops; o++) {
         int i = si[o], j = sj[o];
         double t = val[o];
@@ -146,6 +173,11 @@ int or_tables(int N, int Q, const int32_t *state, const int32_t *address, int64_
                 if (oi == 1) st = stack_real(Ax, Aj, count, S, a, (int32_t)a, 1.0, t);
             } else if (kind[o] == OR_OP_DENSITY2) {
                 if (oi == 1 && oj == 1) st = stack_real(Ax, Aj, count, S, a, (int32_t)a, 1.0, t);
+            } else if (kind[o] == OR_OP_FERMI1 || kind[o] == OR_OP_FERMI2) {
+                int32_t b;
+                double sg;
+                if (fermi_entry(N, kind[o], i, j, dag ? dag[o] : 0, n, address, &b, &sg))
+                    st = stack_real(Ax, Aj, count, S, a, b, sg, t);
             } else {
                 return OR_EINVAL;
             }
@@ -161,11 +193,10 @@ int or_tables(int N, int Q, const int32_t *state, const int32_t *address, int64_
  * the phase phi for OR_OP_HOP_PHASE and ignored otherwise.  Ax is interleaved (re,im). */
 int or_tables_z(int N, int Q, const int32_t *state, const int32_t *address, int64_t count, int nops,
                 const int32_t *kind, const int32_t *si, const int32_t *sj, const double *val, const double *phase,
-                double *Ax_ri, int32_t *Aj)
+                double *Ax_ri, int32_t *Aj, const int32_t *dag)
 {
     int S = OR_Z * (Q < 0 ? N : Q) + 1;
     double complex *Ax = (double complex *)Ax_ri;
-    (void)N;
     for (int o = 0; o < nops; o++) {
         int i = si[o], j = sj[o];
         double t = val[o];
@@ -192,6 +223,10 @@ int or_tables_z(int N, int Q, const int32_t *state, const int32_t *address, int6
                 if (oi == 1) st = stack_cplx(Ax, Aj, count, S, a, (int32_t)a, 1.0, t);
             } else if (kind[o] == OR_OP_DENSITY2) {
                 if (oi == 1 && oj == 1) st = stack_cplx(Ax, Aj, count, S, a, (int32_t)a, 1.0, t);
+            } else if (kind[o] == OR_OP_FERMI1 || kind[o] == OR_OP_FERMI2) {
+                double sg;
+                if (fermi_entry(N, kind[o], i, j, dag ? dag[o] : 0, n, address, &b, &sg))
+                    st = stack_cplx(Ax, Aj, count, S, a, b, sg, t);
             } else {
                 return OR_EINVAL;
             }

@@ -18,7 +18,7 @@ _LIB = None
 
 Z_SLOTS = 10  # fermi_operator.h:11
 
-OP_HOP, OP_CURRENT, OP_DENSITY, OP_DENSITY2, OP_HOP_PHASE = 0, 1, 2, 3, 4
+OP_HOP, OP_CURRENT, OP_DENSITY, OP_DENSITY2, OP_HOP_PHASE, OP_FERMI1, OP_FERMI2 = 0, 1, 2, 3, 4, 5, 6
 
 _p = C.c_void_p
 _i64 = C.c_int64
@@ -42,9 +42,9 @@ def lib():
         L.or_basis.restype = _i64
         L.or_basis.argtypes = [C.c_int, C.c_int, _p, _p]
         L.or_tables.restype = C.c_int
-        L.or_tables.argtypes = [C.c_int, C.c_int, _p, _p, _i64, C.c_int, _p, _p, _p, _p, _p, _p]
+        L.or_tables.argtypes = [C.c_int, C.c_int, _p, _p, _i64, C.c_int, _p, _p, _p, _p, _p, _p, _p]
         L.or_tables_z.restype = C.c_int
-        L.or_tables_z.argtypes = [C.c_int, C.c_int, _p, _p, _i64, C.c_int, _p, _p, _p, _p, _p, _p, _p]
+        L.or_tables_z.argtypes = [C.c_int, C.c_int, _p, _p, _i64, C.c_int, _p, _p, _p, _p, _p, _p, _p, _p]
         L.or_hubbard_apply.restype = None
         L.or_hubbard_apply.argtypes = [_i64, _i64, C.c_int, _p, _p, C.c_int, _p, _p, _p, _p, C.c_double, _i64,
                                        _p, _p, _p]
@@ -121,7 +121,8 @@ class Species:
             self.add(ops)
 
     def add(self, ops):
-        """ops: iterable of (kind, i, j, value[, phase])."""
+        """ops: iterable of (kind, i, j, value[, phase[, dag]]); dag (OP_FERMI1/2 only): bit 0 = 1
+        for an annihilator on site i ('-'), bit 1 the same for site j."""
         ops = list(ops)
         self.ops += ops
         k = np.array([o[0] for o in ops], dtype=np.int32)
@@ -129,15 +130,30 @@ class Species:
         j = np.array([o[2] for o in ops], dtype=np.int32)
         v = np.array([o[3] for o in ops], dtype=np.float64)
         ph = np.array([o[4] if len(o) > 4 else 0.0 for o in ops], dtype=np.float64)
+        dg = np.array([o[5] if len(o) > 5 else 0 for o in ops], dtype=np.int32)
         if self.complex:
             st = lib().or_tables_z(self.N, self.Q, _ptr(self.state), _ptr(self.address), self.count, len(ops),
-                                   _ptr(k), _ptr(i), _ptr(j), _ptr(v), _ptr(ph), _ptr(self.Ax), _ptr(self.Aj))
+                                   _ptr(k), _ptr(i), _ptr(j), _ptr(v), _ptr(ph), _ptr(self.Ax), _ptr(self.Aj),
+                                   _ptr(dg))
         else:
             st = lib().or_tables(self.N, self.Q, _ptr(self.state), _ptr(self.address), self.count, len(ops),
-                                 _ptr(k), _ptr(i), _ptr(j), _ptr(v), _ptr(self.Ax), _ptr(self.Aj))
+                                 _ptr(k), _ptr(i), _ptr(j), _ptr(v), _ptr(self.Ax), _ptr(self.Aj), _ptr(dg))
         if st != 0:
             raise RuntimeError(f"oracle table build failed with status {st}")
         return self
+
+    def dense(self):
+        """H_TBq::Write (fermi_operator.h:227-235): M[j, Aj] += Ax over the nonzero slots, slot by
+        slot in ascending order (numpy; each slot touches each row once)."""
+        n = self.count
+        M = np.zeros((n, n), dtype=self.Ax.dtype)
+        rows = np.arange(n)
+        for s_ in range(self.slots):
+            ax = self.Ax[s_ * n:(s_ + 1) * n]
+            aj = self.Aj[s_ * n:(s_ + 1) * n]
+            nz = ax != 0
+            np.add.at(M, (rows[nz], aj[nz]), ax[nz])
+        return M
 
 
 class Hubbard:
@@ -180,6 +196,36 @@ class Hubbard:
         lib().or_hubbard_apply_rows(self.dim_up, self.dim_dn, up.slots, _ptr(up.Ax), _ptr(up.Aj), dn.slots,
                                     _ptr(dn.Ax), _ptr(dn.Aj), _ptr(up.state), _ptr(dn.state), self.U, self.mask,
                                     _ptr(self.Uvec), _ptr(x), _ptr(y), int(row0), int(nrows))
+
+    def dense(self):
+        """Hubbard::Write (HubMatrix_GPU.cuh:184-230, kernel write_): H[r, c] with r = u*dim_dn + d;
+        per row the nonzero UP slots in slot order (column AjUP*dim_dn + d), then the nonzero DN
+        slots (column u*dim_dn + AjDN), then the interaction on the diagonal (numpy)."""
+        up, dn = self.up, self.dn
+        du, dd = self.dim_up, self.dim_dn
+        D = du * dd
+        H = np.zeros((D, D), dtype=self.dtype())
+        u = np.repeat(np.arange(du), dd)
+        d = np.tile(np.arange(dd), du)
+        r = np.arange(D)
+        for s_ in range(up.slots):
+            ax = up.Ax[s_ * du:(s_ + 1) * du][u]
+            aj = up.Aj[s_ * du:(s_ + 1) * du][u].astype(np.int64)
+            nz = ax != 0
+            np.add.at(H, (r[nz], (aj * dd + d)[nz]), ax[nz])
+        for s_ in range(dn.slots):
+            ax = dn.Ax[s_ * dd:(s_ + 1) * dd][d]
+            aj = dn.Aj[s_ * dd:(s_ + 1) * dd][d].astype(np.int64)
+            nz = ax != 0
+            np.add.at(H, (r[nz], (u * dd + aj)[nz]), ax[nz])
+        if self.U != 0.0 or self.Uvec is not None:
+            dbl = up.state[u].astype(np.int64) & dn.state[d].astype(np.int64) & self.mask
+            if self.Uvec is None:
+                dg = self.U * np.array([bin(int(x)).count("1") for x in dbl], dtype=np.float64)
+            else:
+                dg = np.array([sum(self.Uvec[i] for i in range(up.N) if (int(x) >> i) & 1) for x in dbl])
+            H[r, r] += dg
+        return H
 
     def lanczos_eigen(self, f, maxN=1000, tol=None):
         """Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613).  Returns (status, E0, evec, a, b, nsteps)."""
@@ -346,6 +392,47 @@ def drude_weight(H: Hubbard, J: Hubbard, K: Hubbard, E0, psi, nomega=8000, omega
     lam = continued_fraction(a, b, E0, norm, w)
     kin = float(np.vdot(psi, K.apply(psi)).real)
     return drude_combine(w, lam, kin, omega_step, omega_cutoff, vol), kin, lam
+
+
+def ted_time_evol(W, V, phi0, dt, phi=None):
+    """tED<C>::tED_time_evol (HubMatrix_GPU.cuh:784-797): phi += sum_i e^{-i W_i dt} <V_i|phi0> V_i
+    over the eigenpairs (W ascending, V[:, i] eigenvector i) of a dense eigensolver (numpy/LAPACK
+    here, cuSOLVER syevd in the reference)."""
+    phi0 = np.asarray(phi0, dtype=np.complex128)
+    phi = np.zeros_like(phi0) if phi is None else np.array(phi, dtype=np.complex128)
+    c = (np.cos(W * dt) - 1j * np.sin(W * dt)) * (V.conj().T @ phi0)
+    return phi + V @ c
+
+
+def prev_permutation(seq):
+    """std::prev_permutation on a list in place (the standard algorithm: find the last i with
+    seq[i-1] > seq[i], swap seq[i-1] with the rightmost element smaller than it, reverse the
+    tail).  Returns False (and leaves the last permutation reversed to the largest) at the end."""
+    n = len(seq)
+    i = n - 1
+    while i > 0 and seq[i - 1] <= seq[i]:
+        i -= 1
+    if i == 0:
+        seq.reverse()
+        return False
+    j = n - 1
+    while seq[j] >= seq[i - 1]:
+        j -= 1
+    seq[i - 1], seq[j] = seq[j], seq[i - 1]
+    seq[i:] = seq[i:][::-1]
+    return True
+
+
+def lowmem_basis(N, Q):
+    """LOW_MEM::H_TBq::Labeling (TotalSpin/include/fermi_operator_lowmem.h:50-64): occ[0..Q-1] true,
+    then every prev_permutation of occ; state = sum occ[i] 2^i.  Pure Python (small N)."""
+    occ = [True] * Q + [False] * (N - Q)
+    out = []
+    while True:
+        out.append(sum(1 << i for i in range(N) if occ[i]))
+        if not prev_permutation(occ):
+            break
+    return np.array(out, dtype=np.uint64)
 
 
 def num_threads():

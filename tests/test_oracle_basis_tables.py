@@ -30,7 +30,7 @@ def _ref():
         pytest.skip("reference table builder (oracle/_ref) not built")
     R = C.CDLL(REF_SO)
     R.ref_htbq_tables.restype = C.c_longlong
-    R.ref_htbq_tables.argtypes = [C.c_int] * 3 + [C.c_void_p] * 8
+    R.ref_htbq_tables.argtypes = [C.c_int] * 3 + [C.c_void_p] * 9
     R.ref_count1_bt_sites.restype = C.c_int
     R.ref_count1_bt_sites.argtypes = [C.c_int] * 3
     return R
@@ -42,6 +42,7 @@ def ref_tables(N, Q, ops):
     i = np.array([o[1] for o in ops], np.int32)
     j = np.array([o[2] for o in ops], np.int32)
     v = np.array([o[3] for o in ops], np.float64)
+    dg = np.array([o[5] if len(o) > 5 else 0 for o in ops], np.int32)
     cnt = math.comb(N, Q) if Q >= 0 else (1 << N)
     S = 10 * (Q if Q >= 0 else N) + 1
     st = np.zeros(cnt, np.int32)
@@ -49,7 +50,7 @@ def ref_tables(N, Q, ops):
     Ax = np.zeros(cnt * S)
     Aj = np.zeros(cnt * S, np.int32)
     p = lambda a: a.ctypes.data_as(C.c_void_p)
-    n = R.ref_htbq_tables(N, Q, len(ops), p(k), p(i), p(j), p(v), p(st), p(ad), p(Ax), p(Aj))
+    n = R.ref_htbq_tables(N, Q, len(ops), p(k), p(i), p(j), p(v), p(st), p(ad), p(Ax), p(Aj), p(dg))
     assert n == cnt
     return st, ad, Ax, Aj
 
@@ -114,6 +115,15 @@ def _ops_sets():
                              [(L.OP_DENSITY, i, i, float(v)) for i, v in
                               enumerate(np.random.default_rng(3).uniform(-8, 8, 14))]),
         "full_fock_4": (4, -1, [(L.OP_HOP, 0, 1, -1.0), (L.OP_HOP, 2, 3, -0.5), (L.OP_DENSITY, 1, 1, 2.0)]),
+        # Insert_1Fop / Insert_2Fop, every sign combination and both site orders, mixed with hops
+        "full_fock_6_fermi_ops": (6, -1, [(O.OP_FERMI1, 2, 2, 0.7, 0.0, 0), (O.OP_FERMI1, 4, 4, -1.3, 0.0, 1),
+                                          (O.OP_FERMI2, 1, 4, 0.5, 0.0, 2), (O.OP_FERMI2, 5, 0, 1.1, 0.0, 1),
+                                          (O.OP_FERMI2, 3, 1, 0.9, 0.0, 0), (O.OP_FERMI2, 0, 5, -0.4, 0.0, 3),
+                                          (L.OP_HOP, 0, 1, -1.0), (O.OP_FERMI1, 5, 5, 0.25, 0.0, 0),
+                                          (O.OP_FERMI2, 2, 3, 0.3, 0.0, 1), (L.OP_DENSITY, 3, 3, 0.5)]),
+        # number-changing calls on a sector basis ("use only full Hilbert space"): the target's
+        # address is 0 outside the sector, and the reference stores that column as it is
+        "sector_fermi_misuse": (6, 3, [(O.OP_FERMI1, 1, 1, 1.0, 0.0, 1), (O.OP_FERMI2, 0, 2, 0.5, 0.0, 1)]),
     }
 
 
