@@ -1,7 +1,9 @@
 #!/usr/bin/env python3
 """H.v throughput benchmark: 18-site Kagome Hubbard, N_up = N_dn = 9, FP64.
 
-One step = one on-the-fly H.v (Siro-Harju split) over the full sector,
+One step = one H.v (Siro-Harju split) over the full sector, on the slab layout the Lanczos
+solver keeps its vectors in (bit-identical to the reference layout's H.v, which is timed and
+reported beside it),
 D = 48,620^2 = 2,363,904,400 amplitudes, x = the normalized seed-235 start vector,
 bonds of the 2x3-cell Kagome cluster (KagomeLattice18y, drude_v2_cal_Kagome_randomHop.cu:712-730)
 with matrix element -1 and U = 8.  Inputs are resident in HBM before timing starts.
@@ -26,7 +28,7 @@ sys.path.insert(0, REPO)
 
 METRIC = "H·v HBM GB/s vs peak (18-site Kagome); batched Drude weights/s vs plain @1e-8"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "hv_kagome18y_traffic.txt")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hv_slab_kagome18y_traffic.txt")
 
 
 def log(msg):
@@ -76,11 +78,14 @@ def ground_state_line(H, hq, x, y, N, Q):
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
     lz = Lanczos(hq.count, hq.count)
     lz.random_vector(x)
+    H.slab_layout(True)  # the recurrence runs on the slab layout (start and Ritz vectors converted)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = lz.eigen(H, x, y)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    H.slab_layout(False)
+    torch.cuda.empty_cache()
     log(f"[bench] ground state E0 {e0:.15f} in {lz.nsteps} steps, {t1 - t0:.1f} s")
     s = np.zeros(1)
     call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(y), s.ctypes.data_as(C.POINTER(C.c_double)),
@@ -235,11 +240,49 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     per = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))  # ms
+    rm_mean_ms = sum(per) / len(per)
+    rm_median_ms = per[len(per) // 2]
+    alg_bytes = 16.0 * D
+    log(f"[bench] H.v on the reference layout: mean {rm_mean_ms:.3f} ms, median {rm_median_ms:.3f} ms, "
+        f"wall/step {wall / args.steps * 1e3:.3f} ms")
+
+    # the solver's layout: the same H.v (bit for bit) on the slab layout that ed_lanczos_eigen
+    # keeps its vectors in; the input is converted once before timing, as the solver does
+    t0 = time.perf_counter()
+    H.slab_layout(True)
+    xl = H.to_slab(x)
+    yl = torch.empty_like(xl)
+    torch.cuda.synchronize()
+    slab_setup_s = time.perf_counter() - t0
+    for _ in range(max(1, args.warmup)):  # at least one, for the bitwise check below
+        H.apply_slab(xl, yl)
+    same = bool(torch.equal(H.from_slab(yl), y))
+    log(f"[bench] slab layout ready in {slab_setup_s:.2f} s; H.v bitwise equal to the reference layout: {same}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record()
+        H.apply_slab(xl, yl)
+        ends[k].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))  # ms
     mean_ms = sum(per) / len(per)
     median_ms = per[len(per) // 2]
-    alg_bytes = 16.0 * D
     gbs = alg_bytes / (mean_ms * 1e-3) / 1e9
-    log(f"[bench] H.v mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, wall/step {wall / args.steps * 1e3:.3f} ms")
+    log(f"[bench] H.v on the slab layout: mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, "
+        f"wall/step {wall / args.steps * 1e3:.3f} ms")
+    conv = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    conv[0].record()
+    H.to_slab(x, xl)
+    conv[1].record()
+    torch.cuda.synchronize()
+    conv_ms = conv[0].elapsed_time(conv[1])
+    del xl, yl
+    H.slab_layout(False)
+    torch.cuda.empty_cache()
+    if not same:
+        raise SystemExit("slab-layout H.v differs from the reference-layout H.v")
 
     if args.copy_ref:
         torch.mul(x, 1.0, out=y)
@@ -260,7 +303,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seed-235 minstd_rand start vector, the reference's GenRand)",
         "config": {"workload": f"hubbard_hv_{cl.name}_nup{Q}_ndn{Q}_U{U:g}", "sites": N, "dim": D,
+                   "layout": "slab (the Lanczos solver's vector layout; bit-identical to the reference layout)",
                    "hv_median_ms": round(median_ms, 4), "hv_mean_ms": round(mean_ms, 4),
+                   "hv_reference_layout_mean_ms": round(rm_mean_ms, 4),
+                   "hv_reference_layout_gbs": round(alg_bytes / (rm_mean_ms * 1e-3) / 1e9, 2),
+                   "reference_to_slab_conversion_ms": round(conv_ms, 3),
                    "hv_tiles": {k: v for k, v in tuning.items() if k != "ms_per_apply"}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},

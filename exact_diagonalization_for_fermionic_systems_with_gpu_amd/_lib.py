@@ -59,6 +59,10 @@ SIGNATURES = {
     "ed_hubbard_info": (_i32, [_p, _pi64, _pi64, _pi]),
     "ed_hubbard_apply": (_i32, [_p, _i32, _p, _p, _p]),
     "ed_hubbard_autotune": (_i32, [_p, _i32, _p, _p, _i32, C.POINTER(HvTuning), _p]),
+    "ed_hubbard_slab_layout": (_i32, [_p, _i32, _pi64]),
+    "ed_slab_from_rowmajor": (_i32, [_p, _p, _p, _p]),
+    "ed_slab_to_rowmajor": (_i32, [_p, _p, _p, _p]),
+    "ed_hubbard_apply_slab": (_i32, [_p, _p, _p, _p]),
     "ed_dense_zero": (_i32, [_i64, _i32, _p, _p]),
     "ed_hubbard_write_dense": (_i32, [_p, _p, _p]),
     "ed_tables_write_dense": (_i32, [_i64, _i32, _i32, _p, _p, _p, _p]),

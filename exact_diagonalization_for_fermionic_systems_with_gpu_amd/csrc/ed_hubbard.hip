@@ -667,6 +667,8 @@ void Hubbard::release()
     d_up_cnt = nullptr;
     dfree(otf.up.ops_own);
     dfree(otf.dn.ops_own);
+    slab_free(slab);
+    slab = nullptr;
     d_state_up = d_state_dn = nullptr;
     d_uvec = nullptr;
     d_addr_up = d_addr_dn = nullptr;
@@ -791,7 +793,7 @@ extern "C" int ed_hubbard_apply(ed_hubbard hh, int complex_vectors, const void *
 // two sites is occupied).  Pass 1 of the on-the-fly H.v walks its rows in this order, so the
 // rows read for one row were mostly read for rows visited shortly before and are still in L2
 // (breadth-first level sets instead of colex ranks, whose hops on high sites jump far).
-static int up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> &order)
+int ed::up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> &order)
 {
     int64_t dim = 0;
     ED_TRY(ed_basis_count(N, Q, &dim));

@@ -73,6 +73,9 @@ struct OtfArgs {
     const int32_t *up_cnt;   // entries per row position
 };
 
+struct SlabLayout; // ed_hv_slab.hip
+void slab_free(SlabLayout *s);
+
 struct Hubbard {
     int N = 0;
     bool cplx_tables = false;
@@ -99,6 +102,7 @@ struct Hubbard {
     std::vector<OtfOp> h_ops_up, h_ops_dn; // host copies of otf.up/dn.ops (for ed_batch_create)
     std::vector<double> h_uvec;            // host copy of d_uvec (N entries), empty without one
     bool has_fermi = false; // an Insert_1Fop / Insert_2Fop call: table kernels only
+    SlabLayout *slab = nullptr; // slab-tiled vector layout (ed_hubbard_slab_layout), or nullptr
     void release();
 };
 
@@ -114,6 +118,15 @@ int otf_dispatch(const Hubbard &h, bool cvec, const void *x, void *y, void *vout
                  int step, double *partial, hipStream_t st, int64_t *nblocks);
 int64_t otf_num_blocks(const Hubbard &h);
 int otf_prepare(OtfSpecies &sp, const ed_op *ops, int nops, const int32_t *d_address, hipStream_t st);
+int up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> &order);
+// Lanczos step halves on the slab layout (ed_hv_slab.hip): same contract as hv_lanczos /
+// hv_lanczos_nodot on vectors of slab_len() doubles
+int64_t slab_len(const Hubbard &h);
+int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const double *w, const double *bdev,
+                 int step, bool dot, double *partial, hipStream_t st, int64_t *nblocks);
+int64_t slab_num_partials(const Hubbard &h);
+int slab_from_rowmajor(const Hubbard &h, const double *x, double *xl, hipStream_t st);
+int slab_to_rowmajor(const Hubbard &h, const double *xl, double *x, hipStream_t st);
 
 // deterministic reductions and BLAS-1 (ed_blas.hip)
 int finalize_sum(const double *partial, int64_t n, double *dst, int mode, hipStream_t st);

@@ -200,7 +200,8 @@ static int alloc_work(Work &w, const Hubbard &h, int64_t nreal, int maxN, bool v
         ED_TRY(dmalloc((void **)&w.buf[0], sizeof(double) * (size_t)nreal));
         ED_TRY(dmalloc((void **)&w.buf[1], sizeof(double) * (size_t)nreal));
     }
-    const int64_t np = std::max<int64_t>(hv_num_blocks(h), blas_num_blocks(nreal));
+    const int64_t np = std::max<int64_t>(std::max<int64_t>(hv_num_blocks(h), slab_num_partials(h)),
+                                         blas_num_blocks(nreal));
     ED_TRY(dmalloc((void **)&w.partial, sizeof(double) * (size_t)np));
     ED_TRY(dmalloc((void **)&w.adev, sizeof(double) * (size_t)(maxN + 2)));
     ED_TRY(dmalloc((void **)&w.bdev, sizeof(double) * (size_t)(maxN + 3)));
@@ -210,10 +211,11 @@ static int alloc_work(Work &w, const Hubbard &h, int64_t nreal, int maxN, bool v
 
 // One step N: (A = f, B = v_{N-1}) -> (C = v_N, B = f_{N+1}); a[N], b[N+1] on the device.
 static int lanczos_step(const Hubbard &h, bool cvec, int64_t nreal, double *A, double *B, double *C, Work &w,
-                        int N, bool reorth, hipStream_t st)
+                        int N, bool reorth, bool slab, hipStream_t st)
 {
     int64_t nb;
-    ED_TRY(hv_lanczos(h, cvec, A, B, C, B, w.bdev, N, w.partial, st, &nb));
+    if (slab) ED_TRY(slab_lanczos(h, A, B, C, B, w.bdev, N, true, w.partial, st, &nb));
+    else ED_TRY(hv_lanczos(h, cvec, A, B, C, B, w.bdev, N, w.partial, st, &nb));
     ED_TRY(finalize_sum(w.partial, nb, w.adev + N, 0, st));
     ED_TRY(lanczos_update(nreal, C, B, w.adev, N, w.partial, &nb, st));
     if (reorth) {
@@ -281,26 +283,48 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     const int every = cfg && cfg->check_every > 0 ? cfg->check_every : 10;
     const bool reorth = cfg && cfg->reorth;
     hipStream_t st = stream_of(stream);
-    const int64_t n = h->args.dim_up * h->args.dim_dn, nreal = cvec ? 2 * n : n;
+    const int64_t n = h->args.dim_up * h->args.dim_dn;
+    // real sectors with a slab layout (ed_hubbard_slab_layout) run the whole recurrence on it:
+    // the start vector is converted once and the Ritz vector converted back at the end
+    const bool slab = h->slab && !cvec;
+    const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
 
     Work w;
     ED_TRY(alloc_work(w, *h, nreal, maxN, true));
-    double *ev = (double *)d_evec;
+    struct Own {
+        double *p[2] = {nullptr, nullptr};
+        ~Own()
+        {
+            dfree(p[0]);
+            dfree(p[1]);
+        }
+    } own;
+    double *F = (double *)d_f, *ev = (double *)d_evec;
+    if (slab) {
+        // layout-L copies of f and evec; padding entries stay zero in every vector
+        ED_TRY(dmalloc((void **)&own.p[0], sizeof(double) * (size_t)nreal));
+        if (want_vec) ED_TRY(dmalloc((void **)&own.p[1], sizeof(double) * (size_t)nreal));
+        for (double *b : {w.buf[0], w.buf[1], own.p[0], own.p[1]})
+            if (b) ED_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)nreal, st));
+        ED_TRY(slab_from_rowmajor(*h, (const double *)d_f, own.p[0], st));
+        F = own.p[0];
+        if (want_vec) ev = own.p[1];
+    }
     // evec = f keeps the start vector for the rebuild (HubMatrix_GPU.cuh:566)
-    if (want_vec) ED_HIP(hipMemcpyAsync(ev, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    if (want_vec) ED_HIP(hipMemcpyAsync(ev, F, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     int64_t nb;
-    ED_TRY(dot_partials(nreal, (const double *)d_f, (const double *)d_f, w.partial, &nb, st));
+    ED_TRY(dot_partials(nreal, F, F, w.partial, &nb, st));
     ED_TRY(finalize_sum(w.partial, nb, w.bdev, 1, st));
 
     std::vector<double> a(maxN + 2, 0.0), araw(maxN + 2, 0.0), dl(maxN + 2, 0.0), b(maxN + 3, 0.0), z;
     double anorm = 0.0;
     int lastN = -1, status = ED_ENOCONV;
     int processed = 0; // a[0..processed-1] folded into anorm
-    double *A = (double *)d_f, *B = w.buf[0], *C = w.buf[1];
+    double *A = F, *B = w.buf[0], *C = w.buf[1];
     int N;
     bool stop = false;
     for (N = 0; N < maxN && !stop; ++N) {
-        ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, st));
+        ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, slab, st));
         double *t = A;
         A = B;
         B = C;
@@ -353,13 +377,14 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         return ED_OK;
     }
     // Ritz vector: rerun the recurrence from the saved start (HubMatrix_GPU.cuh:600-612)
-    A = (double *)d_f;
+    A = F;
     B = w.buf[0];
     C = w.buf[1];
     ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
     for (int i = 0; i < Nk + 1; ++i) {
-        ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
+        if (slab) ED_TRY(slab_lanczos(*h, A, B, C, B, w.bdev, i, false, nullptr, st, nullptr));
+        else ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
         ED_TRY(ritz_update(nreal, C, B, ev, w.adev, z[i], i, st));
         if (reorth) {
             int64_t nbk;
@@ -369,6 +394,10 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         A = B;
         B = C;
         C = t;
+    }
+    if (slab) {
+        ED_TRY(slab_to_rowmajor(*h, ev, (double *)d_evec, st));
+        ev = (double *)d_evec;
     }
     ED_TRY(ed_normalize(n, complex_vectors, ev, nullptr, stream));
     ED_HIP(hipStreamSynchronize(st));
@@ -403,7 +432,7 @@ extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int ns
         if ((s = finalize_sum(w.partial, nb, w.bdev, 1, st))) break;
         double *A = f, *B = w.buf[0], *C = w.buf[1];
         for (int N = 0; N < nsteps && s == ED_OK; ++N) {
-            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, st);
+            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, false, st);
             double *t = A;
             A = B;
             B = C;
@@ -502,7 +531,7 @@ extern "C" int ed_krylov_tridiagonal(ed_hubbard hh, const void *d_phi0, int max_
         while (N < max_steps && nt < 0 && s == ED_OK) {
             const int chunk = std::min(10, max_steps - N);
             for (int k = 0; k < chunk && s == ED_OK; ++k, ++N) {
-                s = lanczos_step(*h, true, nreal, A, B, C, w, N, false, st);
+                s = lanczos_step(*h, true, nreal, A, B, C, w, N, false, false, st);
                 double *t = A;
                 A = B;
                 B = C;

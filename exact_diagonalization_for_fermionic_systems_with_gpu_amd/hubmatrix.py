@@ -104,6 +104,37 @@ class Hubbard:
         self.Hubbard_mv(x, y)
         return y
 
+    # --- slab layout (the solver's internal vector layout; no reference counterpart) ----------
+    def slab_layout(self, enable: bool = True) -> int:
+        """Build (or free) the slab layout of this real operator; returns the layout's vector length
+        (0 when freed).  Lanczos.eigen on real vectors then runs its recurrence on it."""
+        n = C.c_int64()
+        call("ed_hubbard_slab_layout", self._h, int(bool(enable)), C.byref(n))
+        self.slab_len = n.value
+        return self.slab_len
+
+    def to_slab(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        self._check(x)
+        if out is None:
+            out = torch.empty(self.slab_len, dtype=torch.float64, device=x.device)
+        call("ed_slab_from_rowmajor", self._h, _raw(x), _raw(out), stream_ptr())
+        return out
+
+    def from_slab(self, xl: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(self.dim, dtype=torch.float64, device=xl.device)
+        self._check(out)
+        call("ed_slab_to_rowmajor", self._h, _raw(xl), _raw(out), stream_ptr())
+        return out
+
+    def apply_slab(self, xl: torch.Tensor, yl: torch.Tensor):
+        """yl = H xl on the slab layout (asynchronous)"""
+        n = getattr(self, "slab_len", 0)
+        for t in (xl, yl):
+            if n == 0 or t.numel() != n or t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("slab vectors are contiguous float64 CUDA tensors of slab_len elements")
+        call("ed_hubbard_apply_slab", self._h, _raw(xl), _raw(yl), stream_ptr())
+
     def Write(self, Hmat: torch.Tensor):
         """Hubbard::Write (HubMatrix_GPU.cuh:219-230): ADDS H into the dense D x D matrix Hmat
         (row-major, Hmat[i_up*dimDN + i_dn, col]; float64, complex128 for complex tables)."""
@@ -196,11 +227,14 @@ class Lanczos(Thrust_Lin):
         """GenRand (seed 235) + normalize (HubMatrix_GPU.cuh:504-539)"""
         call("ed_random_vector", f.numel(), seed, _is_complex(f), _raw(f), 1, stream_ptr())
 
-    def eigen(self, Hub: Hubbard, f: torch.Tensor, evec: torch.Tensor) -> float:
-        """Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613): returns eval, fills evec.  f is clobbered.
-        Raises EdError(ED_ENOCONV) if the convergence test is not met (evec/eval then hold the
-        last Ritz pair)."""
-        Hub._check(f, evec)
+    def eigen(self, Hub: Hubbard, f: torch.Tensor, evec: torch.Tensor | None) -> float:
+        """Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613): returns eval, fills evec (None: the
+        eigenvalue only, no Ritz rebuild).  f is clobbered.  Raises EdError(ED_ENOCONV) if the
+        convergence test is not met (evec/eval then hold the last Ritz pair)."""
+        if evec is None:
+            Hub._check(f)
+        else:
+            Hub._check(f, evec)
         cfg = LanczosCfg(self.max_steps, self.tol, self.check_every, int(self.reorth))
         ev = C.c_double()
         a = (C.c_double * (self.max_steps + 2))()

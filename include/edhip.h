@@ -169,6 +169,25 @@ int ed_dense_eigh(int64_t n, int complex_values, void *d_A, double *d_W, void *s
 int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const double *d_W, const void *d_phi0,
                      double dt, void *d_phi, void *stream);
 
+/* ---------------- slab layout (no counterpart in the reference) ------------------- */
+
+/* The vector layout the Lanczos solver keeps internally for large real sectors: both species'
+ * basis states renumbered in reverse Cuthill-McKee order of their hopping graphs (positions
+ * pu, pd) and the state cut into slabs of 16 dn positions, amplitude (i_up, i_dn) at
+ * ((pd >> 4)*dim_up + pu)*16 + (pd & 15); *len (>= dim_up*dim_dn) doubles per vector, padding
+ * entries zero.  H.v on it (ed_hubbard_apply_slab) gives the same values bit for bit as
+ * ed_hubbard_apply (HubMatrix_GPU.cuh:114-182) on the reference layout, with 16 B gathers that
+ * stay in L2.  enable = 1 builds the layout tables and a scratch vector (len doubles) in the
+ * handle, after which ed_lanczos_eigen on real vectors runs its recurrence on this layout
+ * (converting the start vector in and the Ritz vector out); enable = 0 frees them.  Needs a
+ * real operator from ed_hubbard_create_ops with N <= 26. */
+int ed_hubbard_slab_layout(ed_hubbard h, int enable, int64_t *len);
+/* reference layout psi[i_up*dim_dn + i_dn] -> slab layout (padding set to zero), and back */
+int ed_slab_from_rowmajor(ed_hubbard h, const double *d_x, double *d_xl, void *stream);
+int ed_slab_to_rowmajor(ed_hubbard h, const double *d_xl, double *d_x, void *stream);
+/* y = H x on the slab layout (x != y, x padding zero; y padding is written zero).  Asynchronous. */
+int ed_hubbard_apply_slab(ed_hubbard h, const double *d_xl, double *d_yl, void *stream);
+
 /* ---------------- vector BLAS-1 (Thrust_Lin, HubMatrix_GPU.cuh:291-471) -------- */
 
 /* GenRand + random_vector (HubMatrix_GPU.cuh:504-539): minstd_rand seeded with
