@@ -28,6 +28,7 @@
 // Gathers are 16 B per lane and hit L2 instead of MALL/HBM with TLB misses, at the price of one
 // extra write and read of a vector per H.v (the transposed copy).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -119,19 +120,20 @@ __device__ __forceinline__ int slab_stage(const SlabArgs &a, int64_t p0, int64_t
     return nrows;
 }
 
-// Sum over the row's entries (call order) of value * row[neighbour][c..c+1]; x scaled by sc
-template <bool SCALE>
+// Sum over the row's entries (call order) of value * row[neighbour][c..c+1]; x scaled by sc.
+// The gathers of NB entries are issued before their FMAs (NB loads of 16 B in flight per lane).
+template <bool SCALE, int NB>
 __device__ __forceinline__ void slab_gather(const double *__restrict__ base, const uint32_t *s_ent,
                                             const double *s_val, int o0, int o1, int c, double sc, double &acc0,
                                             double &acc1)
 {
-    for (int j0 = o0; j0 < o1; j0 += 8) {
-        double2 xv[8];
+    for (int j0 = o0; j0 < o1; j0 += NB) {
+        double2 xv[NB];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < NB; ++k)
             if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(base + (int64_t)(s_ent[j0 + k] >> 8) * kSW + c);
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < NB; ++k)
             if (j0 + k < o1) {
                 const double av = s_val[s_ent[j0 + k] & 0xffu];
                 double x0 = xv[k].x, x1 = xv[k].y;
@@ -146,7 +148,7 @@ __device__ __forceinline__ void slab_gather(const double *__restrict__ base, con
 }
 
 // Pass A: z = UP term of x (layout L); xt = sc*x (layout T)
-template <bool SCALE>
+template <bool SCALE, int NB>
 __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
                                                  const double *__restrict__ x, double *__restrict__ z,
                                                  double *__restrict__ xt, const double *__restrict__ bdev, int step)
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int
     if (r < nrows) {
         const int64_t p = p0 + r;
         double acc0 = 0.0, acc1 = 0.0;
-        slab_gather<SCALE>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, sc, acc0, acc1);
+        slab_gather<SCALE, NB>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, sc, acc0, acc1);
         st2(z + (slab * a.du + p) * kSW + c, acc0, acc1);
         double2 xo = *reinterpret_cast<const double2 *>(xs + p * kSW + c);
         if (SCALE) {
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int
 }
 
 // Pass B: y = zin + DN term of xt + diag * xt (+ c w); v = xt (layout L); block partial <v, y>
-template <int FLAGS>
+template <int FLAGS, int NB>
 __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
                                                  const double *__restrict__ xt, const double *zin, const double *win,
                                                  double *yout, double *vout, const double *__restrict__ bdev, int step,
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
     if (r < nrows) {
         double acc0 = s_a[r][c], acc1 = s_a[r][c + 1];
         const double *xs = xt + su * a.dd * kSW;
-        slab_gather<false>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, 1.0, acc0, acc1);
+        slab_gather<false, NB>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, 1.0, acc0, acc1);
         const double2 xo = *reinterpret_cast<const double2 *>(xs + pd * kSW + c);
         xd0 = xo.x;
         xd1 = xo.y;
@@ -336,22 +338,52 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
                       const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks)
 {
     const SlabLayout &L = *h.slab;
+    // unused dynamic LDS per workgroup (bytes): fewer resident workgroups per CU, hence fewer
+    // rows in flight per XCD and a narrower L2 gather window (ED_SLAB_LDS_PAD, default none).
+    // Measured at 18 sites: every pad slows the passes (8, 16, 24, 36 KB: +10, +26, +47, +90 %),
+    // the loss of resident waves outweighs any L2 gain.
+    static const size_t pad = [] {
+        const char *e = getenv("ED_SLAB_LDS_PAD");
+        const long v = e ? atol(e) : 0;
+        return (size_t)(v > 0 && v <= 120000 ? v : 0);
+    }();
     const SlabArgs au = slab_args(h, true), ad = slab_args(h, false);
     const int64_t ntrA = (L.du + kTR - 1) / kTR, tilesA = L.nsd * ntrA, perA = (tilesA + 7) / 8;
     const int64_t ntrB = (L.dd + kTR - 1) / kTR, tilesB = L.nsu * ntrB, perB = (tilesB + 7) / 8;
     if (perA * 8 > 0x7fffffffLL || perB * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab H.v: grid too large");
-    if (flags & HV_SCALE)
-        hipLaunchKernelGGL(k_slab_up<true>, dim3((unsigned)(perA * 8)), dim3(256), 0, st, au, tilesA, perA, ntrA, x, z,
-                           L.xt, bdev, step);
-    else
-        hipLaunchKernelGGL(k_slab_up<false>, dim3((unsigned)(perA * 8)), dim3(256), 0, st, au, tilesA, perA, ntrA, x,
-                           z, L.xt, bdev, step);
+    // gathers in flight per lane (ED_SLAB_NB = 8, 16 or 32).  8 keeps the kernels at 52 VGPRs and
+    // 8 waves per SIMD; 16 and 32 cost occupancy (5 and 3 waves) and ran slower at 18 sites
+    // (56.2, 65.0 and 82.7 ms per H.v): the passes are bound by loads in flight per CU, which
+    // more resident waves raise more than deeper per-wave batches.
+    static const int nb = [] {
+        const char *e = getenv("ED_SLAB_NB");
+        const int v = e ? atoi(e) : 8;
+        return (v == 16 || v == 32) ? v : 8;
+    }();
+    const dim3 gA((unsigned)(perA * 8)), gB((unsigned)(perB * 8));
+#define ED_SLAB_A(S, N) \
+    hipLaunchKernelGGL((k_slab_up<S, N>), gA, dim3(256), pad, st, au, tilesA, perA, ntrA, x, z, L.xt, bdev, step)
+    if (flags & HV_SCALE) {
+        if (nb == 16) ED_SLAB_A(true, 16);
+        else if (nb == 32) ED_SLAB_A(true, 32);
+        else ED_SLAB_A(true, 8);
+    } else {
+        if (nb == 16) ED_SLAB_A(false, 16);
+        else if (nb == 32) ED_SLAB_A(false, 32);
+        else ED_SLAB_A(false, 8);
+    }
+#undef ED_SLAB_A
     ED_CHECK_LAUNCH();
-    const dim3 gB((unsigned)(perB * 8));
     const int fb = flags & ~HV_SCALE;
+#define ED_SLAB_B1(F, N)                                                                                            \
+    hipLaunchKernelGGL((k_slab_dn<F, N>), gB, dim3(256), pad, st, ad, tilesB, perB, ntrB, L.xt, z, w, y, v, bdev,   \
+                       step, L.part)
 #define ED_SLAB_B(F)                                                                                               \
-    hipLaunchKernelGGL(k_slab_dn<F>, gB, dim3(256), 0, st, ad, tilesB, perB, ntrB, L.xt, z, w, y, v, bdev, step, \
-                       L.part)
+    do {                                                                                                           \
+        if (nb == 16) ED_SLAB_B1(F, 16);                                                                           \
+        else if (nb == 32) ED_SLAB_B1(F, 32);                                                                      \
+        else ED_SLAB_B1(F, 8);                                                                                     \
+    } while (0)
     switch (fb) {
     case 0: ED_SLAB_B(0); break;
     case HV_WRITEV: ED_SLAB_B(HV_WRITEV); break;
@@ -361,6 +393,7 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
     default: return fail(ED_EINVAL, "slab H.v: unsupported flags %d", flags);
     }
 #undef ED_SLAB_B
+#undef ED_SLAB_B1
     ED_CHECK_LAUNCH();
     if (fb & HV_DOT) {
         const int64_t nc = (tilesB + 2047) / 2048;

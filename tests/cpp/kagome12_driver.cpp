@@ -3,7 +3,10 @@
 // the DrudeWeight and TotalSpin drivers (drude_v2_cal_Kagome_randomHop.cu:129-220 and
 // ConservedTotalSpin_Kagome_randomHop.cu:110-125): ground state by Lanczos, S(S+1), the Kubo
 // Drude weight D_x, the 6|6 entanglement entropy, the three lowest eigenvalues from the
-// cuSpectra solver and one host-array perform_op.  Prints one JSON object.
+// cuSpectra solver and one host-array perform_op; then the same ground state on the slab layout,
+// and the dense path of the MBL drivers (Hubbard::Write, cusolverDsyevd, tED, H_TBq::Write with
+// Insert_1Fop / Insert_2Fop) and the LOW_MEM basis on small systems.  Prints one JSON object.
+#include <complex>
 #include <cstdio>
 #include <vector>
 
@@ -103,13 +106,69 @@ int main()
         const auto entq = EEq.entropy(psi);
         const double qnorm = kr.norm(psi);
 
+        // the same ground state with the recurrence on the slab layout
+        Hub.slab_layout();
+        Lanczos<double> lanc_s(Hub.dimUP, Hub.dimDN);
+        device_vector<double> fs(D), evs(D);
+        double eval_s = 0.0;
+        lanc_s.random_vector(fs);
+        lanc_s.eigen(Hub, fs, eval_s, evs);
+        const double ovl = lanc.thrust_Dotprod(evec, evs);
+        Hub.slab_layout(false);
+
+        // dense path (Time_evol_test.cu:73-94): 8-site ring, 4/4, U = 4
+        H_TBq RU(8, 4), RD(8, 4);
+        for (int i = 0; i < 8; ++i) {
+            RU.Make_H(i, (i + 1) % 8, -1.0);
+            RD.Make_H(i, (i + 1) % 8, -1.0);
+        }
+        Hubbard Ring(RU, RD, 4.0);
+        const int64_t DR = Ring.dimUP * Ring.dimDN;
+        device_vector<double> Hmat((size_t)(DR * DR), 0.0);
+        Ring.Write(Hmat);
+        tED<double> ted(Ring.dimUP, Ring.dimDN);
+        ted.Make_eigen(Hmat);
+        const std::vector<double> W = ted.W.to_host();
+        std::vector<std::complex<double>> p0((size_t)DR, 0.0);
+        const int64_t k0 = (int64_t)RU.address[0x0f] * Ring.dimDN + RD.address[0xf0];
+        p0[(size_t)k0] = 1.0;
+        device_vector<std::complex<double>> phi0(p0), phi((size_t)DR);
+        ted.tED_time_evol(phi0, phi, 1.5);
+        const std::vector<std::complex<double>> ph1 = phi.to_host();
+        double tnorm = 0.0;
+        for (const auto &z : ph1) tnorm += std::norm(z);
+        Lanczos<double> lr(Ring.dimUP, Ring.dimDN);
+        device_vector<double> fr((size_t)DR), er((size_t)DR);
+        double e_ring = 0.0;
+        lr.random_vector(fr);
+        lr.eigen(Ring, fr, e_ring, er);
+
+        // H_TBq::Write on the full Fock space of 5 sites with Insert_1Fop / Insert_2Fop
+        H_TBq F5(5);
+        F5.Insert_1Fop('+', 2, 0.5);
+        F5.Insert_2Fop('-', 4, '+', 1, -0.75);
+        F5.Make_N(3, 0.25);
+        device_vector<double> M5(32 * 32, 0.0);
+        F5.Write(M5);
+        const std::vector<double> m5 = M5.to_host();
+        double m5w = 0.0; // weighted checksum sum_jk (j*32 + k + 1) M[j, k]
+        for (int j = 0; j < 32 * 32; ++j) m5w += (j + 1) * m5[j];
+
+        const LOW_MEM::H_TBq_basis lm(4, 2);
+
         const auto &ev = eigs.eigenvalues();
         std::printf("{\"E0\": %.17g, \"nsteps\": %d, \"SS\": %.17g, \"D\": %.17g, \"kinetic\": %.17g, "
                     "\"S_vN\": %.17g, \"S_ref\": %.17g, \"n_schmidt\": %lld, \"nconv\": %d, "
                     "\"evals\": [%.17g, %.17g, %.17g], \"rayleigh_host\": %.17g, \"krylov_steps\": %d, "
-                    "\"quench_norm2\": %.17g, \"quench_S_vN\": %.17g}\n",
+                    "\"quench_norm2\": %.17g, \"quench_S_vN\": %.17g, \"E0_slab\": %.17g, \"nsteps_slab\": %d, "
+                    "\"overlap_slab\": %.17g, \"ring_W\": [%.17g, %.17g, %.17g], \"ring_lanczos_E0\": %.17g, "
+                    "\"ted_norm2\": %.17g, \"ted_amp0\": [%.17g, %.17g], \"fock5_checksum\": %.17g, "
+                    "\"lowmem_4_2\": [%llu, %llu, %llu, %llu, %llu, %llu]}\n",
                     eval, lanc.nsteps, SS, Dw, kinetic, ent.S_vN, ent.S_ref, (long long)ent.n_schmidt, nconv, ev[0],
-                    ev[1], ev[2], rq, (int)kr.a.size(), qnorm, entq.S_vN);
+                    ev[1], ev[2], rq, (int)kr.a.size(), qnorm, entq.S_vN, eval_s, lanc_s.nsteps, ovl, W[0], W[1], W[2],
+                    e_ring, tnorm, ph1[(size_t)k0].real(), ph1[(size_t)k0].imag(), m5w,
+                    (unsigned long long)lm.state[0], (unsigned long long)lm.state[1], (unsigned long long)lm.state[2],
+                    (unsigned long long)lm.state[3], (unsigned long long)lm.state[4], (unsigned long long)lm.state[5]);
     } catch (const Error &e) {
         std::fprintf(stderr, "kagome12_driver: %s\n", e.what());
         return 1;

@@ -1,7 +1,9 @@
 """The C++ interface (include/ed/ed.hpp): tests/cpp/kagome12_driver.cpp is written against the
 reference's class names (H_TBq, Hubbard, Lanczos<T>, TotalSpin::Hubbard, compute_sigma_omega,
-EEmatHubAB_GPU, cuSpectra::Hubbard / SymEigsSolver) and built by build().  Its output for the
-12-site Kagome model (6/6, U = 4, hopping -1) is checked against the CPU oracle.
+EEmatHubAB_GPU, cuSpectra::Hubbard / SymEigsSolver, tED, cusolverDsyevd, H_TBq::Insert_1Fop /
+Insert_2Fop / Write, LOW_MEM::H_TBq_basis) and built by build().  Its output for the 12-site Kagome
+model (6/6, U = 4, hopping -1), an 8-site ring (dense path) and a 5-site Fock space is checked
+against the CPU oracle.
 
 Bars: E0, the cuSpectra eigenvalues and the host-array Rayleigh quotient 1e-10 relative; S(S+1)
 1e-8; entropies 1e-8; the Kubo D_x 1e-7 relative (oracle run from its own ground state: the
@@ -69,3 +71,26 @@ def test_kagome12_cpp_driver_against_oracle():
     phi = O.krylov_time_evol(Hq, phi0, ak, bk, mk, zk, 0.2)
     lamq, _ = EE.block_spectra(12, 6, 6, 0x3f, phi)
     assert abs(out["quench_S_vN"] - EE.entropy_from_lambda(lamq)) < 1e-8
+    # the ground state with the recurrence on the slab layout
+    assert abs(out["E0_slab"] - e0) < 1e-10 * abs(e0)
+    assert out["nsteps_slab"] == n
+    assert abs(abs(out["overlap_slab"]) - 1.0) < 1e-8
+    # dense path on the 8-site ring, 4/4, U = 4: eigenvalues, tED amplitude and norm
+    ring = [(L.OP_HOP, i, (i + 1) % 8, -1.0) for i in range(8)]
+    Hr = O.Hubbard(O.Species(8, 4, ring), O.Species(8, 4, ring), 4.0)
+    M = Hr.dense()
+    w, V = np.linalg.eigh(M)
+    assert np.allclose(out["ring_W"], w[:3], rtol=0, atol=1e-12 * np.abs(w).max())
+    assert abs(out["ring_lanczos_E0"] - w[0]) < 1e-10 * abs(w[0])
+    k0 = Hr.up.address[0x0f] * Hr.dim_dn + Hr.dn.address[0xf0]
+    p0 = np.zeros(Hr.dim, dtype=np.complex128)
+    p0[k0] = 1.0
+    import scipy.linalg
+    amp = (scipy.linalg.expm(-1j * M * 1.5) @ p0)[k0]
+    assert abs(complex(*out["ted_amp0"]) - amp) < 1e-11
+    assert abs(out["ted_norm2"] - 1.0) < 1e-12
+    # H_TBq::Write with Insert_1Fop / Insert_2Fop on the 5-site Fock space
+    F = O.Species(5, -1, [(O.OP_FERMI1, 2, 2, 0.5, 0.0, 0), (O.OP_FERMI2, 4, 1, -0.75, 0.0, 1),
+                          (L.OP_DENSITY, 3, 3, 0.25)]).dense()
+    assert out["fock5_checksum"] == float(np.sum((np.arange(32 * 32) + 1) * F.ravel()))
+    assert out["lowmem_4_2"] == [3, 5, 9, 6, 10, 12]
