@@ -534,6 +534,23 @@ static int slab_build(Hubbard &h)
     return ED_OK;
 }
 
+// ed_lanczos_eigen on real vectors of 2^27 amplitudes (1 GiB) and more builds the slab layout on
+// first use when it is supported and three more vectors (the scratch copy and the solver's two
+// layout copies of f and evec) fit in free device memory; ED_SLAB_AUTO=0 turns this off.  The
+// layout stays in the handle (ed_hubbard_slab_layout(h, 0, ...) frees it).
+void slab_auto(Hubbard &h)
+{
+    if (h.slab || h.algo != 3 || h.cplx_tables || h.has_fermi || h.N > 26) return;
+    const char *e = getenv("ED_SLAB_AUTO");
+    if (e && atoi(e) == 0) return;
+    const double D = (double)h.otf.dim_up * (double)h.otf.dim_dn;
+    if (D < (double)(1 << 27)) return;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return;
+    if ((double)free_b < 3.2 * 8.0 * D) return;
+    if (slab_build(h) != ED_OK) (void)hipGetLastError(); // stays on the reference layout
+}
+
 } // namespace ed
 
 using namespace ed;
@@ -542,6 +559,10 @@ extern "C" int ed_hubbard_slab_layout(ed_hubbard hh, int enable, int64_t *len)
 {
     Hubbard *h = reinterpret_cast<Hubbard *>(hh);
     if (!h) return fail(ED_EINVAL, "ed_hubbard_slab_layout: null operator");
+    if (enable < 0) { // query
+        if (len) *len = h->slab ? h->slab->len : 0;
+        return ED_OK;
+    }
     if (!enable) {
         ED_HIP(hipDeviceSynchronize());
         slab_free(h->slab);

@@ -284,8 +284,10 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     const bool reorth = cfg && cfg->reorth;
     hipStream_t st = stream_of(stream);
     const int64_t n = h->args.dim_up * h->args.dim_dn;
-    // real sectors with a slab layout (ed_hubbard_slab_layout) run the whole recurrence on it:
-    // the start vector is converted once and the Ritz vector converted back at the end
+    // real sectors with a slab layout (ed_hubbard_slab_layout, or built here for sectors of 2^27
+    // amplitudes and more) run the whole recurrence on it: the start vector is converted once and
+    // the Ritz vector converted back at the end
+    if (!cvec) slab_auto(*const_cast<Hubbard *>(h));
     const bool slab = h->slab && !cvec;
     const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
 

@@ -113,6 +113,13 @@ class Hubbard:
         self.slab_len = n.value
         return self.slab_len
 
+    def slab_len_query(self) -> int:
+        """length of the slab layout's vectors, 0 when the operator has none (no side effect)"""
+        n = C.c_int64()
+        call("ed_hubbard_slab_layout", self._h, -1, C.byref(n))
+        self.slab_len = n.value
+        return self.slab_len
+
     def to_slab(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         self._check(x)
         if out is None:

@@ -180,7 +180,9 @@ int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const doubl
  * stay in L2.  enable = 1 builds the layout tables and a scratch vector (len doubles) in the
  * handle, after which ed_lanczos_eigen on real vectors runs its recurrence on this layout
  * (converting the start vector in and the Ritz vector out); enable = 0 frees them.  Needs a
- * real operator from ed_hubbard_create_ops with N <= 26. */
+ * real operator from ed_hubbard_create_ops with N <= 26.  ed_lanczos_eigen builds it by itself
+ * for real sectors of 2^27 amplitudes and more when three more vectors fit in free device memory
+ * (ED_SLAB_AUTO=0 turns that off).  enable = -1 only reports *len (0 when there is no layout). */
 int ed_hubbard_slab_layout(ed_hubbard h, int enable, int64_t *len);
 /* reference layout psi[i_up*dim_dn + i_dn] -> slab layout (padding set to zero), and back */
 int ed_slab_from_rowmajor(ed_hubbard h, const double *d_x, double *d_xl, void *stream);

@@ -32,6 +32,12 @@ def _model(name):
         r = rng.uniform(0.8, 1.2, 12)
         ops = L.hubbard_ops(L.KAGOME12, r=r)
         return 12, 5, 7, ops, ops, 0.0, rng.uniform(2, 6, 12)
+    if name in ("chain8_0_3", "chain8_4_8", "chain8_8_0", "chain8_3_3_dn_nocalls"):
+        ops = L.hubbard_ops(L.chain(8, periodic=True), t=-1.0)
+        eu = [(L.OP_DENSITY, i, i, float(v)) for i, v in enumerate(rng.uniform(-2, 2, 8))]
+        qu, qd = {"chain8_0_3": (0, 3), "chain8_4_8": (4, 8), "chain8_8_0": (8, 0),
+                  "chain8_3_3_dn_nocalls": (3, 3)}[name]
+        return 8, qu, qd, ops + eu, ([] if name.endswith("nocalls") else ops), 2.5, None
     if name == "chain14_7_7":
         ops = L.hubbard_ops(L.chain(14), t=-1.0)
         ed = [(L.OP_DENSITY, i, i, float(v)) for i, v in enumerate(rng.uniform(-8, 8, 14))]
@@ -47,7 +53,8 @@ def _build(name):
     return Hubbard(up, dn, U, Uvec=uv), (N, qu, qd, ou, od, U, uv)
 
 
-@pytest.mark.parametrize("name", ["chain8_4_3", "kagome12_6_6", "kagome12_randomhop_uvec", "chain14_7_7"])
+@pytest.mark.parametrize("name", ["chain8_4_3", "kagome12_6_6", "kagome12_randomhop_uvec", "chain14_7_7",
+                                  "chain8_0_3", "chain8_4_8", "chain8_8_0", "chain8_3_3_dn_nocalls"])
 def test_slab_hv_bitwise(name):
     print(f"slab H.v {name}", flush=True)
     H, _ = _build(name)
@@ -102,13 +109,14 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden", "kagome18y_lanczos100_o
 
 @pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
 def test_slab_kagome18y_first_100_steps():
+    """the 18-site fixture, with the slab layout built automatically by the solver"""
     gold = json.load(open(GOLD))
     n = gold["nsteps"]
     print("slab kagome18y 9/9 U=8: 100 Lanczos steps at D = 2.36e9", flush=True)
     hq = H_TBq(18, 9)
     hq.add_ops(L.hubbard_ops(L.KAGOME18Y, t=-1.0))
     H = Hubbard(hq, hq, 8.0)
-    H.slab_layout(True)
+    assert H.slab_len_query() == 0
     f = torch.empty(H.dim, dtype=torch.float64, device=DEV)
     lz = Lanczos(H.dimUP, H.dimDN, max_steps=n, check_every=1000)
     lz.random_vector(f)
@@ -116,7 +124,10 @@ def test_slab_kagome18y_first_100_steps():
         lz.eigen(H, f, None)
     except EdError:
         pass  # 100 steps do not converge: a and b are still returned
+    # a real sector of 2^27 amplitudes and more gets the slab layout by itself
+    assert H.slab_len_query() >= H.dim
     H.slab_layout(False)
+    assert H.slab_len_query() == 0
     del f
     torch.cuda.empty_cache()
     a, b = np.array(lz.a), np.array(lz.b)
