@@ -51,3 +51,47 @@ def test_conserved_total_spin_driver_end_to_end(tmp_path):
         print(f"sample {k}: E0 {e_gpu} (oracle {e0:.12f}), S(S+1) {ss_gpu} (oracle {sxy:.3e})", flush=True)
         assert abs(e_gpu - e0) <= 1e-9 * abs(e0)
         assert abs(sxy_gpu - sxy) <= 1e-8 and abs(ss_gpu - sxy) <= 1e-8
+
+
+def test_hubbard_ee_quench_driver_end_to_end(tmp_path):
+    """MBL_test/src/Hubbard_EE_test.cu ported (tests/cpp/hubbard_ee_quench.cpp): 10-site chain,
+    2 + 2 electrons, U = 4, x = 2, two disorder configurations of six time steps each (seeded);
+    every row's entropy against the oracle (dense eigh of the oracle H, the propagation pinned
+    to scipy's expm, S_ref = -sum s ln s of the A = {0..7} Schmidt values).  Rows carry 10
+    significant digits."""
+    import scipy.linalg
+    from oracle import ee_oracle as EE
+    exe = os.path.join(BIN, "hubbard_ee_quench")
+    if not os.path.exists(exe):
+        pytest.fail("driver port missing: run __graft_entry__.build()")
+    print("Hubbard_EE_test port: L=10, Q=2+2, U=4, x=2, 2 configurations x 6 steps", flush=True)
+    r = subprocess.run([exe, "10", "2", "2", "4", "2", "7", "6", "2", "0", str(tmp_path)], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    text = open(tmp_path / "Hub_EE_L10_Q4_U4_x2.out").read()
+    blocks = [b for b in text.split("\n\n\n") if b.strip()]
+    assert len(blocks) == 2  # "\n" << endl after a configuration, then the next row
+    rows = [[[float(t) for t in ln.split("\t")] for ln in b.strip().split("\n")] for b in blocks]
+    draws = DIO.mt19937_uniform(7, 40)
+    chain = [(L.OP_HOP, i, i + 1, -1.0) for i in range(9)]
+    for xid in range(2):
+        du = draws[20 * xid:20 * xid + 10]
+        dd = draws[20 * xid + 10:20 * xid + 20]
+        up = O.Species(10, 2, chain + [(L.OP_DENSITY, i, i, 2.0 * float(v)) for i, v in enumerate(du)])
+        dn = O.Species(10, 2, chain + [(L.OP_DENSITY, i, i, 2.0 * float(v)) for i, v in enumerate(dd)])
+        H = O.Hubbard(up, dn, 4.0)
+        M = H.dense()
+        W, V = np.linalg.eigh(M)
+        p0 = np.zeros(H.dim, dtype=np.complex128)
+        p0[up.address[192] * H.dim_dn + dn.address[768]] = 1.0
+        assert len(rows[xid]) == 6
+        for tid, (x_out, t_out, se_out) in enumerate(rows[xid]):
+            t = np.exp((tid + 1) * 0.02) - 1.0
+            assert x_out == xid and abs(t_out - t) <= 1e-9 * t
+            psi = O.ted_time_evol(W, V, p0, t)
+            if tid == 5:
+                assert np.max(np.abs(psi - scipy.linalg.expm(-1j * M * t) @ p0)) < 1e-11
+            _, sig = EE.block_spectra(10, 2, 2, 0xff, psi)
+            se = EE.entropy_from_sigma(sig)
+            print(f"config {xid} t {t:.4f}: S {se_out} (oracle {se:.10f})", flush=True)
+            assert abs(se_out - se) <= 1e-9 * max(1.0, abs(se))
