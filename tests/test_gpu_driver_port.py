@@ -141,3 +141,37 @@ def test_drude_weight_driver_end_to_end(tmp_path):
         assert abs(e_gpu - e0) <= 1e-9 * abs(e0)
         assert abs(dxx - d[0]) <= 1e-7 * scale[0] and abs(dyy - d[1]) <= 1e-7 * scale[1]
         assert abs(dav - (d[0] + d[1]) / 2) <= 1e-7 * max(scale)
+
+
+def test_spinless_quench_driver_end_to_end(tmp_path):
+    """MBL_test/src/spinless_fermion_test.cu ported (tests/cpp/spinless_quench.cpp): 10-site
+    chain, 6 fermions, U = 1 (Insert_2Nop), x = 2, two seeded configurations of five steps; the
+    dense single-species matrix (H_TBq::Write), tED with dimDN = 1 and the A = {0..5} entropy
+    against the oracle (Species.dense, numpy eigh, S_ref of the Schmidt values)."""
+    from oracle import ee_oracle as EE
+    exe = os.path.join(BIN, "spinless_quench")
+    if not os.path.exists(exe):
+        pytest.fail("driver port missing: run __graft_entry__.build()")
+    print("spinless_fermion_test port: L=10, Q=6, U=1, x=2, 2 configurations x 5 steps", flush=True)
+    r = subprocess.run([exe, "10", "6", "1", "2", "5", "5", "2", "0", str(tmp_path)], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    text = open(tmp_path / "sf_EE_U1_x2.out").read()
+    blocks = [b for b in text.split("\n\n\n") if b.strip()]
+    rows = [[[float(t) for t in ln.split("\t")] for ln in b.strip().split("\n")] for b in blocks]
+    assert len(rows) == 2 and all(len(b) == 5 for b in rows)
+    draws = DIO.mt19937_uniform(5, 20)
+    base = [(L.OP_HOP, i, i + 1, -1.0) for i in range(9)] + [(L.OP_DENSITY2, i, i + 1, 1.0) for i in range(9)]
+    for xid in range(2):
+        sp = O.Species(10, 6, base + [(L.OP_DENSITY, i, i, 2.0 * float(v)) for i, v in
+                                      enumerate(draws[10 * xid:10 * xid + 10])])
+        W, V = np.linalg.eigh(sp.dense())
+        p0 = np.zeros(sp.count, dtype=np.complex128)
+        p0[sp.address[63]] = 1.0
+        for tid, (x_out, t_out, se_out) in enumerate(rows[xid]):
+            t = np.exp((tid + 1) * 0.02) - 1.0
+            psi = O.ted_time_evol(W, V, p0, t)
+            _, sig = EE.block_spectra(10, 6, 0, 0x3f, psi)
+            se = EE.entropy_from_sigma(sig)
+            print(f"config {xid} t {t:.4f}: S {se_out} (oracle {se:.10f})", flush=True)
+            assert x_out == xid and abs(se_out - se) <= 1e-9 * max(1.0, abs(se))
