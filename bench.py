@@ -123,10 +123,11 @@ def batched_drude_line():
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
-def ensemble_line(n_real, ncv=32):
+def ensemble_line(n_real, ncv=24):
     """14-site open chain, 7/7, U = 4, W = 8: E0 of all realisations in one batched launch, then per
     realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i, eight realisations at
-    a time on separate streams."""
+    a time on separate streams.  ncv = 24 basis vectors for the 4 lowest (32 realisations: 16, 20,
+    24, 32, 40 gave 0.90, 1.00, 1.01, 0.93, 0.82 realisations/s; 12 does not converge for all)."""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
     res = DS.run_ensemble(n_real=n_real, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=ncv, tol=1e-10,
                           log=lambda m: log("[bench] " + m), with_sref=False, workers=8)
@@ -135,7 +136,7 @@ def ensemble_line(n_real, ncv=32):
     return {"value": round(res.realisations_per_s, 3), "unit": "realisations/s", "realisations": n_real,
             "batched_e0_s": round(res.seconds_batched_e0, 3),
             "max_abs_e0_batched_minus_eigs": float(abs(res.e0_batched - res.evals[:, 0]).max()),
-            "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
+            "ncv": ncv, "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
 
 
 def krylov_line(max_steps=1000, dt=0.02):
