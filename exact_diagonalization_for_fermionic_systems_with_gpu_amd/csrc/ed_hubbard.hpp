@@ -124,6 +124,10 @@ int up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> 
 int64_t slab_len(const Hubbard &h);
 int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const double *w, const double *bdev,
                  int step, bool dot, double *partial, hipStream_t st, int64_t *nblocks);
+// one step of the Ritz-vector rebuild on the slab layout: slab_lanczos without the dot, with
+// ritz_update (f = y - a[step] v, evec += zr v) fused into pass B; y ends as f
+int slab_lanczos_ritz(const Hubbard &h, const double *f, double *y, double *v, const double *bdev,
+                      const double *adev, double zr, double *evec, int step, hipStream_t st);
 int64_t slab_num_partials(const Hubbard &h);
 int slab_from_rowmajor(const Hubbard &h, const double *x, double *xl, hipStream_t st);
 int slab_to_rowmajor(const Hubbard &h, const double *xl, double *x, hipStream_t st);

@@ -191,12 +191,18 @@ __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int
         st2(xt + ((pu0 >> 4) * a.dd + pd) * kSW + k, s_x[us * kSW + k][cc], s_x[us * kSW + k + 1][cc]);
 }
 
+// Pass B option of the Ritz-vector rebuild, where a[step] is already known: y is written as
+// f = y - a[step] v and evec += zr v, the arithmetic of ritz_update (ed_blas.hip) element for
+// element, so the rebuilt Lanczos vectors are the forward recurrence's bit for bit.
+enum { SLAB_RITZ = 16 };
+
 // Pass B: y = zin + DN term of xt + diag * xt (+ c w); v = xt (layout L); block partial <v, y>
 template <int FLAGS, int NB>
 __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
                                                  const double *__restrict__ xt, const double *zin, const double *win,
                                                  double *yout, double *vout, const double *__restrict__ bdev, int step,
-                                                 double *__restrict__ partial)
+                                                 double *__restrict__ partial, double *__restrict__ evec,
+                                                 const double *__restrict__ adev, double zr)
 {
     __shared__ double s_val[4 * kMaxOps];
     __shared__ uint32_t s_ent[kTR * kMaxOps];
@@ -264,8 +270,17 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
     s_b[r][c + 1] = xd1;
     __syncthreads();
     if (own) {
-        st2(yout + li, s_a[ds * kSW + k][pr], s_a[ds * kSW + k + 1][pr]);
-        if (FLAGS & HV_WRITEV) st2(vout + li, s_b[ds * kSW + k][pr], s_b[ds * kSW + k + 1][pr]);
+        double y0 = s_a[ds * kSW + k][pr], y1 = s_a[ds * kSW + k + 1][pr];
+        const double v0 = s_b[ds * kSW + k][pr], v1 = s_b[ds * kSW + k + 1][pr];
+        if (FLAGS & SLAB_RITZ) {
+            const double ca = -adev[step];
+            y0 = fma(ca, v0, y0);
+            y1 = fma(ca, v1, y1);
+            const double2 e = ld2(evec + li);
+            st2(evec + li, fma(zr, v0, e.x), fma(zr, v1, e.y));
+        }
+        st2(yout + li, y0, y1);
+        if (FLAGS & HV_WRITEV) st2(vout + li, v0, v1);
     }
     if (FLAGS & HV_DOT) {
         const double sum = block_sum<256>(dot, s_red);
@@ -335,7 +350,8 @@ int64_t slab_num_partials(const Hubbard &h) { return h.slab ? (h.slab->npart + 2
 
 // one H.v on layout L; flags as hv_lanczos (HV_SCALE applies to pass A, which pre-scales xt)
 static int slab_apply(const Hubbard &h, const double *x, double *z, const double *w, double *y, double *v,
-                      const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks)
+                      const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks,
+                      double *evec = nullptr, const double *adev = nullptr, double zr = 0.0)
 {
     const SlabLayout &L = *h.slab;
     // unused dynamic LDS per workgroup (bytes): fewer resident workgroups per CU, hence fewer
@@ -377,7 +393,7 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
     const int fb = flags & ~HV_SCALE;
 #define ED_SLAB_B1(F, N)                                                                                            \
     hipLaunchKernelGGL((k_slab_dn<F, N>), gB, dim3(256), pad, st, ad, tilesB, perB, ntrB, L.xt, z, w, y, v, bdev,   \
-                       step, L.part)
+                       step, L.part, evec, adev, zr)
 #define ED_SLAB_B(F)                                                                                               \
     do {                                                                                                           \
         if (nb == 16) ED_SLAB_B1(F, 16);                                                                           \
@@ -390,6 +406,8 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
     case HV_WRITEV | HV_AXPY: ED_SLAB_B(HV_WRITEV | HV_AXPY); break;
     case HV_WRITEV | HV_DOT: ED_SLAB_B(HV_WRITEV | HV_DOT); break;
     case HV_WRITEV | HV_AXPY | HV_DOT: ED_SLAB_B(HV_WRITEV | HV_AXPY | HV_DOT); break;
+    case HV_WRITEV | SLAB_RITZ: ED_SLAB_B(HV_WRITEV | SLAB_RITZ); break;
+    case HV_WRITEV | HV_AXPY | SLAB_RITZ: ED_SLAB_B(HV_WRITEV | HV_AXPY | SLAB_RITZ); break;
     default: return fail(ED_EINVAL, "slab H.v: unsupported flags %d", flags);
     }
 #undef ED_SLAB_B
@@ -410,6 +428,14 @@ int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const 
     int flags = HV_SCALE | HV_WRITEV | (step > 0 ? HV_AXPY : 0) | (dot ? HV_DOT : 0);
     // the UP partial goes into v (overwritten by pass B with v itself), w and y share a buffer
     return slab_apply(h, f, v, w, y, v, bdev, step, flags, partial, st, nblocks);
+}
+
+int slab_lanczos_ritz(const Hubbard &h, const double *f, double *y, double *v, const double *bdev,
+                      const double *adev, double zr, double *evec, int step, hipStream_t st)
+{
+    if (!evec || !adev) return fail(ED_EINVAL, "slab Ritz step: null vector");
+    const int flags = HV_SCALE | HV_WRITEV | (step > 0 ? HV_AXPY : 0) | SLAB_RITZ;
+    return slab_apply(h, f, v, y, y, v, bdev, step, flags, nullptr, st, nullptr, evec, adev, zr);
 }
 
 int slab_from_rowmajor(const Hubbard &h, const double *x, double *xl, hipStream_t st)

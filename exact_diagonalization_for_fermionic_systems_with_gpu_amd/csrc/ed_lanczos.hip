@@ -385,9 +385,11 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
     for (int i = 0; i < Nk + 1; ++i) {
-        if (slab) ED_TRY(slab_lanczos(*h, A, B, C, B, w.bdev, i, false, nullptr, st, nullptr));
-        else ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
-        ED_TRY(ritz_update(nreal, C, B, ev, w.adev, z[i], i, st));
+        if (slab) ED_TRY(slab_lanczos_ritz(*h, A, B, C, w.bdev, w.adev, z[i], ev, i, st)); // update fused
+        else {
+            ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
+            ED_TRY(ritz_update(nreal, C, B, ev, w.adev, z[i], i, st));
+        }
         if (reorth) {
             int64_t nbk;
             ED_TRY(lanczos_update(nreal, C, B, w.ddev, i, w.partial, &nbk, st));
@@ -418,14 +420,29 @@ extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int ns
     const bool cvec = complex_vectors != 0;
     if (h->cplx_tables && !cvec) return fail(ED_EINVAL, "ed_lanczos_tridiagonal: complex tables need complex vectors");
     hipStream_t st = stream_of(stream);
-    const int64_t n = h->args.dim_up * h->args.dim_dn, nreal = cvec ? 2 * n : n;
+    const int64_t n = h->args.dim_up * h->args.dim_dn;
+    // real sectors run on the slab layout when the model has one (or gets one here, from 2^27
+    // amplitudes on): only the coefficients leave, so f is converted once and nothing comes back
+    if (!cvec) slab_auto(*const_cast<Hubbard *>(h));
+    const bool slab = h->slab && !cvec;
+    const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
     Work w;
     ED_TRY(alloc_work(w, *h, nreal, nsteps, true));
     double *f = nullptr;
     ED_TRY(dmalloc((void **)&f, sizeof(double) * (size_t)nreal));
     int s = ED_OK;
     do {
-        if (hipMemcpyAsync(f, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        if (slab) {
+            // padding entries stay zero in every vector of the recurrence
+            for (double *b : {w.buf[0], w.buf[1], f})
+                if (hipMemsetAsync(b, 0, sizeof(double) * (size_t)nreal, st) != hipSuccess) s = ED_EHIP;
+            if (s) {
+                s = fail(ED_EHIP, "ed_lanczos_tridiagonal: clear failed");
+                break;
+            }
+            if ((s = slab_from_rowmajor(*h, (const double *)d_f, f, st))) break;
+        } else if (hipMemcpyAsync(f, d_f, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st) !=
+                   hipSuccess) {
             s = fail(ED_EHIP, "ed_lanczos_tridiagonal: copy failed");
             break;
         }
@@ -434,7 +451,7 @@ extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int ns
         if ((s = finalize_sum(w.partial, nb, w.bdev, 1, st))) break;
         double *A = f, *B = w.buf[0], *C = w.buf[1];
         for (int N = 0; N < nsteps && s == ED_OK; ++N) {
-            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, false, st);
+            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, slab, st);
             double *t = A;
             A = B;
             B = C;

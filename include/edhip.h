@@ -224,7 +224,8 @@ int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cf
                      void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream);
 
 /* Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): nsteps fixed steps from d_f
- * (not modified); a_out[nsteps], b_out[nsteps+1] with b_out[0] = |f|. */
+ * (not modified); a_out[nsteps], b_out[nsteps+1] with b_out[0] = |f|.  Real vectors run on
+ * the slab layout when the operator has one, or gets one as in ed_lanczos_eigen. */
 int ed_lanczos_tridiagonal(ed_hubbard h, int complex_vectors, int nsteps, const void *d_f, double *a_out,
                            double *b_out, void *stream);
 

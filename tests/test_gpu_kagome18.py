@@ -28,10 +28,16 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden", "kagome18y_lanczos100_o
 
 
 @pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
-def test_kagome18y_first_100_lanczos_steps():
+@pytest.mark.parametrize("layout", ["reference", "slab"])
+def test_kagome18y_first_100_lanczos_steps(layout, monkeypatch):
+    """layout "reference": the recurrence on the reference's vector order (automatic slab
+    layout switched off); "slab": the sector is above 2^27 amplitudes, so the solver builds and
+    uses the slab layout by itself"""
     gold = json.load(open(GOLD))
     n = gold["nsteps"]
-    print("kagome18y 9/9 U=8: 100 Lanczos steps at D = 2.36e9", flush=True)
+    print(f"kagome18y 9/9 U=8: 100 Lanczos steps at D = 2.36e9, {layout} layout", flush=True)
+    if layout == "reference":
+        monkeypatch.setenv("ED_SLAB_AUTO", "0")
     hq = H_TBq(18, 9)
     hq.add_ops(L.hubbard_ops(L.KAGOME18Y, t=-1.0))
     H = Hubbard(hq, hq, 8.0)
@@ -39,6 +45,8 @@ def test_kagome18y_first_100_lanczos_steps():
     lz = Lanczos(H.dimUP, H.dimDN)
     lz.random_vector(f)
     lz.tridiagonal(H, f, n)
+    assert (H.slab_len_query() >= H.dim) == (layout == "slab")
+    H.slab_layout(False)
     a, b = np.array(lz.a), np.array(lz.b)
     del f
     torch.cuda.empty_cache()

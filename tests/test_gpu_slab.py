@@ -91,9 +91,42 @@ def test_slab_lanczos_matches_reference_layout():
     assert abs(e_slab - e_ref) <= 1e-10 * abs(e_ref)
     ov = abs(float(torch.dot(ev, v_ref)))
     assert abs(ov - 1.0) <= 1e-8
+    # the rebuild (Ritz update fused into pass B) gives a unit eigenvector of H
+    assert abs(float(torch.linalg.norm(ev)) - 1.0) <= 1e-12
+    res = float(torch.linalg.norm(H.apply(ev) - e_slab * ev))
+    assert res <= 1e-6 * abs(e_slab), res
     st, e0o, *_ = O.Hubbard(O.Species(N, qu, ou), O.Species(N, qd, od), U).lanczos_eigen(O.random_vector(H.dim))
     assert abs(e_slab - e0o) <= 1e-10 * abs(e0o)
     H.slab_layout(False)
+
+
+def test_slab_tridiagonal_matches_reference_layout():
+    """Lanczos<T>::tridiagonal on the slab layout: the same coefficients as on the reference
+    layout up to the rounding of the two summation orders, and the oracle's to the bar of the
+    reference-layout test (1e-9 absolute over the first 40 steps)"""
+    print("slab tridiagonal kagome12 6/6 U=4, 200 steps", flush=True)
+    H, (N, qu, qd, ou, od, U, uv) = _build("kagome12_6_6")
+    f0 = O.random_vector(H.dim)
+    f = torch.from_numpy(f0).to(DEV)
+    l_ref = Lanczos(H.dimUP, H.dimDN)
+    l_ref.tridiagonal(H, f, 200)
+    H.slab_layout(True)
+    l_slab = Lanczos(H.dimUP, H.dimDN)
+    l_slab.tridiagonal(H, f, 200)
+    assert torch.equal(f, torch.from_numpy(f0).to(DEV))  # d_f is not modified
+    H.slab_layout(False)
+    a, b = np.array(l_slab.a), np.array(l_slab.b)
+    assert len(a) == 200 and len(b) == 201
+    assert np.abs(a[:40] - np.array(l_ref.a[:40])).max() <= 1e-12
+    assert np.abs(b[:41] - np.array(l_ref.b[:41])).max() <= 1e-12
+    ao, bo = O.Hubbard(O.Species(N, qu, ou), O.Species(N, qd, od), U).lanczos_tridiagonal(f0, 200)
+    assert np.allclose(a[:40], ao[:40], rtol=0, atol=1e-9)
+    assert np.allclose(b[:41], bo[:41], rtol=0, atol=1e-9)
+    # the lowest Ritz value of T_200 is converged: it is E0 on both layouts
+    from scipy.linalg import eigvalsh_tridiagonal
+    t_s = eigvalsh_tridiagonal(a, b[1:200], select="i", select_range=(0, 0))[0]
+    t_r = eigvalsh_tridiagonal(np.array(l_ref.a), np.array(l_ref.b[1:200]), select="i", select_range=(0, 0))[0]
+    assert abs(t_s - t_r) <= 1e-10 * abs(t_r)
 
 
 def test_slab_rejects_complex_operator():
