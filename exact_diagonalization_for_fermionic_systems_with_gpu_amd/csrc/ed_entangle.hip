@@ -23,8 +23,12 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -258,6 +262,41 @@ static rocblas_handle solver_handle()
     return h;
 }
 
+// worker streams (default priority) and rocSOLVER handles of one device for the concurrent block
+// solves, created on first use and kept until the process ends; calls that use them take turns
+constexpr int kEeWorkers = 4;
+struct EePool {
+    hipStream_t st[kEeWorkers] = {};
+    rocblas_handle h[kEeWorkers] = {};
+};
+static std::mutex g_ee_mu;
+static EePool *g_ee_pool[64] = {};
+
+// the pool of the current device (g_ee_mu held); nullptr if it cannot be made
+static EePool *ee_pool(int dev)
+{
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (g_ee_pool[dev]) return g_ee_pool[dev];
+    EePool *p = new EePool;
+    bool ok = true;
+    for (int i = 0; i < kEeWorkers && ok; ++i) {
+        ok = hipStreamCreateWithFlags(&p->st[i], hipStreamNonBlocking) == hipSuccess &&
+             rocblas_create_handle(&p->h[i]) == rocblas_status_success &&
+             rocblas_set_stream(p->h[i], p->st[i]) == rocblas_status_success;
+    }
+    if (!ok) {
+        for (int i = 0; i < kEeWorkers; ++i) {
+            if (p->h[i]) rocblas_destroy_handle(p->h[i]);
+            if (p->st[i]) (void)hipStreamDestroy(p->st[i]);
+        }
+        delete p;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_ee_pool[dev] = p;
+    return p;
+}
+
 // spectra of all blocks: lambda (rho eigenvalues, clipped at 0) and sigma (Schmidt values)
 static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const void *d_psi, hipStream_t st,
                          std::vector<double> &lambda, std::vector<double> &sigma, bool want_sigma)
@@ -275,30 +314,82 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
     if (!h) return fail(ED_EHIP, "entanglement: rocblas handle creation failed");
     rocblas_set_stream(h, st);
 
-    double *dM = nullptr, *drho = nullptr, *dW = nullptr, *dE = nullptr;
+    double *dM = nullptr, *dR = nullptr, *dWall = nullptr, *dSall = nullptr, *dE = nullptr;
     int32_t *dcu = nullptr, *dcd = nullptr;
     BlockGeo *dgeo = nullptr;
     rocblas_int *dinfo = nullptr;
-    int64_t maxk = 1, maxK = 1;
-    for (const BlockGeo &G : P.geo) {
+    int64_t maxk = 1, maxK = 1, nvals = 0;
+    std::vector<int> jobs;       // nonempty blocks, plan order
+    std::vector<int64_t> woff;   // offset of each job's values in dWall / dSall
+    for (int g = 0; g < (int)P.geo.size(); ++g) {
+        const BlockGeo &G = P.geo[g];
+        if (G.rows == 0 || G.cols == 0) continue;
         maxk = std::max(maxk, G.rows);
         maxK = std::max(maxK, G.cols);
+        jobs.push_back(g);
+        woff.push_back(nvals);
+        nvals += G.rows;
     }
+    const int njobs = (int)jobs.size();
+    std::vector<int> order(njobs);
+    for (int b = 0; b < njobs; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return P.geo[jobs[x]].rows > P.geo[jobs[y]].rows; });
+    // when the largest block has 2048 rows or more (ED_EE_PARALLEL_MIN), the blocks are solved
+    // kEeWorkers at a time (rocSOLVER's tridiagonal reduction is launch- and matrix-vector bound:
+    // one alone leaves most of the GPU idle).  Each block's arithmetic is the same either way.
+    const char *pm = getenv("ED_EE_PARALLEL_MIN");
+    const int64_t par_min = pm ? std::max<int64_t>(1, atoll(pm)) : 2048;
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    EePool *pool = nullptr;
+    std::unique_lock<std::mutex> pool_lock;
+    int nw = 1;
+    if (maxk >= par_min && njobs > 1) {
+        pool_lock = std::unique_lock<std::mutex>(g_ee_mu);
+        pool = ee_pool(dev);
+        if (pool) nw = std::min(kEeWorkers, njobs);
+        else pool_lock.unlock(); // no worker streams: one block at a time on st
+        // each worker holds one rho block: fewer workers when their blocks would not fit
+        size_t free_b = 0, total_b = 0;
+        if (nw > 1 && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const double blk = (double)(sizeof(double) * NC) * (double)maxk * (double)maxk;
+            const double rest = (double)(sizeof(double) * NC) * (double)P.total;
+            while (nw > 1 && rest + nw * blk > 0.8 * (double)free_b) --nw;
+        }
+    }
+    // a worker's failure is reported by the calling thread (the error text is per thread)
+    std::vector<std::string> wmsg((size_t)nw);
+    auto wfail = [&](int wi, const char *msg, int code) {
+        char buf[160];
+        snprintf(buf, sizeof buf, msg, code);
+        wmsg[wi] = buf;
+    };
+    hipEvent_t ready = nullptr;
+    ED_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } ev_guard{ready};
     int s = ED_OK;
     static thread_local Arena t_ee_arena;
     Scratch S(t_ee_arena, st);
     do {
-        const size_t bM = sizeof(double) * NC * (size_t)P.total, bR = sizeof(double) * NC * (size_t)(maxk * maxk),
-                     bW = sizeof(double) * (size_t)maxk, bE = sizeof(double) * (size_t)std::max(maxk, maxK),
+        const size_t bM = sizeof(double) * NC * (size_t)P.total,
+                     bR = Scratch::round(sizeof(double) * NC * (size_t)(maxk * maxk)),
+                     bV = sizeof(double) * (size_t)std::max<int64_t>(nvals, 1),
+                     bE = Scratch::round(sizeof(double) * (size_t)std::max(maxk, maxK)),
                      bU = sizeof(int32_t) * P.cu.size(), bD = sizeof(int32_t) * P.cd.size(),
-                     bG = sizeof(BlockGeo) * P.geo.size(), bI = sizeof(rocblas_int);
-        if ((s = S.reserve(Scratch::round(bM) + Scratch::round(bR) + Scratch::round(bW) + Scratch::round(bE) +
-                           Scratch::round(bU) + Scratch::round(bD) + Scratch::round(bG) + Scratch::round(bI))))
+                     bG = sizeof(BlockGeo) * P.geo.size(),
+                     bI = sizeof(rocblas_int) * (size_t)std::max(2 * njobs, 1);
+        if ((s = S.reserve(Scratch::round(bM) + nw * bR + 2 * Scratch::round(bV) + nw * bE + Scratch::round(bU) +
+                           Scratch::round(bD) + Scratch::round(bG) + Scratch::round(bI))))
             break;
         dM = (double *)S.take(bM);
-        drho = (double *)S.take(bR);
-        dW = (double *)S.take(bW);
-        dE = (double *)S.take(bE);
+        dR = (double *)S.take(nw * bR);
+        dWall = (double *)S.take(bV);
+        dSall = (double *)S.take(bV);
+        dE = (double *)S.take(nw * bE);
         dcu = (int32_t *)S.take(bU);
         dcd = (int32_t *)S.take(bD);
         dgeo = (BlockGeo *)S.take(bG);
@@ -321,68 +412,100 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
             s = fail(ED_EHIP, "entanglement: gather launch failed");
             break;
         }
-        std::vector<double> w;
-        for (const BlockGeo &G : P.geo) {
-            const int64_t k = G.rows, K = G.cols;
-            if (k == 0 || K == 0) continue;
-            double *A = dM + NC * G.off;
-            const int ntile = (int)((k + TT - 1) / TT);
-            const int64_t nblk = (int64_t)ntile * (ntile + 1) / 2;
-            if (cplx)
-                hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(256), 0, st, k, K, A, drho, ntile);
-            else
-                hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, st, k, K, A, drho, ntile);
-            if (hipGetLastError() != hipSuccess) {
-                s = fail(ED_EHIP, "entanglement: gram launch failed");
-                break;
-            }
-            rocblas_status rs;
-            if (cplx)
-                rs = rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k,
-                                      (rocblas_double_complex *)drho, (rocblas_int)k, dW, dE, dinfo);
-            else
-                rs = rocsolver_dsyevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k, drho,
-                                      (rocblas_int)k, dW, dE, dinfo);
-            if (rs != rocblas_status_success) {
-                s = fail(ED_EHIP, "entanglement: rocsolver syevd failed (%d)", (int)rs);
-                break;
-            }
-            w.assign((size_t)k, 0.0);
-            rocblas_int info = 0;
-            if (hipMemcpyAsync(w.data(), dW, sizeof(double) * k, hipMemcpyDeviceToHost, st) ||
-                hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)) {
-                s = fail(ED_EHIP, "entanglement: readback failed");
-                break;
-            }
-            if (info != 0) {
-                s = fail(ED_EHIP, "entanglement: syevd info %d", (int)info);
-                break;
-            }
-            for (double v : w) lambda.push_back(v > 0.0 ? v : 0.0);
-            if (!want_sigma) continue;
-            // Schmidt values: SVD of the block (column-major K x k, ld K); A is overwritten
-            if (cplx)
-                rs = rocsolver_zgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k,
-                                      (rocblas_double_complex *)A, (rocblas_int)K, dW, nullptr, 1, nullptr, 1, dE,
-                                      rocblas_outofplace, dinfo);
-            else
-                rs = rocsolver_dgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k, A,
-                                      (rocblas_int)K, dW, nullptr, 1, nullptr, 1, dE, rocblas_outofplace, dinfo);
-            if (rs != rocblas_status_success) {
-                s = fail(ED_EHIP, "entanglement: rocsolver gesvd failed (%d)", (int)rs);
-                break;
-            }
-            if (hipMemcpyAsync(w.data(), dW, sizeof(double) * k, hipMemcpyDeviceToHost, st) ||
-                hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)) {
-                s = fail(ED_EHIP, "entanglement: readback failed");
-                break;
-            }
-            if (info != 0) {
-                s = fail(ED_EHIP, "entanglement: gesvd info %d", (int)info);
-                break;
-            }
-            for (double v : w) sigma.push_back(v);
+        if (hipEventRecord(ready, st) != hipSuccess) {
+            s = fail(ED_EHIP, "entanglement: event record failed");
+            break;
         }
+        // the blocks go to the workers largest first; each worker runs Gram, syevd (and gesvd)
+        // on its own stream, rho buffer and rocSOLVER handle, and leaves the values in its
+        // block's slot of dWall / dSall
+        std::atomic<int> next(0);
+        auto worker = [&](int wi, hipStream_t ws, rocblas_handle hw) {
+            if (nw > 1 && hipSetDevice(dev) != hipSuccess) {
+                wfail(wi, "entanglement: worker device selection failed (%d)", 0);
+                return;
+            }
+            if (hipStreamWaitEvent(ws, ready, 0) != hipSuccess) {
+                wfail(wi, "entanglement: worker stream wait failed (%d)", 0);
+                return;
+            }
+            double *rho = dR + (size_t)wi * (bR / sizeof(double)), *E = dE + (size_t)wi * (bE / sizeof(double));
+            for (int q = next.fetch_add(1); q < njobs; q = next.fetch_add(1)) {
+                const int b = order[q];
+                const BlockGeo &G = P.geo[jobs[b]];
+                const int64_t k = G.rows, K = G.cols;
+                double *A = dM + NC * G.off;
+                const int ntile = (int)((k + TT - 1) / TT);
+                const int64_t nblk = (int64_t)ntile * (ntile + 1) / 2;
+                if (cplx)
+                    hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
+                else
+                    hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
+                if (hipGetLastError() != hipSuccess) {
+                    wfail(wi, "entanglement: gram launch failed (%d)", 0);
+                    return;
+                }
+                rocblas_status rs;
+                if (cplx)
+                    rs = rocsolver_zheevd(hw, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k,
+                                          (rocblas_double_complex *)rho, (rocblas_int)k, dWall + woff[b], E,
+                                          dinfo + b);
+                else
+                    rs = rocsolver_dsyevd(hw, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k, rho,
+                                          (rocblas_int)k, dWall + woff[b], E, dinfo + b);
+                if (rs != rocblas_status_success) {
+                    wfail(wi, "entanglement: rocsolver syevd failed (%d)", (int)rs);
+                    return;
+                }
+                if (!want_sigma) continue;
+                // Schmidt values: SVD of the block (column-major K x k, ld K); A is overwritten
+                if (cplx)
+                    rs = rocsolver_zgesvd(hw, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k,
+                                          (rocblas_double_complex *)A, (rocblas_int)K, dSall + woff[b], nullptr, 1,
+                                          nullptr, 1, E, rocblas_outofplace, dinfo + njobs + b);
+                else
+                    rs = rocsolver_dgesvd(hw, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k,
+                                          A, (rocblas_int)K, dSall + woff[b], nullptr, 1, nullptr, 1, E,
+                                          rocblas_outofplace, dinfo + njobs + b);
+                if (rs != rocblas_status_success) {
+                    wfail(wi, "entanglement: rocsolver gesvd failed (%d)", (int)rs);
+                    return;
+                }
+            }
+        };
+        if (nw == 1) {
+            worker(0, st, h);
+        } else {
+            std::vector<std::thread> th;
+            for (int wi = 0; wi < nw; ++wi) th.emplace_back(worker, wi, pool->st[wi], pool->h[wi]);
+            for (std::thread &t : th) t.join();
+            for (int wi = 0; wi < nw; ++wi)
+                if (hipStreamSynchronize(pool->st[wi]) != hipSuccess && wmsg[wi].empty())
+                    wfail(wi, "entanglement: worker stream failed (%d)", 0);
+        }
+        for (int wi = 0; wi < nw && s == ED_OK; ++wi)
+            if (!wmsg[wi].empty()) s = fail(ED_EHIP, "%s", wmsg[wi].c_str());
+        if (s) break;
+        std::vector<double> wv((size_t)nvals), sv(want_sigma ? (size_t)nvals : 0);
+        std::vector<rocblas_int> info((size_t)(2 * njobs), 0);
+        if (hipMemcpyAsync(wv.data(), dWall, sizeof(double) * (size_t)nvals, hipMemcpyDeviceToHost, st) ||
+            (want_sigma &&
+             hipMemcpyAsync(sv.data(), dSall, sizeof(double) * (size_t)nvals, hipMemcpyDeviceToHost, st)) ||
+            hipMemcpyAsync(info.data(), dinfo, sizeof(rocblas_int) * (size_t)(want_sigma ? 2 * njobs : njobs),
+                           hipMemcpyDeviceToHost, st) ||
+            hipStreamSynchronize(st)) {
+            s = fail(ED_EHIP, "entanglement: readback failed");
+            break;
+        }
+        for (int b = 0; b < njobs && s == ED_OK; ++b) {
+            if (info[b] != 0) s = fail(ED_EHIP, "entanglement: syevd info %d", (int)info[b]);
+            else if (want_sigma && info[njobs + b] != 0)
+                s = fail(ED_EHIP, "entanglement: gesvd info %d", (int)info[njobs + b]);
+        }
+        if (s) break;
+        // block order, as the blocks are listed in the plan
+        for (double v : wv) lambda.push_back(v > 0.0 ? v : 0.0);
+        for (double v : sv) sigma.push_back(v);
     } while (0);
     return s;
 }
