@@ -76,7 +76,24 @@ struct SlabArgs {
     int64_t mask;
     const double *Uvec;
     int has_diag;
+    int swz; // tiles per CU slot in the tile order (slab_tile); 0 or 1: plain order
 };
+
+// Tile of workgroup b.  The 8 XCDs take contiguous eighths of the tiles (workgroups are dealt to
+// the XCDs round-robin).  With swz > 1, each run of 32 * swz consecutive workgroups of one XCD
+// takes its tiles transposed: workgroups i and i + 32 (the next one dealt to the same CU when
+// dispatch goes round-robin over the XCD's 32 CUs) get adjacent tiles, so that the workgroups
+// resident on one CU gather from overlapping rows.  A bijection on every full run; the last
+// partial run keeps the plain order.
+__device__ __forceinline__ int64_t slab_tile(int64_t b, int64_t per_xcd, int swz)
+{
+    int64_t i = b >> 3;
+    if (swz > 1) {
+        const int64_t G = 32 * (int64_t)swz, g = i / G, r = i - g * G;
+        if ((g + 1) * G <= per_xcd) i = g * G + (r & 31) * swz + (r >> 5);
+    }
+    return (b & 7) * per_xcd + i;
+}
 
 __device__ __forceinline__ double slab_diag(int32_t su, int32_t sd, const SlabArgs &a)
 {
@@ -158,7 +175,7 @@ __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int
     __shared__ int s_off[kTR + 1];
     __shared__ double s_x[kTR][kSW + 1];
     const int64_t b = blockIdx.x;
-    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    const int64_t t = slab_tile(b, per_xcd, a.swz);
     if (t >= ntiles) return; // uniform over the workgroup
     const int64_t slab = t / ntr, p0 = (t - slab * ntr) * kTR;
     const int64_t pend = (p0 + kTR < a.du) ? p0 + kTR : a.du;
@@ -211,7 +228,7 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
     __shared__ double s_b[kTR][kSW + 1];
     __shared__ double s_red[4];
     const int64_t b = blockIdx.x;
-    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    const int64_t t = slab_tile(b, per_xcd, a.swz);
     if (t >= ntiles) return; // uniform over the workgroup
     const int64_t su = t / ntr, p0 = (t - su * ntr) * kTR;
     const int64_t pend = (p0 + kTR < a.dd) ? p0 + kTR : a.dd;
@@ -342,6 +359,7 @@ static SlabArgs slab_args(const Hubbard &h, bool up)
     a.mask = h.otf.mask;
     a.Uvec = h.otf.Uvec;
     a.has_diag = h.otf.has_diag;
+    a.swz = 0;
     return a;
 }
 
@@ -363,7 +381,16 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
         const long v = e ? atol(e) : 0;
         return (size_t)(v > 0 && v <= 120000 ? v : 0);
     }();
-    const SlabArgs au = slab_args(h, true), ad = slab_args(h, false);
+    // tile order on the XCDs (slab_tile; ED_SLAB_SWZ = 0, 2, 4, 8, 16 or 32, default 8).  At 18
+    // sites, H.v took 54.9-56.1 ms in the plain order, 53.7 ms with 8 adjacent tiles per CU slot
+    // (2, 4, 16, 32: 54.9, 55.8, 54.2, 54.8 ms), on two boxes.
+    static const int swz = [] {
+        const char *e = getenv("ED_SLAB_SWZ");
+        const int v = e ? atoi(e) : 8;
+        return (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) ? v : 0;
+    }();
+    SlabArgs au = slab_args(h, true), ad = slab_args(h, false);
+    au.swz = ad.swz = swz;
     const int64_t ntrA = (L.du + kTR - 1) / kTR, tilesA = L.nsd * ntrA, perA = (tilesA + 7) / 8;
     const int64_t ntrB = (L.dd + kTR - 1) / kTR, tilesB = L.nsu * ntrB, perB = (tilesB + 7) / 8;
     if (perA * 8 > 0x7fffffffLL || perB * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab H.v: grid too large");
