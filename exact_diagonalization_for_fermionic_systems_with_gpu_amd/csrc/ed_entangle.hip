@@ -76,7 +76,9 @@ __global__ __launch_bounds__(256) void k_gather_blocks(int64_t dim_up, int64_t d
 // ---------------------------------------------------------------------------------
 // 2. rho = A A^T (A A^H), A row-major k x K (lda = K), rho column-major k x k (ld = k),
 //    upper-triangle tiles only.  256 threads = 4 waves, 64x64 tile per workgroup, each
-//    wave a 32x32 quadrant = 2x2 MFMA 16x16 tiles; K staged through LDS 16 at a time.
+//    wave a 32x32 quadrant = 2x2 MFMA 16x16 tiles; K staged through LDS 16 at a time, the
+//    next 16 loaded into registers while the MFMAs of the current 16 run.  (128x128 tiles,
+//    at one wave per SIMD, measured no faster at 18 sites: 0.623 against 0.618 s.)
 //    MFMA f64 16x16x4 operand layout: A lane l -> (row l%16, k l/16); B lane l ->
 //    (col l%16, k l/16); D lane l -> (rows l/16 + 4r, col l%16), r = 0..3 (checked on
 //    gfx950 with a one-wave decode of (i+1)(j+1) products).
@@ -104,37 +106,54 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
     __shared__ double sa[NC][TT][TK + 1];
     __shared__ double sb[NC][TT][TK + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
-    dbl4 acc[2][2], acci[2][2];
+    constexpr int NP = TT / 32;
+    const int wi = (wave >> 1) * (TT / 2), wj = (wave & 1) * (TT / 2);
+    dbl4 acc[NP][NP], acci[NP][NP];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NP; ++q) {
             acc[p][q] = dbl4{0, 0, 0, 0};
             acci[p][q] = dbl4{0, 0, 0, 0};
         }
-    for (int64_t k0 = 0; k0 < K; k0 += TK) {
-        // stage 64 x 16 of rows i0.. and j0..
-        for (int e = tid; e < TT * TK; e += 256) {
-            const int r = e / TK, c = e % TK;
-            const int64_t kk = k0 + c;
-            const int64_t ri = i0 + r, rj = j0 + r;
+    // the next 64 x 16 chunks of rows i0.. and j0.. are loaded into registers while the MFMAs
+    // of the current chunk run (4 elements per thread per matrix; zero outside the block)
+    constexpr int EPT = TT * TK / 256;
+    double ra[EPT][NC], rb[EPT][NC];
+    auto fetch = [&](int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * 256, r = e / TK, c = e % TK;
+            const int64_t kk = k0 + c, ri = i0 + r, rj = j0 + r;
 #pragma unroll
             for (int z = 0; z < NC; ++z) {
-                sa[z][r][c] = (ri < k && kk < K) ? A[(ri * K + kk) * NC + z] : 0.0;
-                sb[z][r][c] = (rj < k && kk < K) ? A[(rj * K + kk) * NC + z] : 0.0;
+                ra[q][z] = (ri < k && kk < K) ? A[(ri * K + kk) * NC + z] : 0.0;
+                rb[q][z] = (rj < k && kk < K) ? A[(rj * K + kk) * NC + z] : 0.0;
+            }
+        }
+    };
+    fetch(0);
+    for (int64_t k0 = 0; k0 < K; k0 += TK) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * 256, r = e / TK, c = e % TK;
+#pragma unroll
+            for (int z = 0; z < NC; ++z) {
+                sa[z][r][c] = ra[q][z];
+                sb[z][r][c] = rb[q][z];
             }
         }
         __syncthreads();
+        if (k0 + TK < K) fetch(k0 + TK);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
             const int kl = kk + (lane >> 4), rl = lane & 15;
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
+            for (int p = 0; p < NP; ++p) {
                 const double ar = sa[0][wi + p * 16 + rl][kl];
                 const double ai = CPLX ? sa[NC - 1][wi + p * 16 + rl][kl] : 0.0;
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
+                for (int q = 0; q < NP; ++q) {
                     const double br = sb[0][wj + q * 16 + rl][kl];
                     acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc[p][q], 0, 0, 0);
                     if (CPLX) {
@@ -150,9 +169,9 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
     }
     // store: rho(i, j) column-major at i + j*k
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NP; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t i = i0 + wi + p * 16 + (lane >> 4) + 4 * r;

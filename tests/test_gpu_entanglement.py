@@ -107,3 +107,4 @@ def test_concurrent_block_solves_match_sequential(N, Q, nA, monkeypatch):
     if N == 12:
         lam_o, _ = E.block_spectra(N, Q, Q, (1 << nA) - 1, psi.cpu().numpy())
         assert np.allclose(lam_4, np.sort(lam_o)[::-1], atol=1e-13)
+
