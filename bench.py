@@ -68,15 +68,17 @@ def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
     return 16.0 * elems / dt2 / 1e9, rows2, dt2, O.num_threads()
 
 
-def ground_state_line(H, hq, x, y, N, Q):
+def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     """Lanczos ground state (reference convergence test) of the benchmark Hamiltonian from the
-    seed-235 start, Ritz vector rebuilt by recomputation, then S_vN of sites 0..N/2-1."""
+    seed-235 start, Ritz vector rebuilt by recomputation, then S_vN of sites 0..N/2-1.  reorth:
+    a second Gram-Schmidt pass of every f against v_N (DGKS), the reorthogonalisation that three
+    resident vectors allow."""
     import ctypes as C
     import numpy as np
     import torch
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import Lanczos
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
-    lz = Lanczos(hq.count, hq.count)
+    lz = Lanczos(hq.count, hq.count, reorth=reorth)
     lz.random_vector(x)
     H.slab_layout(True)  # the recurrence runs on the slab layout (start and Ritz vectors converted)
     torch.cuda.synchronize()
@@ -93,8 +95,8 @@ def ground_state_line(H, hq, x, y, N, Q):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     log(f"[bench] S_vN({N // 2}|{N - N // 2}) {s[0]:.15f} in {t2 - t1:.1f} s")
-    return {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "S_vN": float(s[0]),
-            "entropy_s": round(t2 - t1, 2)}
+    return {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "reorth": bool(reorth),
+            "S_vN": float(s[0]), "entropy_s": round(t2 - t1, 2)}
 
 
 def batched_drude_line():
@@ -187,6 +189,8 @@ def main():
     ap.add_argument("--ground-state", action="store_true",
                     help="also run the Lanczos ground state of the same workload (Ritz vector rebuilt) and its "
                          "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")
+    ap.add_argument("--reorth", action="store_true",
+                    help="with --ground-state: a DGKS pass of each new Lanczos vector against v_N")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -328,7 +332,7 @@ def main():
                                          f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
         del x_host
     if args.ground_state:
-        out["ground_state"] = ground_state_line(H, hq, x, y, N, Q)
+        out["ground_state"] = ground_state_line(H, hq, x, y, N, Q, reorth=args.reorth)
     if not args.no_extras:
         del x, y
         torch.cuda.empty_cache()

@@ -70,25 +70,25 @@ def test_kagome12_ground_state_entropy():
     assert abs(sr - E.entropy_from_sigma(sig_o)) <= 1e-8 * abs(sr)
 
 
-def _entropy_vn(N, Q, nA, psi):
+def _entropy_vn(N, Q, nA, psi, cplx=False):
     import ctypes as C
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
     out = np.zeros(1)
-    call("ed_entanglement_entropy_vn", N, Q, Q, (1 << nA) - 1, 0, dptr(psi), out.ctypes.data_as(C.POINTER(C.c_double)),
+    call("ed_entanglement_entropy_vn", N, Q, Q, (1 << nA) - 1, int(cplx), dptr(psi), out.ctypes.data_as(C.POINTER(C.c_double)),
          stream_ptr())
     return float(out[0])
 
 
-@pytest.mark.parametrize("N,Q,nA", [(12, 6, 6), (16, 8, 8)])
-def test_concurrent_block_solves_match_sequential(N, Q, nA, monkeypatch):
+@pytest.mark.parametrize("N,Q,nA,cplx", [(12, 6, 6, False), (12, 6, 6, True), (16, 8, 8, False)])
+def test_concurrent_block_solves_match_sequential(N, Q, nA, cplx, monkeypatch):
     """the blocks solved on four worker streams give the one-stream spectra bit for bit (same
     arithmetic per block; the threshold ED_EE_PARALLEL_MIN selects the path).  16 sites 8|8 has
     blocks of 4,900 rows, which take the concurrent path by default.  The Schmidt values (SVD of
     each block) are compared at 12 sites only: rocSOLVER's bidiagonal SVD takes minutes on the
     4,900-row blocks."""
-    print(f"ee concurrent vs sequential N={N} {Q}/{Q} A=0..{nA - 1}", flush=True)
+    print(f"ee concurrent vs sequential N={N} {Q}/{Q} A=0..{nA - 1} cplx={cplx}", flush=True)
     hq = H_TBq(N, Q)
-    psi = torch.from_numpy(_state(hq.count * hq.count, N, False)).cuda()
+    psi = torch.from_numpy(_state(hq.count * hq.count, N, cplx)).cuda()
     ee = EEmatHubAB_GPU(list(range(nA)), nA, hq, hq)
     runs = {}
     for label, par_min in (("one stream", "1000000000"), ("four streams", "1")):
@@ -96,7 +96,7 @@ def test_concurrent_block_solves_match_sequential(N, Q, nA, monkeypatch):
         print(f"  {label}: spectrum", flush=True)
         lam = ee.spectrum(psi)
         print(f"  {label}: S_vN", flush=True)
-        svn = _entropy_vn(N, Q, nA, psi)
+        svn = _entropy_vn(N, Q, nA, psi, cplx)
         full = ee.entropy(psi) if N <= 12 else None
         runs[label] = (lam, svn, full)
     (lam_1, s_1, f_1), (lam_4, s_4, f_4) = runs["one stream"], runs["four streams"]

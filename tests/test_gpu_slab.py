@@ -172,3 +172,30 @@ def test_slab_kagome18y_first_100_steps():
     from scipy.linalg import eigvalsh_tridiagonal
     theta = eigvalsh_tridiagonal(a, b[1:n], select="i", select_range=(0, 0))[0]
     assert abs(theta - gold["ritz_lowest"]) < 1e-10 * abs(gold["ritz_lowest"]), (theta, gold["ritz_lowest"])
+
+
+@pytest.mark.parametrize("layout", ["reference", "slab"])
+def test_lanczos_local_reorthogonalisation(layout):
+    """the DGKS pass of f against v_N (ed_lanczos_cfg.reorth) on both layouts: E0 at the 1e-10 bar
+    against the oracle, a unit Ritz vector with a small residual, and the recurrence it changes is
+    still a Lanczos recurrence (the same E0 as without the pass, to 1e-10)"""
+    print(f"Lanczos with reorth, kagome12 6/6 U=4, {layout} layout", flush=True)
+    H, (N, qu, qd, ou, od, U, uv) = _build("kagome12_6_6")
+    if layout == "slab":
+        H.slab_layout(True)
+    f = torch.empty(H.dim, dtype=torch.float64, device=DEV)
+    ev = torch.empty_like(f)
+    lz = Lanczos(H.dimUP, H.dimDN, reorth=True)
+    lz.random_vector(f)
+    e_re = lz.eigen(H, f, ev)
+    lz0 = Lanczos(H.dimUP, H.dimDN)
+    lz0.random_vector(f)
+    e_plain = lz0.eigen(H, f, None)
+    if layout == "slab":
+        H.slab_layout(False)
+    st, e0o, *_ = O.Hubbard(O.Species(N, qu, ou), O.Species(N, qd, od), U).lanczos_eigen(O.random_vector(H.dim))
+    assert abs(e_re - e0o) <= 1e-10 * abs(e0o)
+    assert abs(e_re - e_plain) <= 1e-10 * abs(e0o)
+    assert abs(float(torch.linalg.norm(ev)) - 1.0) <= 1e-12
+    res = float(torch.linalg.norm(H.apply(ev) - e_re * ev))
+    assert res <= 1e-6 * abs(e_re), res
