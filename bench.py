@@ -99,6 +99,33 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
             "S_vN": float(s[0]), "entropy_s": round(t2 - t1, 2)}
 
 
+def entropy_only_line(N=18, Q=9, seed=5):
+    """S_vN(N/2 | N/2) of a seeded random normalized state of the N-site, Q/Q sector: the rho_A
+    blocks (FP64 MFMA Gram) and their eigenvalues alone, to profile them without a ground state.
+    The block sizes, and so the Gram FLOPs (2.789e13 at 18 sites), do not depend on the state."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
+    torch.cuda.set_device(0)
+    hq = H_TBq(N, Q)
+    D = hq.count * hq.count
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    psi = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+    psi /= torch.linalg.norm(psi)
+    s = np.zeros(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(psi), s.ctypes.data_as(C.POINTER(C.c_double)),
+         stream_ptr())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"[bench] S_vN({N // 2}|{N - N // 2}) of a random state {s[0]:.15f} in {t1 - t0:.1f} s")
+    return {"S_vN": float(s[0]), "entropy_s": round(t1 - t0, 2), "dim": D,
+            "workload": f"random_state_N{N}_nup{Q}_ndn{Q}_cut{N // 2}"}
+
+
 def batched_drude_line():
     """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
     weight per sector; batched (ed_batch_lowest, hipGraph) vs the plain per-instance loop on the
@@ -191,6 +218,8 @@ def main():
                          "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")
     ap.add_argument("--reorth", action="store_true",
                     help="with --ground-state: a DGKS pass of each new Lanczos vector against v_N")
+    ap.add_argument("--entropy-only", action="store_true",
+                    help="only S_vN(9|9) of a seeded random 18-site 9/9 state (rho_A Gram + block eigenvalues)")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -204,6 +233,9 @@ def main():
 
     if args.drude_only:
         print(json.dumps({"batched_drude": batched_drude_line()}), flush=True)
+        return
+    if args.entropy_only:
+        print(json.dumps({"entropy": entropy_only_line()}), flush=True)
         return
 
     import torch
