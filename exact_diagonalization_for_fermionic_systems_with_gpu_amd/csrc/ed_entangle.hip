@@ -76,8 +76,8 @@ __global__ __launch_bounds__(256) void k_gather_blocks(int64_t dim_up, int64_t d
 // ---------------------------------------------------------------------------------
 // 2. rho = A A^T (A A^H), A row-major k x K (lda = K), rho column-major k x k (ld = k),
 //    upper-triangle tiles only.  256 threads = 4 waves, 64x64 tile per workgroup, each
-//    wave a 32x32 quadrant = 2x2 MFMA 16x16 tiles; K staged through LDS 16 at a time, the
-//    next 16 loaded into registers while the MFMAs of the current 16 run.  (128x128 tiles,
+//    wave a 32x32 quadrant = 2x2 MFMA 16x16 tiles; K staged through two LDS stages 16 at a
+//    time, the next 16 loaded into registers while the MFMAs of the current 16 run.  (128x128 tiles,
 //    at one wave per SIMD, measured no faster at 18 sites: 0.623 against 0.618 s.)
 //    MFMA f64 16x16x4 operand layout: A lane l -> (row l%16, k l/16); B lane l ->
 //    (col l%16, k l/16); D lane l -> (rows l/16 + 4r, col l%16), r = 0..3 (checked on
@@ -103,8 +103,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
     const int64_t tj = b;
     const int64_t i0 = ti * TT, j0 = tj * TT;
     constexpr int NC = CPLX ? 2 : 1;
-    __shared__ double sa[NC][TT][TK + 1];
-    __shared__ double sb[NC][TT][TK + 1];
+    __shared__ double sa[2][NC][TT][TK + 1]; // two stages: one is read by the MFMAs while the
+    __shared__ double sb[2][NC][TT][TK + 1]; // next chunk is stored into the other
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int NP = TT / 32;
     const int wi = (wave >> 1) * (TT / 2), wj = (wave & 1) * (TT / 2);
@@ -132,32 +132,37 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
             }
         }
     };
-    fetch(0);
-    for (int64_t k0 = 0; k0 < K; k0 += TK) {
+    auto stash = [&](int st) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * 256, r = e / TK, c = e % TK;
 #pragma unroll
             for (int z = 0; z < NC; ++z) {
-                sa[z][r][c] = ra[q][z];
-                sb[z][r][c] = rb[q][z];
+                sa[st][z][r][c] = ra[q][z];
+                sb[st][z][r][c] = rb[q][z];
             }
         }
-        __syncthreads();
-        if (k0 + TK < K) fetch(k0 + TK);
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    int cur = 0;
+    for (int64_t k0 = 0; k0 < K; k0 += TK) {
+        const bool more = k0 + TK < K;
+        if (more) fetch(k0 + TK);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
             const int kl = kk + (lane >> 4), rl = lane & 15;
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                const double ar = sa[0][wi + p * 16 + rl][kl];
-                const double ai = CPLX ? sa[NC - 1][wi + p * 16 + rl][kl] : 0.0;
+                const double ar = sa[cur][0][wi + p * 16 + rl][kl];
+                const double ai = CPLX ? sa[cur][NC - 1][wi + p * 16 + rl][kl] : 0.0;
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
-                    const double br = sb[0][wj + q * 16 + rl][kl];
+                    const double br = sb[cur][0][wj + q * 16 + rl][kl];
                     acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc[p][q], 0, 0, 0);
                     if (CPLX) {
-                        const double bi = sb[NC - 1][wj + q * 16 + rl][kl];
+                        const double bi = sb[cur][NC - 1][wj + q * 16 + rl][kl];
                         acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc[p][q], 0, 0, 0);
                         acci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acci[p][q], 0, 0, 0);
                         acci[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, bi, acci[p][q], 0, 0, 0);
@@ -165,7 +170,10 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const d
                 }
             }
         }
+        // one barrier per chunk: the stage just read is overwritten only after the next barrier
+        if (more) stash(cur ^ 1);
         __syncthreads();
+        cur ^= 1;
     }
     // store: rho(i, j) column-major at i + j*k
 #pragma unroll
