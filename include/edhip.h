@@ -328,7 +328,10 @@ int ed_continued_fraction(int n, const double *a, const double *b, double energy
  * each block rho_A = Psi Psi^T (Psi^H) is formed with FP64 MFMA and diagonalised
  * (values only).  out[0] = S_vN = -sum lambda ln lambda; out[1] = S_ref = -sum sigma ln sigma
  * with sigma from an SVD of each block (the reference's formula).  nsv_out (may be NULL):
- * number of nonzero Schmidt values.  Synchronises the stream. */
+ * number of nonzero Schmidt values.  Synchronises the stream.  When the largest block has 2048
+ * rows or more (ED_EE_PARALLEL_MIN), the blocks are solved by four worker threads on streams of
+ * their own, after the stream's earlier work; the results are those of the one-stream solve bit
+ * for bit.  The same holds for the two calls below. */
 int ed_entanglement_entropy(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                             double *out, int64_t *nsv_out, void *stream);
 /* S_vN only (out[0] of ed_entanglement_entropy without the per-block SVDs, which dominate the
