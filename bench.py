@@ -144,11 +144,15 @@ def batched_drude_line():
     E0b, Db, tb, _ = min(runs_b, key=lambda r: r[2])
     E0p, Dp, tp, _ = min(runs_p, key=lambda r: r[2])
     nz = np.abs(Dg) > 1e-6
+    # every one of the 208 instances: batched vs plain E0, and E0 even in the flux
+    e0_diff = float(max(np.max(np.abs(rb[0] - rp[0])) for rb in runs_b for rp in runs_p))
+    e0_asym = float(max(np.max(np.abs(r[0][:, 1:] - r[0][:, :0:-1])) for r in runs_b + runs_p))
     err = float(max(max(np.max(np.abs(r[1][nz] - Dg[nz]) / np.abs(Dg[nz])) for r in runs_b + runs_p), 0.0))
     log(f"[bench] batched Drude sweep {tb:.3f} s (runs {[round(r[2], 3) for r in runs_b]}), plain {tp:.3f} s "
         f"(runs {[round(r[2], 3) for r in runs_p]}), max rel err vs oracle {err:.2e}")
     return {"value": round(13 / tb, 3), "unit": "Drude weights/s", "plain": round(13 / tp, 3),
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
+            "max_abs_e0_batched_minus_plain": e0_diff, "max_abs_e0_flux_asymmetry": e0_asym,
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
@@ -309,6 +313,20 @@ def main():
     gbs = alg_bytes / (mean_ms * 1e-3) / 1e9
     log(f"[bench] H.v on the slab layout: mean {mean_ms:.3f} ms, median {median_ms:.3f} ms, "
         f"wall/step {wall / args.steps * 1e3:.3f} ms")
+    # gather ceiling: each pass's line gathers alone (same tiles, tile order and entry tables, no
+    # arithmetic, no streams); pass 1 reads the transposed copy the last H.v left in the scratch
+    ceil_ms = []
+    for p in (0, 1):
+        H.slab_gather_ceiling(xl, p)
+        torch.cuda.synchronize()
+        for k in range(args.steps):
+            starts[k].record()
+            H.slab_gather_ceiling(xl, p)
+            ends[k].record()
+        torch.cuda.synchronize()
+        ceil_ms.append(sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps)
+    log(f"[bench] gather ceiling: UP {ceil_ms[0]:.3f} ms, DN {ceil_ms[1]:.3f} ms; "
+        f"H.v / ceiling = {mean_ms / sum(ceil_ms):.3f}")
     conv = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     conv[0].record()
     H.to_slab(x, xl)
@@ -345,6 +363,9 @@ def main():
                    "hv_reference_layout_mean_ms": round(rm_mean_ms, 4),
                    "hv_reference_layout_gbs": round(alg_bytes / (rm_mean_ms * 1e-3) / 1e9, 2),
                    "reference_to_slab_conversion_ms": round(conv_ms, 3),
+                   "gather_ceiling_ms": {"up": round(ceil_ms[0], 4), "dn": round(ceil_ms[1], 4),
+                                         "total": round(sum(ceil_ms), 4)},
+                   "hv_over_gather_ceiling": round(mean_ms / sum(ceil_ms), 4),
                    "hv_tiles": {k: v for k, v in tuning.items() if k != "ms_per_apply"}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -63,6 +63,7 @@ SIGNATURES = {
     "ed_slab_from_rowmajor": (_i32, [_p, _p, _p, _p]),
     "ed_slab_to_rowmajor": (_i32, [_p, _p, _p, _p]),
     "ed_hubbard_apply_slab": (_i32, [_p, _p, _p, _p]),
+    "ed_hubbard_slab_gather_ceiling": (_i32, [_p, _i32, _p, _p]),
     "ed_dense_zero": (_i32, [_i64, _i32, _p, _p]),
     "ed_hubbard_write_dense": (_i32, [_p, _p, _p]),
     "ed_tables_write_dense": (_i32, [_i64, _i32, _i32, _p, _p, _p, _p]),

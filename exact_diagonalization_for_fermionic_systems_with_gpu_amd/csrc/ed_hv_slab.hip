@@ -305,6 +305,48 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
     }
 }
 
+// Gather ceiling: the line gathers of one pass alone.  Same grid, tile order, LDS footprint,
+// entry staging and per-lane gather batches as k_slab_up (UP entries over layout L) or k_slab_dn
+// (DN entries over layout T), but no value table, no FMAs and no stream loads or stores: the
+// loaded amplitudes are only summed, and the sum is stored under a condition that real data
+// never meets, so that the loads stay live.  Its time is what the gathers of that pass cost.
+template <int NB>
+__global__ __launch_bounds__(256) void k_slab_ceiling(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
+                                                      int64_t drows, const double *__restrict__ x,
+                                                      double *__restrict__ sink, int64_t nsink)
+{
+    __shared__ double s_val[4 * kMaxOps];
+    __shared__ uint32_t s_ent[kTR * kMaxOps];
+    __shared__ int s_off[kTR + 1];
+    __shared__ double s_a[kTR][kSW + 1];
+    __shared__ double s_b[kTR][kSW + 1];
+    const int64_t b = blockIdx.x;
+    const int64_t t = slab_tile(b, per_xcd, a.swz);
+    if (t >= ntiles) return; // uniform over the workgroup
+    const int64_t slab = t / ntr, p0 = (t - slab * ntr) * kTR;
+    const int64_t pend = (p0 + kTR < drows) ? p0 + kTR : drows;
+    const int nrows = slab_stage(a, p0, pend, s_val, s_ent, s_off);
+    __syncthreads();
+    const int tid = threadIdx.x, r = tid >> 3, c = (tid & 7) * 2;
+    const double *xs = x + slab * drows * kSW;
+    double acc = 0.0;
+    if (r < nrows) {
+        const int o0 = s_off[r], o1 = s_off[r + 1];
+        for (int j0 = o0; j0 < o1; j0 += NB) {
+            double2 xv[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k)
+                if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(xs + (int64_t)(s_ent[j0 + k] >> 8) * kSW + c);
+#pragma unroll
+            for (int k = 0; k < NB; ++k)
+                if (j0 + k < o1) acc += xv[k].x + xv[k].y;
+        }
+    }
+    s_a[r][c] = acc;      // the LDS arrays of the real passes are declared (same occupancy) and
+    s_b[r][c] = s_val[0]; // touched so that the compiler keeps them
+    if (acc == -1.0e300 && t < nsink) sink[t] = s_a[r][c] + s_b[r][c];
+}
+
 // fixed-order sums of 2048 consecutive partials (first level of the <v, y> reduction)
 __global__ __launch_bounds__(256) void k_slab_chunks(int64_t n, const double *__restrict__ in, double *__restrict__ out)
 {
@@ -640,6 +682,25 @@ extern "C" int ed_slab_to_rowmajor(ed_hubbard hh, const double *d_xl, double *d_
     const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
     if (!h || !h->slab || !d_x || !d_xl) return fail(ED_EINVAL, "ed_slab_to_rowmajor: no slab layout or null buffer");
     return slab_to_rowmajor(*h, d_xl, d_x, stream_of(stream));
+}
+
+extern "C" int ed_hubbard_slab_gather_ceiling(ed_hubbard hh, int pass, const double *d_xl, void *stream)
+{
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    if (!h || !h->slab || !d_xl || (pass != 0 && pass != 1))
+        return fail(ED_EINVAL, "ed_hubbard_slab_gather_ceiling: no slab layout, null buffer or pass not 0/1");
+    const SlabLayout &L = *h->slab;
+    SlabArgs a = slab_args(*h, pass == 0);
+    a.swz = 8;
+    const int64_t rows = pass == 0 ? L.du : L.dd, slabs = pass == 0 ? L.nsd : L.nsu;
+    const int64_t ntr = (rows + kTR - 1) / kTR, tiles = slabs * ntr, per = (tiles + 7) / 8;
+    if (per * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab gather ceiling: grid too large");
+    // pass 1 gathers over the layout-T scratch, as k_slab_dn does
+    const double *src = pass == 0 ? d_xl : L.xt;
+    hipLaunchKernelGGL(k_slab_ceiling<8>, dim3((unsigned)(per * 8)), dim3(256), 0, stream_of(stream), a, tiles, per,
+                       ntr, rows, src, L.part, L.npart);
+    ED_CHECK_LAUNCH();
+    return ED_OK;
 }
 
 extern "C" int ed_hubbard_apply_slab(ed_hubbard hh, const double *d_xl, double *d_yl, void *stream)

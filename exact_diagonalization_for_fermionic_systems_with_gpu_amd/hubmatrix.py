@@ -120,27 +120,52 @@ class Hubbard:
         self.slab_len = n.value
         return self.slab_len
 
+    def _slab_vec(self, *ts):
+        """slab vectors: contiguous float64 CUDA tensors of exactly the handle's layout length"""
+        n = self.slab_len_query()
+        if n == 0:
+            raise ValueError("the operator has no slab layout (slab_layout(True) builds it)")
+        for t in ts:
+            if t.numel() != n or t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError(f"slab vectors are contiguous float64 CUDA tensors of {n} elements")
+        return n
+
+    def _rowmajor_real(self, t):
+        self._check(t)
+        if t.dtype != torch.float64:
+            raise ValueError("the slab layout holds real (float64) vectors")
+
     def to_slab(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        self._check(x)
+        """reference layout -> slab layout (padding zero); asynchronous"""
+        self._rowmajor_real(x)
         if out is None:
-            out = torch.empty(self.slab_len, dtype=torch.float64, device=x.device)
+            out = torch.empty(self.slab_len_query(), dtype=torch.float64, device=x.device)
+        self._slab_vec(out)
         call("ed_slab_from_rowmajor", self._h, _raw(x), _raw(out), stream_ptr())
         return out
 
     def from_slab(self, xl: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """slab layout -> reference layout; asynchronous"""
+        self._slab_vec(xl)
         if out is None:
             out = torch.empty(self.dim, dtype=torch.float64, device=xl.device)
-        self._check(out)
+        self._rowmajor_real(out)
         call("ed_slab_to_rowmajor", self._h, _raw(xl), _raw(out), stream_ptr())
         return out
 
     def apply_slab(self, xl: torch.Tensor, yl: torch.Tensor):
         """yl = H xl on the slab layout (asynchronous)"""
-        n = getattr(self, "slab_len", 0)
-        for t in (xl, yl):
-            if n == 0 or t.numel() != n or t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
-                raise ValueError("slab vectors are contiguous float64 CUDA tensors of slab_len elements")
+        self._slab_vec(xl, yl)
+        if xl.data_ptr() == yl.data_ptr():
+            raise ValueError("apply_slab: x and y must be different buffers")
         call("ed_hubbard_apply_slab", self._h, _raw(xl), _raw(yl), stream_ptr())
+
+    def slab_gather_ceiling(self, xl: torch.Tensor, pass_: int):
+        """the line gathers of one slab H.v pass alone (0: UP over xl, 1: DN over the layout's
+        transposed scratch), with the kernels' tiles and tile order but no arithmetic or streams:
+        a measurement probe of the gather cost; asynchronous, writes nothing the caller sees"""
+        self._slab_vec(xl)
+        call("ed_hubbard_slab_gather_ceiling", self._h, int(pass_), _raw(xl), stream_ptr())
 
     def Write(self, Hmat: torch.Tensor):
         """Hubbard::Write (HubMatrix_GPU.cuh:219-230): ADDS H into the dense D x D matrix Hmat

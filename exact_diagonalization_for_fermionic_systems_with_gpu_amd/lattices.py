@@ -205,18 +205,27 @@ def current_ops(cl: Cluster, r=None):
     return jx, jy
 
 
-def lieb_twisted_ops(phi, t=1.0, rings=LIEB12_X_RINGS):
+def lieb_twisted_ops(phi, t=1.0, rings=LIEB12_X_RINGS, spread=True):
     """Lieb12 with a Peierls twist: total phase phi around every x ring, spread as phi/4 on
     each of its 4 bonds (direction first->second of the bond list); y bonds real.
     Hopping matrix element -t (Make_H(i, j, -1.0) in TotalSpinExcited_Lieb_dilute.cu:155).
-    phi = pi is gauge-equivalent to the reference's sign-flipped x bonds
-    (TotalSpinExcited_Lieb_randomHop.cu:184)."""
+    spread=False puts the whole phase on the last bond of each ring (bonds 3 and 7, the ones
+    the reference flips), a boundary twist; both are gauge-equivalent, and phi = pi gives the
+    reference's sign-flipped x bonds (TotalSpinExcited_Lieb_randomHop.cu:176-188)."""
     xset = {k: len(r) for r in rings for k in r}
+    last = {r[-1] for r in rings}
     ops = []
     for k, (i, j) in enumerate(LIEB12.bonds):
         if k in xset:
             # hop i -> j along the ring picks up e^{+i phi/L}: term c+_j c_i
-            ops.append((OP_HOP_PHASE, j, i, -t, phi / xset[k]))
+            ph = phi / xset[k] if spread else (phi if k in last else 0.0)
+            ops.append((OP_HOP_PHASE, j, i, -t, ph))
         else:
             ops.append((OP_HOP, i, j, -t))
     return ops
+
+
+def lieb_pi_flipped_ops(t=1.0):
+    """LiebLattice12 of TotalSpinExcited_Lieb_randomHop.cu:176-188 with r_i = 1: Make_H(i, j, -t)
+    on every bond except x bonds 3 and 7, which get +t (a pi flux through each x ring)."""
+    return [(OP_HOP, i, j, t if k in LIEB12_PI_X_BONDS else -t) for k, (i, j) in enumerate(LIEB12.bonds)]

@@ -189,6 +189,12 @@ int ed_slab_from_rowmajor(ed_hubbard h, const double *d_x, double *d_xl, void *s
 int ed_slab_to_rowmajor(ed_hubbard h, const double *d_xl, double *d_x, void *stream);
 /* y = H x on the slab layout (x != y, x padding zero; y padding is written zero).  Asynchronous. */
 int ed_hubbard_apply_slab(ed_hubbard h, const double *d_xl, double *d_yl, void *stream);
+/* Measurement probe (no counterpart in the reference): the line gathers of one pass of
+ * ed_hubbard_apply_slab alone -- pass 0 the UP gathers over d_xl, pass 1 the DN gathers over the
+ * layout's transposed scratch -- with the same grid, tile order, LDS footprint and entry tables,
+ * but without the value table, the FMAs and the stream loads and stores.  Its time is the cost
+ * of that pass's gathers.  Asynchronous; writes nothing the caller sees. */
+int ed_hubbard_slab_gather_ceiling(ed_hubbard h, int pass, const double *d_xl, void *stream);
 
 /* ---------------- vector BLAS-1 (Thrust_Lin, HubMatrix_GPU.cuh:291-471) -------- */
 

@@ -54,3 +54,30 @@ def test_graph_replay_equals_eager():
     _, _, ev_e, ns_e, _ = _sweep_subset(False)
     assert np.array_equal(ns_g, ns_e)
     assert np.array_equal(ev_g.view(np.uint64), ev_e.view(np.uint64))
+
+
+def test_full_sweep_all_instances():
+    """All 208 instances of the headline sweep: batched E0 equals the plain per-instance E0, and
+    E0 is even in the flux, E0(Phi_k) = E0(Phi_{16-k}), both to 1e-10 relative.  At Phi = pi the
+    twisted complex operator gives the E0 of the reference's real pi-flux cluster (x bonds 3 and 7
+    flipped, LiebLattice12 of TotalSpinExcited_Lieb_randomHop.cu:176-188) on the same GPU."""
+    print("batched lieb12 full sweep", flush=True)
+    E0b, Db, _, nsb = BT.lieb12_drude_sweep(batched=True)
+    E0p, Dp, _, nsp = BT.lieb12_drude_sweep(batched=False)
+    scale = np.maximum(1.0, np.abs(E0p))
+    assert np.all(np.abs(E0b - E0p) <= 1e-10 * scale), np.max(np.abs(E0b - E0p))
+    for E in (E0b, E0p):
+        for k in range(1, 16):
+            assert np.all(np.abs(E[:, k] - E[:, 16 - k]) <= 1e-10 * scale[:, k]), k
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+    ops = L.lieb_pi_flipped_ops()
+    hs = []
+    for nu in (3, 5, 6):
+        hu, hd = H_TBq(12, nu), H_TBq(12, 12 - nu)
+        hu.add_ops(ops)
+        hd.add_ops(ops)
+        hs.append(Hubbard(hu, hd, 4.0))
+    epi, _ = BT.plain_lowest(hs, False)
+    for e, nu in zip(epi, (3, 5, 6)):
+        assert abs(E0b[nu, 8] - e) <= 1e-10 * max(1.0, abs(e)), (nu, E0b[nu, 8], e)

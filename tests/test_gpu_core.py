@@ -267,6 +267,21 @@ def test_s2_parity_and_singlet():
     y = to_np(S.Hubbard_mv_new(torch.from_numpy(x).to(DEV)))
     yo = O.s2_apply(O.Species(12, 5, ops), O.Species(12, 6, ops), x)
     assert np.max(np.abs(y - yo)) <= 1e-14 * np.max(np.abs(yo))
+    # ground states by the GPU Lanczos, total spin by the GPU S^2 kernel: the 12-electron
+    # ground state (U = 4) is a singlet, the 11-electron one a doublet, S(S+1) = S_z^2 + S_xy
+    for qu, ss_expect in ((6, 0.0), (5, 0.75)):
+        hu = gpu_species(12, qu, ops)
+        H = Hubbard(hu, gd, 4.0)
+        f = torch.empty(H.dim, dtype=torch.float64, device=DEV)
+        ev = torch.empty_like(f)
+        lz = Lanczos(hu.count, gd.count)
+        lz.random_vector(f)
+        lz.eigen(H, f, ev)
+        ev /= torch.linalg.vector_norm(ev)
+        S2 = TotalSpinHubbard(hu, gd)
+        sz = 0.5 * (qu - 6)
+        ss = sz * sz + float(torch.dot(ev, S2.Hubbard_mv_new(ev)).item())
+        assert abs(ss - ss_expect) <= 1e-8, (qu, ss)
 
 
 @pytest.mark.parametrize("order", ["0", "1"])
