@@ -295,6 +295,8 @@ def main():
     yl = torch.empty_like(xl)
     torch.cuda.synchronize()
     slab_setup_s = time.perf_counter() - t0
+    slab_tuning = H.slab_autotune(xl, yl, reps=2)
+    log(f"[bench] slab autotune: {slab_tuning}")
     for _ in range(max(1, args.warmup)):  # at least one, for the bitwise check below
         H.apply_slab(xl, yl)
     same = bool(torch.equal(H.from_slab(yl), y))
@@ -366,7 +368,8 @@ def main():
                    "gather_ceiling_ms": {"up": round(ceil_ms[0], 4), "dn": round(ceil_ms[1], 4),
                                          "total": round(sum(ceil_ms), 4)},
                    "hv_over_gather_ceiling": round(mean_ms / sum(ceil_ms), 4),
-                   "hv_tiles": {k: v for k, v in tuning.items() if k != "ms_per_apply"}},
+                   "hv_tiles": {k: v for k, v in tuning.items() if k != "ms_per_apply"},
+                   "slab_tiles": {k: v for k, v in slab_tuning.items() if k != "ms_per_apply"}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": None,

@@ -37,6 +37,11 @@ class HvTuning(C.Structure):
                 ("row_order", C.c_int), ("up_table", C.c_int), ("ms_per_apply", C.c_double)]
 
 
+class SlabTuning(C.Structure):
+    _fields_ = [("tile_rows", C.c_int), ("gathers_in_flight", C.c_int), ("tile_order", C.c_int),
+                ("ms_per_apply", C.c_double), ("candidate_ms", C.c_double * 8)]
+
+
 class EdError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"libedhip status {status}: {msg}")
@@ -64,6 +69,9 @@ SIGNATURES = {
     "ed_slab_to_rowmajor": (_i32, [_p, _p, _p, _p]),
     "ed_hubbard_apply_slab": (_i32, [_p, _p, _p, _p]),
     "ed_hubbard_slab_gather_ceiling": (_i32, [_p, _i32, _p, _p]),
+    "ed_hubbard_slab_set_shape": (_i32, [_p, _i32, _i32, _i32]),
+    "ed_slab_auto_decision": (_i32, [C.c_double, C.c_double, _i64, _i64, _i32]),
+    "ed_hubbard_slab_autotune": (_i32, [_p, _p, _p, _i32, C.POINTER(SlabTuning), _p]),
     "ed_dense_zero": (_i32, [_i64, _i32, _p, _p]),
     "ed_hubbard_write_dense": (_i32, [_p, _p, _p]),
     "ed_tables_write_dense": (_i32, [_i64, _i32, _i32, _p, _p, _p, _p]),

@@ -131,7 +131,8 @@ int slab_lanczos_ritz(const Hubbard &h, const double *f, double *y, double *v, c
 int64_t slab_num_partials(const Hubbard &h);
 int slab_from_rowmajor(const Hubbard &h, const double *x, double *xl, hipStream_t st);
 int slab_to_rowmajor(const Hubbard &h, const double *xl, double *x, hipStream_t st);
-void slab_auto(Hubbard &h);
+// builds the slab layout for a large real sector when it and nvec more layout vectors fit
+void slab_auto(Hubbard &h, int nvec);
 
 // deterministic reductions and BLAS-1 (ed_blas.hip)
 int finalize_sum(const double *partial, int64_t n, double *dst, int mode, hipStream_t st);

@@ -17,12 +17,13 @@
 //   layout T (scratch): the same with the roles exchanged, slabs of 16 up positions:
 //         ((pu >> 4) * dim_dn + pd) * 16 + (pu & 15).
 //
-// Pass A (UP): tiles of 32 up positions x one slab, slab-major, slabs split over the 8 XCDs.
+// Pass A (UP): tiles of TR (32 or 64, the autotuner's choice) up positions x one slab, slab-major,
+//   slabs split over the 8 XCDs.
 //   A lane owns 2 adjacent columns (16 B), 8 lanes a row, so one wavefront load fetches 8 full
 //   128 B lines (the j-th neighbour row of 8 rows).  It writes the UP partial in layout L and
 //   the tile's own (scaled) amplitudes, transposed through LDS, in layout T.
 // Pass B (DN + diagonal + Lanczos epilogue): the DN term is an UP-type pass over layout T: tiles
-//   of 32 dn positions x one slab of 16 up positions.  It reads the UP partial (and the Lanczos
+//   of TR dn positions x one slab of 16 up positions.  It reads the UP partial (and the Lanczos
 //   w) tile in layout L transposed through LDS, adds the DN gathers and the interaction, and
 //   writes y (and v) back in layout L.
 // Gathers are 16 B per lane and hit L2 instead of MALL/HBM with TLB misses, at the price of one
@@ -38,7 +39,8 @@
 namespace ed {
 
 constexpr int kSW = 16;     // positions per slab: one 128 B line of a row
-constexpr int kTR = 32;     // rows per tile: 256 threads = 32 rows x 8 lanes x 2 columns
+// tile rows TR (32 or 64; 8 lanes x 2 columns per row, so 8 TR threads) and gathers in flight
+// per lane NB (8 or 16) are template parameters chosen by the slab autotuner
 constexpr int kMaxOps = 64; // calls per species (the on-the-fly limit)
 
 struct SlabLayout {
@@ -52,7 +54,8 @@ struct SlabLayout {
     double *val_u = nullptr, *val_d = nullptr;  // value code -> matrix element (4 per call)
     double *xt = nullptr;                       // layout-T scratch
     double *part = nullptr;                     // pass-B block partials of <v, y>
-    int64_t npart = 0;
+    int64_t npart = 0;                          // pass-B tiles at the smallest tile (32 rows)
+    int tr = 32, nb = 8, swz = 8;               // launch shape: tile rows, gathers in flight, tile order
 };
 
 void slab_free(SlabLayout *s)
@@ -124,16 +127,18 @@ __device__ __forceinline__ double2 ld2(const double *p)
 }
 
 // LDS staging shared by both passes: value table and the CSR entries of rows [p0, pend)
+template <int TR>
 __device__ __forceinline__ int slab_stage(const SlabArgs &a, int64_t p0, int64_t pend, double *s_val,
                                           uint32_t *s_ent, int *s_off)
 {
+    constexpr int NT = 8 * TR;
     const int tid = threadIdx.x;
-    for (int q = tid; q < 4 * a.nops; q += 256) s_val[q] = a.val[q];
+    for (int q = tid; q < 4 * a.nops; q += NT) s_val[q] = a.val[q];
     const int nrows = (int)(pend - p0);
     const int64_t e0 = a.ptr[p0];
-    if (tid <= kTR) s_off[tid] = (int)(a.ptr[tid <= nrows ? p0 + tid : pend] - e0);
+    if (tid <= TR) s_off[tid] = (int)(a.ptr[tid <= nrows ? p0 + tid : pend] - e0);
     const int ne = (int)(a.ptr[pend] - e0);
-    for (int q = tid; q < ne; q += 256) s_ent[q] = a.ent[e0 + q];
+    for (int q = tid; q < ne; q += NT) s_ent[q] = a.ent[e0 + q];
     return nrows;
 }
 
@@ -165,21 +170,21 @@ __device__ __forceinline__ void slab_gather(const double *__restrict__ base, con
 }
 
 // Pass A: z = UP term of x (layout L); xt = sc*x (layout T)
-template <bool SCALE, int NB>
-__global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
+template <bool SCALE, int NB, int TR>
+__global__ __launch_bounds__(8 * TR) void k_slab_up(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
                                                  const double *__restrict__ x, double *__restrict__ z,
                                                  double *__restrict__ xt, const double *__restrict__ bdev, int step)
 {
     __shared__ double s_val[4 * kMaxOps];
-    __shared__ uint32_t s_ent[kTR * kMaxOps];
-    __shared__ int s_off[kTR + 1];
-    __shared__ double s_x[kTR][kSW + 1];
+    __shared__ uint32_t s_ent[TR * kMaxOps];
+    __shared__ int s_off[TR + 1];
+    __shared__ double s_x[TR][kSW + 1];
     const int64_t b = blockIdx.x;
     const int64_t t = slab_tile(b, per_xcd, a.swz);
     if (t >= ntiles) return; // uniform over the workgroup
-    const int64_t slab = t / ntr, p0 = (t - slab * ntr) * kTR;
-    const int64_t pend = (p0 + kTR < a.du) ? p0 + kTR : a.du;
-    const int nrows = slab_stage(a, p0, pend, s_val, s_ent, s_off);
+    const int64_t slab = t / ntr, p0 = (t - slab * ntr) * TR;
+    const int64_t pend = (p0 + TR < a.du) ? p0 + TR : a.du;
+    const int nrows = slab_stage<TR>(a, p0, pend, s_val, s_ent, s_off);
     __syncthreads();
     const double sc = SCALE ? 1.0 / bdev[step] : 1.0;
     const int tid = threadIdx.x, r = tid >> 3, c = (tid & 7) * 2;
@@ -214,25 +219,25 @@ __global__ __launch_bounds__(256) void k_slab_up(SlabArgs a, int64_t ntiles, int
 enum { SLAB_RITZ = 16 };
 
 // Pass B: y = zin + DN term of xt + diag * xt (+ c w); v = xt (layout L); block partial <v, y>
-template <int FLAGS, int NB>
-__global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
+template <int FLAGS, int NB, int TR>
+__global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
                                                  const double *__restrict__ xt, const double *zin, const double *win,
                                                  double *yout, double *vout, const double *__restrict__ bdev, int step,
                                                  double *__restrict__ partial, double *__restrict__ evec,
                                                  const double *__restrict__ adev, double zr)
 {
     __shared__ double s_val[4 * kMaxOps];
-    __shared__ uint32_t s_ent[kTR * kMaxOps];
-    __shared__ int s_off[kTR + 1];
-    __shared__ double s_a[kTR][kSW + 1];
-    __shared__ double s_b[kTR][kSW + 1];
-    __shared__ double s_red[4];
+    __shared__ uint32_t s_ent[TR * kMaxOps];
+    __shared__ int s_off[TR + 1];
+    __shared__ double s_a[TR][kSW + 1];
+    __shared__ double s_b[TR][kSW + 1];
+    __shared__ double s_red[TR / 8];
     const int64_t b = blockIdx.x;
     const int64_t t = slab_tile(b, per_xcd, a.swz);
     if (t >= ntiles) return; // uniform over the workgroup
-    const int64_t su = t / ntr, p0 = (t - su * ntr) * kTR;
-    const int64_t pend = (p0 + kTR < a.dd) ? p0 + kTR : a.dd;
-    const int nrows = slab_stage(a, p0, pend, s_val, s_ent, s_off);
+    const int64_t su = t / ntr, p0 = (t - su * ntr) * TR;
+    const int64_t pend = (p0 + TR < a.dd) ? p0 + TR : a.dd;
+    const int nrows = slab_stage<TR>(a, p0, pend, s_val, s_ent, s_off);
     const int tid = threadIdx.x;
     // the tile's UP partial (and w) in layout L: 2 dn slabs x 16 up positions, 32 lines
     const int ds = tid >> 7, rem = tid & 127, pr = rem >> 3, k = (rem & 7) * 2;
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
         if (FLAGS & HV_WRITEV) st2(vout + li, v0, v1);
     }
     if (FLAGS & HV_DOT) {
-        const double sum = block_sum<256>(dot, s_red);
+        const double sum = block_sum<8 * TR>(dot, s_red);
         if (tid == 0) partial[t] = sum;
     }
 }
@@ -310,22 +315,22 @@ __global__ __launch_bounds__(256) void k_slab_dn(SlabArgs a, int64_t ntiles, int
 // (DN entries over layout T), but no value table, no FMAs and no stream loads or stores: the
 // loaded amplitudes are only summed, and the sum is stored under a condition that real data
 // never meets, so that the loads stay live.  Its time is what the gathers of that pass cost.
-template <int NB>
-__global__ __launch_bounds__(256) void k_slab_ceiling(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
-                                                      int64_t drows, const double *__restrict__ x,
-                                                      double *__restrict__ sink, int64_t nsink)
+template <int NB, int TR>
+__global__ __launch_bounds__(8 * TR) void k_slab_ceiling(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
+                                                         int64_t drows, const double *__restrict__ x,
+                                                         double *__restrict__ sink, int64_t nsink)
 {
     __shared__ double s_val[4 * kMaxOps];
-    __shared__ uint32_t s_ent[kTR * kMaxOps];
-    __shared__ int s_off[kTR + 1];
-    __shared__ double s_a[kTR][kSW + 1];
-    __shared__ double s_b[kTR][kSW + 1];
+    __shared__ uint32_t s_ent[TR * kMaxOps];
+    __shared__ int s_off[TR + 1];
+    __shared__ double s_a[TR][kSW + 1];
+    __shared__ double s_b[TR][kSW + 1];
     const int64_t b = blockIdx.x;
     const int64_t t = slab_tile(b, per_xcd, a.swz);
     if (t >= ntiles) return; // uniform over the workgroup
-    const int64_t slab = t / ntr, p0 = (t - slab * ntr) * kTR;
-    const int64_t pend = (p0 + kTR < drows) ? p0 + kTR : drows;
-    const int nrows = slab_stage(a, p0, pend, s_val, s_ent, s_off);
+    const int64_t slab = t / ntr, p0 = (t - slab * ntr) * TR;
+    const int64_t pend = (p0 + TR < drows) ? p0 + TR : drows;
+    const int nrows = slab_stage<TR>(a, p0, pend, s_val, s_ent, s_off);
     __syncthreads();
     const int tid = threadIdx.x, r = tid >> 3, c = (tid & 7) * 2;
     const double *xs = x + slab * drows * kSW;
@@ -401,74 +406,33 @@ static SlabArgs slab_args(const Hubbard &h, bool up)
     a.mask = h.otf.mask;
     a.Uvec = h.otf.Uvec;
     a.has_diag = h.otf.has_diag;
-    a.swz = 0;
+    a.swz = L.swz;
     return a;
 }
 
 int64_t slab_len(const Hubbard &h) { return h.slab ? h.slab->len : 0; }
 int64_t slab_num_partials(const Hubbard &h) { return h.slab ? (h.slab->npart + 2047) / 2048 : 0; }
 
-// one H.v on layout L; flags as hv_lanczos (HV_SCALE applies to pass A, which pre-scales xt)
-static int slab_apply(const Hubbard &h, const double *x, double *z, const double *w, double *y, double *v,
-                      const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks,
-                      double *evec = nullptr, const double *adev = nullptr, double zr = 0.0)
+template <int NB, int TR>
+static int slab_apply_t(const Hubbard &h, const double *x, double *z, const double *w, double *y, double *v,
+                        const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks,
+                        double *evec, const double *adev, double zr)
 {
     const SlabLayout &L = *h.slab;
-    // unused dynamic LDS per workgroup (bytes): fewer resident workgroups per CU, hence fewer
-    // rows in flight per XCD and a narrower L2 gather window (ED_SLAB_LDS_PAD, default none).
-    // Measured at 18 sites: every pad slows the passes (8, 16, 24, 36 KB: +10, +26, +47, +90 %),
-    // the loss of resident waves outweighs any L2 gain.
-    static const size_t pad = [] {
-        const char *e = getenv("ED_SLAB_LDS_PAD");
-        const long v = e ? atol(e) : 0;
-        return (size_t)(v > 0 && v <= 120000 ? v : 0);
-    }();
-    // tile order on the XCDs (slab_tile; ED_SLAB_SWZ = 0, 2, 4, 8, 16 or 32, default 8).  At 18
-    // sites, H.v took 54.9-56.1 ms in the plain order, 53.7 ms with 8 adjacent tiles per CU slot
-    // (2, 4, 16, 32: 54.9, 55.8, 54.2, 54.8 ms), on two boxes.
-    static const int swz = [] {
-        const char *e = getenv("ED_SLAB_SWZ");
-        const int v = e ? atoi(e) : 8;
-        return (v == 2 || v == 4 || v == 8 || v == 16 || v == 32) ? v : 0;
-    }();
     SlabArgs au = slab_args(h, true), ad = slab_args(h, false);
-    au.swz = ad.swz = swz;
-    const int64_t ntrA = (L.du + kTR - 1) / kTR, tilesA = L.nsd * ntrA, perA = (tilesA + 7) / 8;
-    const int64_t ntrB = (L.dd + kTR - 1) / kTR, tilesB = L.nsu * ntrB, perB = (tilesB + 7) / 8;
+    const int64_t ntrA = (L.du + TR - 1) / TR, tilesA = L.nsd * ntrA, perA = (tilesA + 7) / 8;
+    const int64_t ntrB = (L.dd + TR - 1) / TR, tilesB = L.nsu * ntrB, perB = (tilesB + 7) / 8;
     if (perA * 8 > 0x7fffffffLL || perB * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab H.v: grid too large");
-    // gathers in flight per lane (ED_SLAB_NB = 8, 16 or 32).  8 keeps the kernels at 52 VGPRs and
-    // 8 waves per SIMD; 16 and 32 cost occupancy (5 and 3 waves) and ran slower at 18 sites
-    // (56.2, 65.0 and 82.7 ms per H.v): the passes are bound by loads in flight per CU, which
-    // more resident waves raise more than deeper per-wave batches.
-    static const int nb = [] {
-        const char *e = getenv("ED_SLAB_NB");
-        const int v = e ? atoi(e) : 8;
-        return (v == 16 || v == 32) ? v : 8;
-    }();
-    const dim3 gA((unsigned)(perA * 8)), gB((unsigned)(perB * 8));
-#define ED_SLAB_A(S, N) \
-    hipLaunchKernelGGL((k_slab_up<S, N>), gA, dim3(256), pad, st, au, tilesA, perA, ntrA, x, z, L.xt, bdev, step)
-    if (flags & HV_SCALE) {
-        if (nb == 16) ED_SLAB_A(true, 16);
-        else if (nb == 32) ED_SLAB_A(true, 32);
-        else ED_SLAB_A(true, 8);
-    } else {
-        if (nb == 16) ED_SLAB_A(false, 16);
-        else if (nb == 32) ED_SLAB_A(false, 32);
-        else ED_SLAB_A(false, 8);
-    }
-#undef ED_SLAB_A
+    const dim3 gA((unsigned)(perA * 8)), gB((unsigned)(perB * 8)), blk(8 * TR);
+    if (flags & HV_SCALE)
+        hipLaunchKernelGGL((k_slab_up<true, NB, TR>), gA, blk, 0, st, au, tilesA, perA, ntrA, x, z, L.xt, bdev, step);
+    else
+        hipLaunchKernelGGL((k_slab_up<false, NB, TR>), gA, blk, 0, st, au, tilesA, perA, ntrA, x, z, L.xt, bdev, step);
     ED_CHECK_LAUNCH();
     const int fb = flags & ~HV_SCALE;
-#define ED_SLAB_B1(F, N)                                                                                            \
-    hipLaunchKernelGGL((k_slab_dn<F, N>), gB, dim3(256), pad, st, ad, tilesB, perB, ntrB, L.xt, z, w, y, v, bdev,   \
-                       step, L.part, evec, adev, zr)
 #define ED_SLAB_B(F)                                                                                               \
-    do {                                                                                                           \
-        if (nb == 16) ED_SLAB_B1(F, 16);                                                                           \
-        else if (nb == 32) ED_SLAB_B1(F, 32);                                                                      \
-        else ED_SLAB_B1(F, 8);                                                                                     \
-    } while (0)
+    hipLaunchKernelGGL((k_slab_dn<F, NB, TR>), gB, blk, 0, st, ad, tilesB, perB, ntrB, L.xt, z, w, y, v, bdev, step, \
+                       L.part, evec, adev, zr)
     switch (fb) {
     case 0: ED_SLAB_B(0); break;
     case HV_WRITEV: ED_SLAB_B(HV_WRITEV); break;
@@ -480,7 +444,6 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
     default: return fail(ED_EINVAL, "slab H.v: unsupported flags %d", flags);
     }
 #undef ED_SLAB_B
-#undef ED_SLAB_B1
     ED_CHECK_LAUNCH();
     if (fb & HV_DOT) {
         const int64_t nc = (tilesB + 2047) / 2048;
@@ -489,6 +452,19 @@ static int slab_apply(const Hubbard &h, const double *x, double *z, const double
         if (nblocks) *nblocks = nc;
     }
     return ED_OK;
+}
+
+// one H.v on layout L with the layout's launch shape; flags as hv_lanczos (HV_SCALE applies to
+// pass A, which pre-scales xt)
+static int slab_apply(const Hubbard &h, const double *x, double *z, const double *w, double *y, double *v,
+                      const double *bdev, int step, int flags, double *partial, hipStream_t st, int64_t *nblocks,
+                      double *evec = nullptr, const double *adev = nullptr, double zr = 0.0)
+{
+    const SlabLayout &L = *h.slab;
+#define ED_SLAB_T(NB, TR) slab_apply_t<NB, TR>(h, x, z, w, y, v, bdev, step, flags, partial, st, nblocks, evec, adev, zr)
+    if (L.tr == 64) return L.nb == 16 ? ED_SLAB_T(16, 64) : ED_SLAB_T(8, 64);
+    return L.nb == 16 ? ED_SLAB_T(16, 32) : ED_SLAB_T(8, 32);
+#undef ED_SLAB_T
 }
 
 int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const double *w, const double *bdev,
@@ -614,7 +590,7 @@ static int slab_build(Hubbard &h)
     L->xt_len = L->nsu * dd * kSW;
     L->nops_u = h.otf.up.nops;
     L->nops_d = h.otf.dn.nops;
-    L->npart = L->nsu * ((dd + kTR - 1) / kTR);
+    L->npart = L->nsu * ((dd + 31) / 32);
     int s = ED_OK;
     if ((s = upload(&L->pos_u, posu)) || (s = upload(&L->pos_d, posd)) || (s = upload(&L->u_at, atu)) ||
         (s = upload(&L->d_at, atd)) || (s = upload(&L->st_u, spu)) || (s = upload(&L->st_d, spd)) ||
@@ -629,26 +605,40 @@ static int slab_build(Hubbard &h)
     return ED_OK;
 }
 
-// ed_lanczos_eigen on real vectors of 2^27 amplitudes (1 GiB) and more builds the slab layout on
-// first use when it is supported and three more vectors (the scratch copy and the solver's two
-// layout copies of f and evec) fit in free device memory; ED_SLAB_AUTO=0 turns this off.  The
-// layout stays in the handle (ed_hubbard_slab_layout(h, 0, ...) frees it).
-void slab_auto(Hubbard &h)
+// ed_lanczos_eigen and ed_lanczos_tridiagonal on real vectors of 2^27 amplitudes (1 GiB) and more
+// build the slab layout on first use when it is supported and its scratch vector plus the nvec
+// layout vectors the solver will allocate fit in free device memory (keeping 5 % of the card
+// free); ED_SLAB_AUTO=0 turns this off.  The layout stays in the handle
+// (ed_hubbard_slab_layout(h, 0, ...) frees it).
+static int64_t slab_len_of(int64_t du, int64_t dd) { return ((dd + kSW - 1) / kSW) * du * kSW; }
+
+static bool slab_auto_fits(double free_bytes, double total_bytes, int64_t du, int64_t dd, int nvec)
+{
+    if ((double)du * (double)dd < (double)(1 << 27)) return false;
+    return free_bytes >= (double)(nvec + 1) * 8.0 * (double)slab_len_of(du, dd) + 0.05 * total_bytes;
+}
+
+void slab_auto(Hubbard &h, int nvec)
 {
     if (h.slab || h.algo != 3 || h.cplx_tables || h.has_fermi || h.N > 26) return;
     const char *e = getenv("ED_SLAB_AUTO");
     if (e && atoi(e) == 0) return;
-    const double D = (double)h.otf.dim_up * (double)h.otf.dim_dn;
-    if (D < (double)(1 << 27)) return;
+    const int64_t du = h.otf.dim_up, dd = h.otf.dim_dn;
+    if ((double)du * (double)dd < (double)(1 << 27)) return;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return;
-    if ((double)free_b < 3.2 * 8.0 * D) return;
+    if (!slab_auto_fits((double)free_b, (double)total_b, du, dd, nvec)) return;
     if (slab_build(h) != ED_OK) (void)hipGetLastError(); // stays on the reference layout
 }
 
 } // namespace ed
 
 using namespace ed;
+
+extern "C" int ed_slab_auto_decision(double free_bytes, double total_bytes, int64_t dim_up, int64_t dim_dn, int nvec)
+{
+    return slab_auto_fits(free_bytes, total_bytes, dim_up, dim_dn, nvec) ? 1 : 0;
+}
 
 extern "C" int ed_hubbard_slab_layout(ed_hubbard hh, int enable, int64_t *len)
 {
@@ -684,22 +674,120 @@ extern "C" int ed_slab_to_rowmajor(ed_hubbard hh, const double *d_xl, double *d_
     return slab_to_rowmajor(*h, d_xl, d_x, stream_of(stream));
 }
 
+template <int NB, int TR>
+static int slab_ceiling_t(const Hubbard &h, int pass, const double *d_xl, hipStream_t st)
+{
+    const SlabLayout &L = *h.slab;
+    const SlabArgs a = slab_args(h, pass == 0);
+    const int64_t rows = pass == 0 ? L.du : L.dd, slabs = pass == 0 ? L.nsd : L.nsu;
+    const int64_t ntr = (rows + TR - 1) / TR, tiles = slabs * ntr, per = (tiles + 7) / 8;
+    if (per * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab gather ceiling: grid too large");
+    // pass 1 gathers over the layout-T scratch, as k_slab_dn does
+    const double *src = pass == 0 ? d_xl : L.xt;
+    hipLaunchKernelGGL((k_slab_ceiling<NB, TR>), dim3((unsigned)(per * 8)), dim3(8 * TR), 0, st, a, tiles, per, ntr,
+                       rows, src, L.part, L.npart);
+    ED_CHECK_LAUNCH();
+    return ED_OK;
+}
+
 extern "C" int ed_hubbard_slab_gather_ceiling(ed_hubbard hh, int pass, const double *d_xl, void *stream)
 {
     const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
     if (!h || !h->slab || !d_xl || (pass != 0 && pass != 1))
         return fail(ED_EINVAL, "ed_hubbard_slab_gather_ceiling: no slab layout, null buffer or pass not 0/1");
     const SlabLayout &L = *h->slab;
-    SlabArgs a = slab_args(*h, pass == 0);
-    a.swz = 8;
-    const int64_t rows = pass == 0 ? L.du : L.dd, slabs = pass == 0 ? L.nsd : L.nsu;
-    const int64_t ntr = (rows + kTR - 1) / kTR, tiles = slabs * ntr, per = (tiles + 7) / 8;
-    if (per * 8 > 0x7fffffffLL) return fail(ED_EUNSUPPORTED, "slab gather ceiling: grid too large");
-    // pass 1 gathers over the layout-T scratch, as k_slab_dn does
-    const double *src = pass == 0 ? d_xl : L.xt;
-    hipLaunchKernelGGL(k_slab_ceiling<8>, dim3((unsigned)(per * 8)), dim3(256), 0, stream_of(stream), a, tiles, per,
-                       ntr, rows, src, L.part, L.npart);
-    ED_CHECK_LAUNCH();
+    hipStream_t st = stream_of(stream);
+    if (L.tr == 64) return L.nb == 16 ? slab_ceiling_t<16, 64>(*h, pass, d_xl, st) : slab_ceiling_t<8, 64>(*h, pass, d_xl, st);
+    return L.nb == 16 ? slab_ceiling_t<16, 32>(*h, pass, d_xl, st) : slab_ceiling_t<8, 32>(*h, pass, d_xl, st);
+}
+
+extern "C" int ed_hubbard_slab_set_shape(ed_hubbard hh, int tile_rows, int gathers_in_flight, int tile_order)
+{
+    Hubbard *h = reinterpret_cast<Hubbard *>(hh);
+    if (!h || !h->slab) return fail(ED_EINVAL, "ed_hubbard_slab_set_shape: no slab layout");
+    if ((tile_rows != 32 && tile_rows != 64) || (gathers_in_flight != 8 && gathers_in_flight != 16) ||
+        (tile_order != 0 && tile_order != 2 && tile_order != 4 && tile_order != 8 && tile_order != 16))
+        return fail(ED_EINVAL, "ed_hubbard_slab_set_shape: tile rows 32/64, gathers 8/16, tile order 0/2/4/8/16");
+    h->slab->tr = tile_rows;
+    h->slab->nb = gathers_in_flight;
+    h->slab->swz = tile_order;
+    return ED_OK;
+}
+
+// Slab autotuner: times the launch shapes of the two passes on the caller's vectors and keeps the
+// fastest in the layout.  Stage 1: tile rows (32, 64) x gathers in flight per lane (8, 16) in the
+// default tile order; stage 2: the tile order (plain, or runs of 2, 4, 8, 16 adjacent tiles per
+// CU slot, slab_tile) with the best stage-1 shape.  Every candidate adds the same terms in the same
+// order per element, so y does not depend on the choice.
+extern "C" int ed_hubbard_slab_autotune(ed_hubbard hh, const double *d_xl, double *d_yl, int reps,
+                                        ed_slab_tuning *best, void *stream)
+{
+    Hubbard *h = reinterpret_cast<Hubbard *>(hh);
+    if (!h || !h->slab || !d_xl || !d_yl || d_xl == d_yl || reps < 1)
+        return fail(ED_EINVAL, "ed_hubbard_slab_autotune: no slab layout, null or aliased buffers, or reps < 1");
+    SlabLayout &L = *h->slab;
+    hipStream_t st = stream_of(stream);
+    hipEvent_t e0, e1;
+    ED_HIP(hipEventCreate(&e0));
+    ED_HIP(hipEventCreate(&e1));
+    double cand[8];
+    for (double &c : cand) c = -1.0;
+    int s = ED_OK;
+    auto time_one = [&](float &ms) -> int {
+        ED_TRY(slab_apply(*h, d_xl, d_yl, nullptr, d_yl, nullptr, nullptr, 0, 0, nullptr, st, nullptr));
+        ED_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r)
+            ED_TRY(slab_apply(*h, d_xl, d_yl, nullptr, d_yl, nullptr, nullptr, 0, 0, nullptr, st, nullptr));
+        ED_HIP(hipEventRecord(e1, st));
+        ED_HIP(hipEventSynchronize(e1));
+        ED_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        return ED_OK;
+    };
+    float best_ms = -1.0f;
+    int btr = L.tr, bnb = L.nb, bswz = L.swz, ci = 0;
+    L.swz = 8;
+    for (int tr : {32, 64})
+        for (int nb : {8, 16}) {
+            if (s) break;
+            L.tr = tr;
+            L.nb = nb;
+            float ms = 0.0f;
+            if ((s = time_one(ms))) break;
+            cand[ci++] = ms;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                btr = tr;
+                bnb = nb;
+                bswz = 8;
+            }
+        }
+    for (int swz : {0, 2, 4, 16}) {
+        if (s) break;
+        L.tr = btr;
+        L.nb = bnb;
+        L.swz = swz;
+        float ms = 0.0f;
+        if ((s = time_one(ms))) break;
+        cand[ci++] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            bswz = swz;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    L.tr = btr;
+    L.nb = bnb;
+    L.swz = bswz;
+    if (s) return s;
+    if (best) {
+        best->tile_rows = btr;
+        best->gathers_in_flight = bnb;
+        best->tile_order = bswz;
+        best->ms_per_apply = best_ms;
+        for (int i = 0; i < 8; ++i) best->candidate_ms[i] = cand[i];
+    }
     return ED_OK;
 }
 

@@ -287,7 +287,8 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     // real sectors with a slab layout (ed_hubbard_slab_layout, or built here for sectors of 2^27
     // amplitudes and more) run the whole recurrence on it: the start vector is converted once and
     // the Ritz vector converted back at the end
-    if (!cvec) slab_auto(*const_cast<Hubbard *>(h));
+    // layout vectors of the slab path: w.buf[0..1], the copy of f and (with a Ritz vector) of evec
+    if (!cvec) slab_auto(*const_cast<Hubbard *>(h), want_vec ? 4 : 3);
     const bool slab = h->slab && !cvec;
     const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
 
@@ -423,7 +424,7 @@ extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int ns
     const int64_t n = h->args.dim_up * h->args.dim_dn;
     // real sectors run on the slab layout when the model has one (or gets one here, from 2^27
     // amplitudes on): only the coefficients leave, so f is converted once and nothing comes back
-    if (!cvec) slab_auto(*const_cast<Hubbard *>(h));
+    if (!cvec) slab_auto(*const_cast<Hubbard *>(h), 3); // w.buf[0..1] and f
     const bool slab = h->slab && !cvec;
     const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
     Work w;

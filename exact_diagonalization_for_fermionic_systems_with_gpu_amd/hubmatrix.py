@@ -160,6 +160,23 @@ class Hubbard:
             raise ValueError("apply_slab: x and y must be different buffers")
         call("ed_hubbard_apply_slab", self._h, _raw(xl), _raw(yl), stream_ptr())
 
+    def slab_autotune(self, xl: torch.Tensor, yl: torch.Tensor, reps: int = 3) -> dict:
+        """Time the slab H.v launch shapes (tile rows x gathers in flight, then tile order) on xl
+        (yl is overwritten) and keep the fastest in the layout; returns the choice.  Results do not
+        depend on the choice."""
+        self._slab_vec(xl, yl)
+        if xl.data_ptr() == yl.data_ptr():
+            raise ValueError("slab_autotune: x and y must be different buffers")
+        from ._lib import SlabTuning
+        t = SlabTuning()
+        call("ed_hubbard_slab_autotune", self._h, _raw(xl), _raw(yl), int(reps), C.byref(t), stream_ptr())
+        return {"tile_rows": t.tile_rows, "gathers_in_flight": t.gathers_in_flight, "tile_order": t.tile_order,
+                "ms_per_apply": t.ms_per_apply, "candidate_ms": [round(v, 3) for v in t.candidate_ms]}
+
+    def slab_set_shape(self, tile_rows: int = 32, gathers_in_flight: int = 8, tile_order: int = 8):
+        """pin the slab H.v launch shape (e.g. a choice of slab_autotune from an earlier run)"""
+        call("ed_hubbard_slab_set_shape", self._h, int(tile_rows), int(gathers_in_flight), int(tile_order))
+
     def slab_gather_ceiling(self, xl: torch.Tensor, pass_: int):
         """the line gathers of one slab H.v pass alone (0: UP over xl, 1: DN over the layout's
         transposed scratch), with the kernels' tiles and tile order but no arithmetic or streams:

@@ -180,18 +180,43 @@ int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const doubl
  * stay in L2.  enable = 1 builds the layout tables and a scratch vector (len doubles) in the
  * handle, after which ed_lanczos_eigen on real vectors runs its recurrence on this layout
  * (converting the start vector in and the Ritz vector out); enable = 0 frees them.  Needs a
- * real operator from ed_hubbard_create_ops with N <= 26.  ed_lanczos_eigen builds it by itself
- * for real sectors of 2^27 amplitudes and more when three more vectors fit in free device memory
- * (ED_SLAB_AUTO=0 turns that off).  enable = -1 only reports *len (0 when there is no layout). */
+ * real operator from ed_hubbard_create_ops with N <= 26.  ed_lanczos_eigen and
+ * ed_lanczos_tridiagonal build it by themselves for real sectors of 2^27 amplitudes and more when
+ * its scratch vector and every layout vector the solver allocates fit in free device memory with
+ * 5 % of the card to spare (ed_slab_auto_decision; ED_SLAB_AUTO=0 turns that off).  enable = -1
+ * only reports *len (0 when there is no layout). */
 int ed_hubbard_slab_layout(ed_hubbard h, int enable, int64_t *len);
+/* The solvers' decision above, host only: 1 when a dim_up x dim_dn real sector would get the slab
+ * layout with free_bytes of total_bytes free and nvec solver vectors on the layout (eigen with a
+ * Ritz vector: 4; eigenvalue only or tridiagonal: 3), else 0. */
+int ed_slab_auto_decision(double free_bytes, double total_bytes, int64_t dim_up, int64_t dim_dn, int nvec);
 /* reference layout psi[i_up*dim_dn + i_dn] -> slab layout (padding set to zero), and back */
 int ed_slab_from_rowmajor(ed_hubbard h, const double *d_x, double *d_xl, void *stream);
 int ed_slab_to_rowmajor(ed_hubbard h, const double *d_xl, double *d_x, void *stream);
 /* y = H x on the slab layout (x != y, x padding zero; y padding is written zero).  Asynchronous. */
 int ed_hubbard_apply_slab(ed_hubbard h, const double *d_xl, double *d_yl, void *stream);
+/* Slab-layout autotuner (no counterpart in the reference; the reference has one launch shape,
+ * HubMatrix_GPU.cuh:54-61): times the launch shapes of the slab H.v with `reps` applications of
+ * H to d_xl (into d_yl, overwritten) and keeps the fastest in the layout -- first tile rows (32,
+ * 64) x gathers in flight per lane (8, 16), then the tile order (plain, or runs of 2, 4, 8, 16
+ * adjacent tiles per CU slot) with the best of those; every candidate gives the same y bit for
+ * bit.  best (may be NULL) gets the choice and its time per H.v; candidate_ms holds the times in
+ * the order tried ([32,8], [32,16], [64,8], [64,16] at order 8, then orders 0, 2, 4, 16). */
+typedef struct {
+    int tile_rows, gathers_in_flight, tile_order;
+    double ms_per_apply;
+    double candidate_ms[8];
+} ed_slab_tuning;
+int ed_hubbard_slab_autotune(ed_hubbard h, const double *d_xl, double *d_yl, int reps, ed_slab_tuning *best,
+                             void *stream);
+/* Sets the slab H.v launch shape directly (a choice the autotuner made earlier): tile rows 32 or
+ * 64, gathers in flight 8 or 16, tile order 0, 2, 4, 8 or 16.  Host only; results do not depend on
+ * it.  Default 32, 8, 8. */
+int ed_hubbard_slab_set_shape(ed_hubbard h, int tile_rows, int gathers_in_flight, int tile_order);
 /* Measurement probe (no counterpart in the reference): the line gathers of one pass of
  * ed_hubbard_apply_slab alone -- pass 0 the UP gathers over d_xl, pass 1 the DN gathers over the
- * layout's transposed scratch -- with the same grid, tile order, LDS footprint and entry tables,
+ * layout's transposed scratch -- with the same launch shape (the autotuner's choice), grid, tile
+ * order, LDS footprint and entry tables,
  * but without the value table, the FMAs and the stream loads and stores.  Its time is the cost
  * of that pass's gathers.  Asynchronous; writes nothing the caller sees. */
 int ed_hubbard_slab_gather_ceiling(ed_hubbard h, int pass, const double *d_xl, void *stream);

@@ -199,3 +199,52 @@ def test_lanczos_local_reorthogonalisation(layout):
     assert abs(float(torch.linalg.norm(ev)) - 1.0) <= 1e-12
     res = float(torch.linalg.norm(H.apply(ev) - e_re * ev))
     assert res <= 1e-6 * abs(e_re), res
+
+
+def test_slab_shapes_bitwise_and_autotune():
+    """Every slab launch shape (tile rows x gathers in flight x tile order) gives the reference
+    layout's H.v bit for bit, and the autotuner's choice is one of them."""
+    print("slab shapes chain14", flush=True)
+    H, _ = _build("chain14_7_7")
+    H.slab_layout(True)
+    x = torch.from_numpy(np.random.default_rng(5).normal(size=H.dim)).to(DEV)
+    yref = H.apply(x)
+    xl = H.to_slab(x)
+    yl = torch.empty_like(xl)
+    for tr in (32, 64):
+        for nb in (8, 16):
+            for order in (0, 2, 4, 8, 16):
+                H.slab_set_shape(tr, nb, order)
+                yl.fill_(float("nan"))
+                H.apply_slab(xl, yl)
+                assert torch.equal(H.from_slab(yl), yref), (tr, nb, order)
+    best = H.slab_autotune(xl, yl, reps=1)
+    assert best["tile_rows"] in (32, 64) and best["gathers_in_flight"] in (8, 16)
+    assert best["tile_order"] in (0, 2, 4, 8, 16) and best["ms_per_apply"] > 0
+    assert abs(best["ms_per_apply"] - min(v for v in best["candidate_ms"] if v > 0)) <= 1e-3
+    H.apply_slab(xl, yl)
+    assert torch.equal(H.from_slab(yl), yref)
+    with pytest.raises(EdError):
+        H.slab_set_shape(48, 8, 8)
+
+
+def test_slab_buffer_checks():
+    """The layout conversions and H.v check their buffers against the handle's layout length and
+    dtype before any kernel runs."""
+    H, _ = _build("kagome12_6_6")
+    n = H.slab_layout(True)
+    x = torch.zeros(H.dim, dtype=torch.float64, device=DEV)
+    with pytest.raises(ValueError):
+        H.to_slab(x, torch.empty(n - 1, dtype=torch.float64, device=DEV))
+    with pytest.raises(ValueError):
+        H.to_slab(x.to(torch.complex128))
+    with pytest.raises(ValueError):
+        H.from_slab(torch.empty(n + 16, dtype=torch.float64, device=DEV))
+    with pytest.raises(ValueError):
+        H.from_slab(torch.empty(n, dtype=torch.float32, device=DEV))
+    xl = H.to_slab(x)
+    with pytest.raises(ValueError):
+        H.apply_slab(xl, xl)
+    H.slab_layout(False)
+    with pytest.raises(ValueError):
+        H.to_slab(x)
