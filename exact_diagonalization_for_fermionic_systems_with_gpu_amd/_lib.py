@@ -79,6 +79,7 @@ SIGNATURES = {
     "ed_ted_time_evol": (_i32, [_i64, _i32, _p, _p, _p, _d, _p, _p]),
     "ed_random_vector": (_i32, [_i64, C.c_uint32, _i32, _p, _i32, _p]),
     "ed_dot": (_i32, [_i64, _i32, _p, _p, _pd, _p]),
+    "ed_dot_noalloc": (_i32, [_i64, _i32, _p, _p, _p, _p]),
     "ed_axpy": (_i32, [_i64, _i32, _d, _p, _p, _p]),
     "ed_scal": (_i32, [_i64, _i32, _d, _p, _p]),
     "ed_normalize": (_i32, [_i64, _i32, _p, _pd, _p]),

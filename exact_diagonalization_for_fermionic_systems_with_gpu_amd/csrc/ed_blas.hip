@@ -7,6 +7,8 @@
 // index order.  Plain and fused paths therefore give bit-identical scalars, and a
 // run gives the same scalars every time.  Complex vectors are handled as 2n reals
 // (Re<a,b> and |a|^2 are real dots of the interleaved pairs).
+#include <mutex>
+
 #include "ed_common.hpp"
 #include "ed_hubbard.hpp"
 
@@ -297,6 +299,67 @@ extern "C" int ed_normalize(int64_t n, int complex_vectors, void *d_x, double *n
     if (norm2_out) {
         ED_HIP(hipMemcpyAsync(norm2_out, g_scratch.scal, sizeof(double), hipMemcpyDeviceToHost, st));
         ED_HIP(hipStreamSynchronize(st));
+    }
+    return ED_OK;
+}
+
+// ---- allocation-free dot product ---------------------------------------------------------------
+// ed_dot_noalloc takes its block partials from this file-scope device array instead of a workspace
+// it allocates: 2^21 partials (16 MiB on every GPU the library runs on, held until the process
+// ends), i.e. vectors of at most 2^21 * 2048 = 2^32 real elements.  In host code `scratch` names a
+// host-side stand-in; kernels get the device address from hipGetSymbolAddress, looked up once per
+// device with that device current.
+constexpr int64_t kNoAllocPartials = (int64_t)1 << 21;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the calling thread's current device's copy of `scratch`
+int scratch_ptr(double **p)
+{
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ED_EUNSUPPORTED, "ed_dot_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *q = nullptr;
+        ED_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)q;
+    }
+    *p = g_scratch_ptr[dev];
+    return ED_OK;
+}
+
+__global__ void k_store_zero(double *p) { *p = 0.0; }
+} // namespace
+
+extern "C" int ed_dot_noalloc(int64_t n, int complex_vectors, const void *d_a, const void *d_b, double *d_out,
+                              void *stream)
+{
+    if (n < 1 || !d_a || !d_b || !d_out) return fail(ED_EINVAL, "ed_dot_noalloc: bad arguments");
+    const int64_t nreal = complex_vectors ? 2 * n : n;
+    if (blas_num_blocks(nreal) > kNoAllocPartials)
+        return fail(ED_EINVAL, "ed_dot_noalloc: %lld real elements exceed the fixed partial buffer (at most %lld)",
+                    (long long)nreal, (long long)(kNoAllocPartials * kChunk));
+    double *part = nullptr;
+    ED_TRY(scratch_ptr(&part));
+    hipStream_t st = stream_of(stream);
+    int64_t nb;
+    ED_TRY(dot_partials(nreal, (const double *)d_a, (const double *)d_b, part, &nb, st));
+    ED_TRY(finalize_sum(part, nb, d_out, 0, st));
+    if (complex_vectors) {
+        // stream order: the finalize above has read the partials before k_dot_im overwrites them
+        const int64_t nb2 = blas_num_blocks(n);
+        hipLaunchKernelGGL(k_dot_im, dim3((unsigned)nb2), dim3(kBB), 0, st, n, (const double *)d_a,
+                           (const double *)d_b, part);
+        ED_CHECK_LAUNCH();
+        ED_TRY(finalize_sum(part, nb2, d_out + 1, 0, st));
+    } else {
+        hipLaunchKernelGGL(k_store_zero, dim3(1), dim3(1), 0, st, d_out + 1);
+        ED_CHECK_LAUNCH();
     }
     return ED_OK;
 }

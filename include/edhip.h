@@ -230,6 +230,15 @@ int ed_random_vector(int64_t n, uint32_t seed, int complex_vectors, void *d_v, i
 /* thrust_Dotprod (HubMatrix_GPU.cuh:451-471): out[0] = Re<a,b>, out[1] = Im<a,b> (0 for
  * real); deterministic reduction; synchronises the stream. */
 int ed_dot(int64_t n, int complex_vectors, const void *d_a, const void *d_b, double *out, void *stream);
+/* Allocation-free variant of ed_dot.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and reduction order as ed_dot, so the same value bit for bit, but the
+ * block partials live in a fixed library-owned device array (16 MiB per GPU, held until the
+ * process ends): n real elements (2n for complex vectors) must not exceed 2^32, else ED_EINVAL.
+ * d_out is DEVICE memory for two doubles, {Re<a,b>, Im<a,b>} (Im written 0 for real vectors).
+ * Asynchronous; allocates nothing and never synchronizes.  Calls that overlap on one GPU (other
+ * streams or host threads) share the array and race: serialize them. */
+int ed_dot_noalloc(int64_t n, int complex_vectors, const void *d_a, const void *d_b, double *d_out, void *stream);
 /* y = alpha*x + y (real alpha) */
 int ed_axpy(int64_t n, int complex_vectors, double alpha, const void *d_x, void *d_y, void *stream);
 /* x = alpha*x */
