@@ -10,14 +10,59 @@
 //     first slot whose value is exactly 0.0 -- the same slot the reference picks.
 // Both are integer/byte work, bit-exact against the reference's own fermi_operator.h.
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
 #include <vector>
 
 #include "ed_common.hpp"
+#include "ed_thread.hpp"
 
 namespace ed {
+
+// Set when this library's static objects are destroyed (process exit).  libedhip depends on the
+// HIP runtime, so its statics are destroyed before the runtime's: until the flag is set, HIP calls
+// from thread-exit destructors are safe.
+static std::atomic<bool> g_exiting{false};
+namespace {
+struct ExitFlag {
+    ~ExitFlag() { g_exiting.store(true); }
+} g_exit_flag;
+
+struct ThreadBlas {
+    rocblas_handle h = nullptr;
+    bool tried = false;
+    ~ThreadBlas()
+    {
+        if (h && !process_exiting()) (void)rocblas_destroy_handle(h);
+    }
+};
+thread_local ThreadBlas t_blas;
+} // namespace
+
+bool process_exiting() { return g_exiting.load(); }
+
+rocblas_handle thread_blas_handle()
+{
+    if (!t_blas.tried) {
+        t_blas.tried = true;
+        if (rocblas_create_handle(&t_blas.h) != rocblas_status_success) t_blas.h = nullptr;
+    }
+    return t_blas.h;
+}
+
+void blas_initialize_once()
+{
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return;
+    rocblas_initialize();
+    done[dev] = true;
+}
 
 static thread_local std::string g_err;
 
@@ -82,7 +127,7 @@ void dfree(void *p)
 
 Arena::~Arena()
 {
-    if (base) (void)hipFree(base);
+    if (base && !process_exiting()) (void)hipFree(base);
 }
 
 Scratch::~Scratch()

@@ -10,6 +10,7 @@
 #include <mutex>
 
 #include "ed_common.hpp"
+#include "ed_thread.hpp"
 #include "ed_hubbard.hpp"
 
 namespace ed {
@@ -209,8 +210,13 @@ struct Scratch {
     double *partial = nullptr;
     double *scal = nullptr;
     int64_t cap = 0;
-    // freed only when a larger buffer is needed: device memory is not released at
-    // thread exit, where the HIP runtime may already be gone
+    // grow-only; released when the thread exits (not during process teardown, ed_thread.hpp)
+    ~Scratch()
+    {
+        if (process_exiting()) return;
+        dfree(partial);
+        dfree(scal);
+    }
     int ensure(int64_t n)
     {
         if (!scal) ED_TRY(dmalloc((void **)&scal, sizeof(double) * 8));

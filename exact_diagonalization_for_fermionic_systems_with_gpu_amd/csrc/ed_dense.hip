@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "ed_common.hpp"
+#include "ed_thread.hpp"
 #include "ed_hubbard.hpp"
 
 namespace ed {
@@ -204,16 +205,7 @@ unsigned grid_for(int64_t n)
     return (unsigned)(g < 65536 ? (g > 0 ? g : 1) : 65536);
 }
 
-rocblas_handle dense_handle()
-{
-    thread_local rocblas_handle h = nullptr;
-    thread_local bool tried = false;
-    if (!tried) {
-        tried = true;
-        if (rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
-    }
-    return h;
-}
+rocblas_handle dense_handle() { return thread_blas_handle(); }
 
 } // namespace
 

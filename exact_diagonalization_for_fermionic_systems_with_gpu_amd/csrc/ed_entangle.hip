@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "ed_common.hpp"
+#include "ed_thread.hpp"
 
 namespace ed {
 
@@ -276,18 +277,8 @@ static int make_plan(int N, int Qu, int Qd, uint32_t maskA, Plan &P)
     return ED_OK;
 }
 
-// one rocBLAS handle per host thread (a handle carries its stream, so threads that work on
-// different streams must not share one); kept until the process ends
-static rocblas_handle solver_handle()
-{
-    thread_local rocblas_handle h = nullptr;
-    thread_local bool tried = false;
-    if (!tried) {
-        tried = true;
-        if (rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
-    }
-    return h;
-}
+// the calling thread's rocBLAS handle (ed_thread.hpp: one per host thread, destroyed at exit)
+static rocblas_handle solver_handle() { return thread_blas_handle(); }
 
 // worker streams (default priority) and rocSOLVER handles of one device for the concurrent block
 // solves, created on first use and kept until the process ends; calls that use them take turns
@@ -295,7 +286,23 @@ constexpr int kEeWorkers = 4;
 struct EePool {
     hipStream_t st[kEeWorkers] = {};
     rocblas_handle h[kEeWorkers] = {};
+    bool warm[2][2] = {}; // [complex][with Schmidt values]: routines already run on a calling thread
 };
+
+// a k x k symmetric (Hermitian) test matrix, column-major: 1/(1+|i-j|) (+ i 0.1 sign(j-i))
+__global__ void k_warm_matrix(int64_t k, int cplx, double *A)
+{
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= k * k) return;
+    const int64_t i = q % k, j = q / k;
+    const double re = 1.0 / (1.0 + (double)(i > j ? i - j : j - i));
+    if (cplx) {
+        A[2 * q] = re;
+        A[2 * q + 1] = i < j ? 0.1 : (i > j ? -0.1 : 0.0);
+    } else {
+        A[q] = re;
+    }
+}
 static std::mutex g_ee_mu;
 static EePool *g_ee_pool[64] = {};
 
@@ -500,6 +507,31 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 }
             }
         };
+        if (nw > 1 && !pool->warm[cplx ? 1 : 0][want_sigma ? 1 : 0]) {
+            // Before the first worker threads on this device: rocBLAS's kernels are loaded and each
+            // rocSOLVER routine the workers call is run once on this thread, so that no kernel is
+            // loaded by several threads at once on their first calls.
+            blas_initialize_once();
+            const int64_t kw = std::min<int64_t>(maxk, 2048);
+            hipLaunchKernelGGL(k_warm_matrix, dim3((unsigned)((kw * kw + 255) / 256)), dim3(256), 0, st, kw,
+                               cplx ? 1 : 0, dR);
+            rocblas_status rs = cplx ? rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)kw,
+                                                        (rocblas_double_complex *)dR, (rocblas_int)kw, dWall, dE, dinfo)
+                                     : rocsolver_dsyevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)kw, dR,
+                                                        (rocblas_int)kw, dWall, dE, dinfo);
+            if (rs == rocblas_status_success && want_sigma)
+                rs = cplx ? rocsolver_zgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)kw,
+                                             (rocblas_int)kw, (rocblas_double_complex *)dR, (rocblas_int)kw, dSall,
+                                             nullptr, 1, nullptr, 1, dE, rocblas_outofplace, dinfo + njobs)
+                          : rocsolver_dgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)kw,
+                                             (rocblas_int)kw, dR, (rocblas_int)kw, dSall, nullptr, 1, nullptr, 1, dE,
+                                             rocblas_outofplace, dinfo + njobs);
+            if (rs != rocblas_status_success || hipStreamSynchronize(st) != hipSuccess) {
+                s = fail(ED_EHIP, "entanglement: solver warm-up failed (%d)", (int)rs);
+                break;
+            }
+            pool->warm[cplx ? 1 : 0][want_sigma ? 1 : 0] = true;
+        }
         if (nw == 1) {
             worker(0, st, h);
         } else {
