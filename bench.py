@@ -50,6 +50,16 @@ def read_traffic():
         return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
     """Oracle H.v on a bounded row sample; returns (GB/s by the same 16 B/element count, rows, secs)."""
     import numpy as np
@@ -82,10 +92,32 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     lz.random_vector(x)
     H.slab_layout(True)  # the recurrence runs on the slab layout (start and Ritz vectors converted)
     torch.cuda.synchronize()
+    cur, peak = C.c_int64(), C.c_int64()
+    call("ed_memory_stats", C.byref(cur), C.byref(peak), 1)
     t0 = time.perf_counter()
     e0 = lz.eigen(H, x, y)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    call("ed_memory_stats", C.byref(cur), C.byref(peak), 0)
+    D = hq.count * hq.count
+    # one forward step: the fixed-length recurrence (ed_lanczos_tridiagonal, same step kernels
+    # without the convergence checks) at 2 and 12 steps; the difference is 10 steps
+    f0 = torch.empty_like(x)
+    lz.random_vector(f0)
+    a_out, b_out = np.zeros(16), np.zeros(16)
+    tt = []
+    for ns in (2, 12):
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        call("ed_lanczos_tridiagonal", H.handle, 0, ns, dptr(f0), a_out.ctypes.data_as(C.POINTER(C.c_double)),
+             b_out.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
+        torch.cuda.synchronize()
+        tt.append(time.perf_counter() - t_a)
+    ms_step = (tt[1] - tt[0]) / 10 * 1e3
+    del f0
+    # the caller's start and Ritz vectors plus everything the library held at its peak (the slab
+    # layout's tables and scratch, the solver's work vectors); rocBLAS workspaces not included
+    peak_bytes = 2 * 8 * D + peak.value
     H.slab_layout(False)
     torch.cuda.empty_cache()
     log(f"[bench] ground state E0 {e0:.15f} in {lz.nsteps} steps, {t1 - t0:.1f} s")
@@ -96,7 +128,50 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     t2 = time.perf_counter()
     log(f"[bench] S_vN({N // 2}|{N - N // 2}) {s[0]:.15f} in {t2 - t1:.1f} s")
     return {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "reorth": bool(reorth),
+            "ms_per_forward_step": round(ms_step, 2), "peak_device_gb": round(peak_bytes / 1e9, 2),
+            "peak_device_vectors": round(peak_bytes / (8.0 * D), 2),
             "S_vN": float(s[0]), "entropy_s": round(t2 - t1, 2)}
+
+
+def kagome12_line(with_cpu=True):
+    """The plain method at the reference's CPU-runnable size: 12-site Kagome, 6/6, U = 4, hopping -1,
+    Lanczos ground state (reference convergence test, Ritz vector rebuilt) from the seed-235 start
+    on the GPU, and the same solve by the oracle (C restatement of Lanczos<T>::eigen, OpenMP) on the
+    host cores, E0 compared to 1e-10."""
+    import numpy as np
+    import torch
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import H_TBq, Hubbard, Lanczos
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+    ops = L.hubbard_ops(L.KAGOME12, t=-1.0)
+    hq = H_TBq(12, 6)
+    hq.add_ops(ops)
+    H = Hubbard(hq, hq, 4.0)
+    f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+    ev = torch.empty_like(f)
+    lz = Lanczos(hq.count, hq.count)
+    for _ in range(2):  # the first solve also loads the kernels
+        lz.random_vector(f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0 = lz.eigen(H, f, ev)
+        torch.cuda.synchronize()
+        gpu_s = time.perf_counter() - t0
+    out = {"E0": e0, "lanczos_steps": lz.nsteps, "gpu_s": round(gpu_s, 4),
+           "workload": "kagome12_nup6_ndn6_U4_lanczos_ritz"}
+    if with_cpu:
+        from oracle import oracle as O
+        osp = O.Species(12, 6, ops)
+        Ho = O.Hubbard(osp, osp, 4.0)
+        t0 = time.perf_counter()
+        st, e0o, *_ = Ho.lanczos_eigen(O.random_vector(Ho.dim))
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_s, 3), "unit": "s per ground state (E0 + Ritz vector)",
+                               "cores": O.num_threads(), "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "the whole solve", "E0": e0o}
+        out["rel_err_vs_oracle"] = abs(e0 - e0o) / abs(e0o)
+    log(f"[bench] kagome12 ground state E0 {e0:.12f} in {lz.nsteps} steps, {gpu_s:.3f} s on the GPU"
+        + (f", {out['cpu_baseline']['value']} s on {out['cpu_baseline']['cores']} host cores" if with_cpu else ""))
+    return out
 
 
 def entropy_only_line(N=18, Q=9, seed=5):
@@ -126,7 +201,7 @@ def entropy_only_line(N=18, Q=9, seed=5):
             "workload": f"random_state_N{N}_nup{Q}_ndn{Q}_cut{N // 2}"}
 
 
-def batched_drude_line():
+def batched_drude_line(with_cpu=False):
     """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
     weight per sector; batched (ed_batch_lowest, hipGraph) vs the plain per-instance loop on the
     same GPU, both checked against the oracle's Drude weights (tests/golden fixture)."""
@@ -150,7 +225,25 @@ def batched_drude_line():
     err = float(max(max(np.max(np.abs(r[1][nz] - Dg[nz]) / np.abs(Dg[nz])) for r in runs_b + runs_p), 0.0))
     log(f"[bench] batched Drude sweep {tb:.3f} s (runs {[round(r[2], 3) for r in runs_b]}), plain {tp:.3f} s "
         f"(runs {[round(r[2], 3) for r in runs_p]}), max rel err vs oracle {err:.2e}")
+    cpu = None
+    if with_cpu:
+        # the oracle's complex Lanczos E0 (the restatement of Lanczos<T>::eigen) of the 13 sectors at
+        # one flux, Phi_1: 13 of the 208 instances, scaled to the sweep
+        import math
+        from oracle import oracle as O
+        from exact_diagonalization_for_fermionic_systems_with_gpu_amd import lattices as L
+        ops = L.lieb_twisted_ops(2 * math.pi / 16)
+        t0 = time.perf_counter()
+        for nu in range(13):
+            Ho = O.Hubbard(O.Species(12, nu, ops, True), O.Species(12, 12 - nu, ops, True), 4.0)
+            Ho.lanczos_eigen(O.random_vector(Ho.dim, complex_=True))
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(13 / (cs * 16), 4), "unit": "Drude weights/s", "cores": O.num_threads(),
+               "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"oracle E0 of the 13 sectors at one flux ({cs:.1f} s), times 16 fluxes"}
+        log(f"[bench] Drude sweep CPU sample: 13 instances in {cs:.1f} s")
     return {"value": round(13 / tb, 3), "unit": "Drude weights/s", "plain": round(13 / tp, 3),
+            "cpu_baseline": cpu,
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
             "max_abs_e0_batched_minus_plain": e0_diff, "max_abs_e0_flux_asymmetry": e0_asym,
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
@@ -218,8 +311,11 @@ def main():
                     help="skip the batched Drude sweep and the disorder-ensemble lines")
     ap.add_argument("--ensemble", type=int, default=128, help="realisations of the 14-site chain ensemble")
     ap.add_argument("--ground-state", action="store_true",
-                    help="also run the Lanczos ground state of the same workload (Ritz vector rebuilt) and its "
-                         "half-cluster entropy S_vN from the particle-number blocks of rho_A (FP64 MFMA Gram)")
+                    help="run the ground-state line even with --no-extras")
+    ap.add_argument("--no-ground-state", action="store_true",
+                    help="skip the Lanczos ground state of the same workload (Ritz vector rebuilt), its "
+                         "half-cluster entropy S_vN (particle-number blocks of rho_A, FP64 MFMA Gram) and peak "
+                         "device memory")
     ap.add_argument("--reorth", action="store_true",
                     help="with --ground-state: a DGKS pass of each new Lanczos vector against v_N")
     ap.add_argument("--entropy-only", action="store_true",
@@ -384,15 +480,17 @@ def main():
         torch.from_numpy(x_host).copy_(x)
         cg, rows, secs, thr = cpu_baseline(Hor, x_host, hq.count)
         out["cpu_baseline"] = {"value": round(cg, 3), "unit": "GB/s", "cores": thr, "kind": "port",
+                               "cpu_model": cpu_model(),
                                "sample": f"oracle H.v (C restatement of Hubbard_mv_), rows {16}..{16 + rows} "
                                          f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
         del x_host
-    if args.ground_state:
+    if args.ground_state or not (args.no_extras or args.no_ground_state):
         out["ground_state"] = ground_state_line(H, hq, x, y, N, Q, reorth=args.reorth)
     if not args.no_extras:
         del x, y
         torch.cuda.empty_cache()
-        out["batched_drude"] = batched_drude_line()
+        out["batched_drude"] = batched_drude_line(with_cpu=not args.no_cpu)
+        out["kagome12_ground_state"] = kagome12_line(with_cpu=not args.no_cpu)
         out["disorder_ensemble"] = ensemble_line(args.ensemble)
         out["krylov_quench"] = krylov_line()
     print(json.dumps(out), flush=True)

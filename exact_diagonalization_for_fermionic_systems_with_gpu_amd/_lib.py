@@ -70,6 +70,7 @@ SIGNATURES = {
     "ed_hubbard_apply_slab": (_i32, [_p, _p, _p, _p]),
     "ed_hubbard_slab_gather_ceiling": (_i32, [_p, _i32, _p, _p]),
     "ed_hubbard_slab_set_shape": (_i32, [_p, _i32, _i32, _i32]),
+    "ed_memory_stats": (_i32, [_pi64, _pi64, _i32]),
     "ed_slab_auto_decision": (_i32, [C.c_double, C.c_double, _i64, _i64, _i32]),
     "ed_hubbard_slab_autotune": (_i32, [_p, _p, _p, _i32, C.POINTER(SlabTuning), _p]),
     "ed_dense_zero": (_i32, [_i64, _i32, _p, _p]),

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -107,6 +108,14 @@ int64_t binom(int n, int k)
     return binom_host().c[n][k];
 }
 
+// Device memory the library allocates itself (dmalloc/dfree, all devices together): current and
+// peak bytes, for ed_memory_stats.  rocBLAS/rocSOLVER workspaces are not counted.
+namespace {
+std::mutex g_mem_mu;
+std::unordered_map<void *, size_t> g_mem_size;
+int64_t g_mem_cur = 0, g_mem_peak = 0;
+} // namespace
+
 int dmalloc(void **p, size_t bytes)
 {
     *p = nullptr;
@@ -117,17 +126,28 @@ int dmalloc(void **p, size_t bytes)
         (void)hipGetLastError();
         return fail(ED_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_mem_size[*p] = bytes;
+    g_mem_cur += (int64_t)bytes;
+    if (g_mem_cur > g_mem_peak) g_mem_peak = g_mem_cur;
     return ED_OK;
 }
 
 void dfree(void *p)
 {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    (void)hipFree(p);
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_mem_size.find(p);
+    if (it != g_mem_size.end()) {
+        g_mem_cur -= (int64_t)it->second;
+        g_mem_size.erase(it);
+    }
 }
 
 Arena::~Arena()
 {
-    if (base && !process_exiting()) (void)hipFree(base);
+    if (base && !process_exiting()) dfree(base);
 }
 
 Scratch::~Scratch()
@@ -216,6 +236,15 @@ __global__ void k_basis_unrank_lowmem(int N, int Q, int64_t count, const int64_t
 } // namespace ed
 
 using namespace ed;
+
+extern "C" int ed_memory_stats(int64_t *current_bytes, int64_t *peak_bytes, int reset_peak)
+{
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    if (current_bytes) *current_bytes = g_mem_cur;
+    if (peak_bytes) *peak_bytes = g_mem_peak;
+    if (reset_peak) g_mem_peak = g_mem_cur;
+    return ED_OK;
+}
 
 extern "C" const char *ed_last_error(void) { return g_err.c_str(); }
 

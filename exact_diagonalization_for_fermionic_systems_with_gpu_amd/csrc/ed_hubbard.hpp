@@ -122,6 +122,8 @@ int up_row_order(int N, int Q, const ed_op *ops, int nops, std::vector<int32_t> 
 // Lanczos step halves on the slab layout (ed_hv_slab.hip): same contract as hv_lanczos /
 // hv_lanczos_nodot on vectors of slab_len() doubles
 int64_t slab_len(const Hubbard &h);
+// the length a slab layout of this operator has (or would have): dim_up * dim_dn when dim_dn is even
+int64_t slab_len_for(const Hubbard &h);
 int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const double *w, const double *bdev,
                  int step, bool dot, double *partial, hipStream_t st, int64_t *nblocks);
 // one step of the Ritz-vector rebuild on the slab layout: slab_lanczos without the dot, with

@@ -43,6 +43,20 @@ constexpr int kSW = 16;     // positions per slab: one 128 B line of a row
 // per lane NB (8 or 16) are template parameters chosen by the slab autotuner
 constexpr int kMaxOps = 64; // calls per species (the on-the-fly limit)
 
+// Width of the last slab of a species with dim positions: the remainder of dim / 16 rounded up to
+// even (16 B loads stay aligned), 16 when dim is a multiple of 16.  Every other slab is 16 wide,
+// so a layout vector has slab_len_of(rows, dim) = rows * (dim rounded up to that width) entries:
+// exactly rows * dim when dim is even.
+static int last_slab_width(int64_t dim)
+{
+    const int r = (int)(dim - ((dim + kSW - 1) / kSW - 1) * kSW);
+    return r + (r & 1);
+}
+static int64_t slab_len_of(int64_t rows, int64_t dim)
+{
+    return ((dim + kSW - 1) / kSW - 1) * rows * kSW + rows * last_slab_width(dim);
+}
+
 struct SlabLayout {
     int64_t du = 0, dd = 0, nsd = 0, nsu = 0, len = 0, xt_len = 0;
     int nops_u = 0, nops_d = 0;
@@ -56,6 +70,7 @@ struct SlabLayout {
     double *part = nullptr;                     // pass-B block partials of <v, y>
     int64_t npart = 0;                          // pass-B tiles at the smallest tile (32 rows)
     int tr = 32, nb = 8, swz = 8;               // launch shape: tile rows, gathers in flight, tile order
+    int wl = kSW, wt = kSW;                     // widths of the last slab of layout L and of layout T
 };
 
 void slab_free(SlabLayout *s)
@@ -79,7 +94,8 @@ struct SlabArgs {
     int64_t mask;
     const double *Uvec;
     int has_diag;
-    int swz; // tiles per CU slot in the tile order (slab_tile); 0 or 1: plain order
+    int swz;    // tiles per CU slot in the tile order (slab_tile); 0 or 1: plain order
+    int wl, wt; // line widths of the last slab of layout L (dn) and of layout T (up); others 16
 };
 
 // Tile of workgroup b.  The 8 XCDs take contiguous eighths of the tiles (workgroups are dealt to
@@ -145,7 +161,7 @@ __device__ __forceinline__ int slab_stage(const SlabArgs &a, int64_t p0, int64_t
 // Sum over the row's entries (call order) of value * row[neighbour][c..c+1]; x scaled by sc.
 // The gathers of NB entries are issued before their FMAs (NB loads of 16 B in flight per lane).
 template <bool SCALE, int NB>
-__device__ __forceinline__ void slab_gather(const double *__restrict__ base, const uint32_t *s_ent,
+__device__ __forceinline__ void slab_gather(const double *__restrict__ base, int W, const uint32_t *s_ent,
                                             const double *s_val, int o0, int o1, int c, double sc, double &acc0,
                                             double &acc1)
 {
@@ -153,7 +169,7 @@ __device__ __forceinline__ void slab_gather(const double *__restrict__ base, con
         double2 xv[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-            if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(base + (int64_t)(s_ent[j0 + k] >> 8) * kSW + c);
+            if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(base + (int64_t)(s_ent[j0 + k] >> 8) * W + c);
 #pragma unroll
         for (int k = 0; k < NB; ++k)
             if (j0 + k < o1) {
@@ -189,12 +205,13 @@ __global__ __launch_bounds__(8 * TR) void k_slab_up(SlabArgs a, int64_t ntiles, 
     const double sc = SCALE ? 1.0 / bdev[step] : 1.0;
     const int tid = threadIdx.x, r = tid >> 3, c = (tid & 7) * 2;
     const double *xs = x + slab * a.du * kSW;
-    if (r < nrows) {
+    const int W = slab == a.nsd - 1 ? a.wl : kSW; // line width of this slab (lanes c >= W idle)
+    if (r < nrows && c < W) {
         const int64_t p = p0 + r;
         double acc0 = 0.0, acc1 = 0.0;
-        slab_gather<SCALE, NB>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, sc, acc0, acc1);
-        st2(z + (slab * a.du + p) * kSW + c, acc0, acc1);
-        double2 xo = *reinterpret_cast<const double2 *>(xs + p * kSW + c);
+        slab_gather<SCALE, NB>(xs, W, s_ent, s_val, s_off[r], s_off[r + 1], c, sc, acc0, acc1);
+        st2(z + slab * a.du * kSW + p * W + c, acc0, acc1);
+        double2 xo = *reinterpret_cast<const double2 *>(xs + p * W + c);
         if (SCALE) {
             xo.x = sc * xo.x;
             xo.y = sc * xo.y;
@@ -206,11 +223,13 @@ __global__ __launch_bounds__(8 * TR) void k_slab_up(SlabArgs a, int64_t ntiles, 
         s_x[r][c + 1] = 0.0;
     }
     __syncthreads();
-    // transposed copy: (pu = p0 + 16 us + k, pd = 16 slab + cc) -> xt[((pu >> 4) dd + pd) 16 + (pu & 15)]
+    // transposed copy: (pu = p0 + 16 us + k, pd = 16 slab + cc) -> layout T,
+    // (pu >> 4) dd 16 + pd Wt + (pu & 15) with Wt the width of up slab pu >> 4
     const int us = tid >> 7, rem = tid & 127, cc = rem >> 3, k = (rem & 7) * 2;
     const int64_t pd = slab * kSW + cc, pu0 = p0 + us * kSW;
-    if (pd < a.dd && pu0 < a.du)
-        st2(xt + ((pu0 >> 4) * a.dd + pd) * kSW + k, s_x[us * kSW + k][cc], s_x[us * kSW + k + 1][cc]);
+    const int Wt = (pu0 >> 4) == a.nsu - 1 ? a.wt : kSW;
+    if (pd < a.dd && pu0 < a.du && k < Wt)
+        st2(xt + (pu0 >> 4) * a.dd * kSW + pd * Wt + k, s_x[us * kSW + k][cc], s_x[us * kSW + k + 1][cc]);
 }
 
 // Pass B option of the Ritz-vector rebuild, where a[step] is already known: y is written as
@@ -239,11 +258,13 @@ __global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, 
     const int64_t pend = (p0 + TR < a.dd) ? p0 + TR : a.dd;
     const int nrows = slab_stage<TR>(a, p0, pend, s_val, s_ent, s_off);
     const int tid = threadIdx.x;
-    // the tile's UP partial (and w) in layout L: 2 dn slabs x 16 up positions, 32 lines
+    // the tile's UP partial (and w) in layout L: TR/16 dn slabs x 16 up positions; a line of dn
+    // slab dsl has Wl entries (the last slab may be narrower)
     const int ds = tid >> 7, rem = tid & 127, pr = rem >> 3, k = (rem & 7) * 2;
     const int64_t pul = su * kSW + pr, dsl = (p0 >> 4) + ds;
-    const bool own = pul < a.du && dsl < a.nsd;
-    const int64_t li = (dsl * a.du + pul) * kSW + k;
+    const int Wl = dsl == a.nsd - 1 ? a.wl : kSW;
+    const bool own = pul < a.du && dsl < a.nsd && k < Wl;
+    const int64_t li = dsl * a.du * kSW + pul * Wl + k;
     {
         double2 zz = make_double2(0.0, 0.0), ww = make_double2(0.0, 0.0);
         if (own) {
@@ -258,12 +279,13 @@ __global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, 
     __syncthreads();
     const int r = tid >> 3, c = (tid & 7) * 2;
     const int64_t pd = p0 + r, pu = su * kSW + c;
+    const int Wu = su == a.nsu - 1 ? a.wt : kSW; // line width of this up slab in layout T
     double o0 = 0.0, o1 = 0.0, xd0 = 0.0, xd1 = 0.0, dot = 0.0;
-    if (r < nrows) {
+    if (r < nrows && c < Wu) {
         double acc0 = s_a[r][c], acc1 = s_a[r][c + 1];
         const double *xs = xt + su * a.dd * kSW;
-        slab_gather<false, NB>(xs, s_ent, s_val, s_off[r], s_off[r + 1], c, 1.0, acc0, acc1);
-        const double2 xo = *reinterpret_cast<const double2 *>(xs + pd * kSW + c);
+        slab_gather<false, NB>(xs, Wu, s_ent, s_val, s_off[r], s_off[r + 1], c, 1.0, acc0, acc1);
+        const double2 xo = *reinterpret_cast<const double2 *>(xs + pd * Wu + c);
         xd0 = xo.x;
         xd1 = xo.y;
         if (a.has_diag) {
@@ -317,8 +339,9 @@ __global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, 
 // never meets, so that the loads stay live.  Its time is what the gathers of that pass cost.
 template <int NB, int TR>
 __global__ __launch_bounds__(8 * TR) void k_slab_ceiling(SlabArgs a, int64_t ntiles, int64_t per_xcd, int64_t ntr,
-                                                         int64_t drows, const double *__restrict__ x,
-                                                         double *__restrict__ sink, int64_t nsink)
+                                                         int64_t drows, int64_t nslabs, int wlast,
+                                                         const double *__restrict__ x, double *__restrict__ sink,
+                                                         int64_t nsink)
 {
     __shared__ double s_val[4 * kMaxOps];
     __shared__ uint32_t s_ent[TR * kMaxOps];
@@ -334,14 +357,15 @@ __global__ __launch_bounds__(8 * TR) void k_slab_ceiling(SlabArgs a, int64_t nti
     __syncthreads();
     const int tid = threadIdx.x, r = tid >> 3, c = (tid & 7) * 2;
     const double *xs = x + slab * drows * kSW;
+    const int W = slab == nslabs - 1 ? wlast : kSW;
     double acc = 0.0;
-    if (r < nrows) {
+    if (r < nrows && c < W) {
         const int o0 = s_off[r], o1 = s_off[r + 1];
         for (int j0 = o0; j0 < o1; j0 += NB) {
             double2 xv[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k)
-                if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(xs + (int64_t)(s_ent[j0 + k] >> 8) * kSW + c);
+                if (j0 + k < o1) xv[k] = *reinterpret_cast<const double2 *>(xs + (int64_t)(s_ent[j0 + k] >> 8) * W + c);
 #pragma unroll
             for (int k = 0; k < NB; ++k)
                 if (j0 + k < o1) acc += xv[k].x + xv[k].y;
@@ -366,14 +390,17 @@ __global__ __launch_bounds__(256) void k_slab_chunks(int64_t n, const double *__
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// reference layout -> layout L (the narrow last slab's padding column, if any, written zero)
 __global__ __launch_bounds__(256) void k_slab_from_rm(SlabArgs a, const int32_t *__restrict__ u_at,
                                                       const int32_t *__restrict__ d_at, const double *__restrict__ x,
                                                       double *__restrict__ xl, int64_t len)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= len) return;
-    const int64_t slab = i / (a.du * kSW), rem = i - slab * a.du * kSW;
-    const int64_t p = rem >> 4, pd = slab * kSW + (rem & 15);
+    int64_t slab = i / (a.du * kSW);
+    if (slab > a.nsd - 1) slab = a.nsd - 1;
+    const int W = slab == a.nsd - 1 ? a.wl : kSW;
+    const int64_t rem = i - slab * a.du * kSW, p = rem / W, pd = slab * kSW + (rem - p * W);
     xl[i] = pd < a.dd ? x[(int64_t)u_at[p] * a.dd + d_at[pd]] : 0.0;
 }
 
@@ -384,8 +411,9 @@ __global__ __launch_bounds__(256) void k_slab_to_rm(SlabArgs a, const int32_t *_
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.du * a.dd) return;
     const int64_t u = i / a.dd, d = i - u * a.dd;
-    const int64_t pd = pos_d[d];
-    x[i] = xl[((pd >> 4) * a.du + pos_u[u]) * kSW + (pd & 15)];
+    const int64_t pd = pos_d[d], slab = pd >> 4;
+    const int W = slab == a.nsd - 1 ? a.wl : kSW;
+    x[i] = xl[slab * a.du * kSW + (int64_t)pos_u[u] * W + (pd & 15)];
 }
 
 static SlabArgs slab_args(const Hubbard &h, bool up)
@@ -407,10 +435,13 @@ static SlabArgs slab_args(const Hubbard &h, bool up)
     a.Uvec = h.otf.Uvec;
     a.has_diag = h.otf.has_diag;
     a.swz = L.swz;
+    a.wl = L.wl;
+    a.wt = L.wt;
     return a;
 }
 
 int64_t slab_len(const Hubbard &h) { return h.slab ? h.slab->len : 0; }
+int64_t slab_len_for(const Hubbard &h) { return slab_len_of(h.otf.dim_up, h.otf.dim_dn); }
 int64_t slab_num_partials(const Hubbard &h) { return h.slab ? (h.slab->npart + 2047) / 2048 : 0; }
 
 template <int NB, int TR>
@@ -586,8 +617,10 @@ static int slab_build(Hubbard &h)
     L->dd = dd;
     L->nsd = (dd + kSW - 1) / kSW;
     L->nsu = (du + kSW - 1) / kSW;
-    L->len = L->nsd * du * kSW;
-    L->xt_len = L->nsu * dd * kSW;
+    L->wl = last_slab_width(dd);
+    L->wt = last_slab_width(du);
+    L->len = slab_len_of(du, dd);
+    L->xt_len = slab_len_of(dd, du);
     L->nops_u = h.otf.up.nops;
     L->nops_d = h.otf.dn.nops;
     L->npart = L->nsu * ((dd + 31) / 32);
@@ -597,6 +630,7 @@ static int slab_build(Hubbard &h)
         (s = upload(&L->ptr_u, ptru)) || (s = upload(&L->ptr_d, ptrd)) || (s = upload(&L->ent_u, entu)) ||
         (s = upload(&L->ent_d, entd)) || (s = upload(&L->val_u, valu)) || (s = upload(&L->val_d, vald)) ||
         (s = dmalloc((void **)&L->xt, sizeof(double) * (size_t)L->xt_len)) ||
+        (s = hipMemset(L->xt, 0, sizeof(double) * (size_t)L->xt_len) == hipSuccess ? ED_OK : ED_EHIP) ||
         (s = dmalloc((void **)&L->part, sizeof(double) * (size_t)L->npart))) {
         slab_free(L);
         return s;
@@ -610,7 +644,6 @@ static int slab_build(Hubbard &h)
 // layout vectors the solver will allocate fit in free device memory (keeping 5 % of the card
 // free); ED_SLAB_AUTO=0 turns this off.  The layout stays in the handle
 // (ed_hubbard_slab_layout(h, 0, ...) frees it).
-static int64_t slab_len_of(int64_t du, int64_t dd) { return ((dd + kSW - 1) / kSW) * du * kSW; }
 
 static bool slab_auto_fits(double free_bytes, double total_bytes, int64_t du, int64_t dd, int nvec)
 {
@@ -685,7 +718,7 @@ static int slab_ceiling_t(const Hubbard &h, int pass, const double *d_xl, hipStr
     // pass 1 gathers over the layout-T scratch, as k_slab_dn does
     const double *src = pass == 0 ? d_xl : L.xt;
     hipLaunchKernelGGL((k_slab_ceiling<NB, TR>), dim3((unsigned)(per * 8)), dim3(8 * TR), 0, st, a, tiles, per, ntr,
-                       rows, src, L.part, L.npart);
+                       rows, slabs, pass == 0 ? L.wl : L.wt, src, L.part, L.npart);
     ED_CHECK_LAUNCH();
     return ED_OK;
 }

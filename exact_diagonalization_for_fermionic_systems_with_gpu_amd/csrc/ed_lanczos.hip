@@ -287,8 +287,9 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     // real sectors with a slab layout (ed_hubbard_slab_layout, or built here for sectors of 2^27
     // amplitudes and more) run the whole recurrence on it: the start vector is converted once and
     // the Ritz vector converted back at the end
-    // layout vectors of the slab path: w.buf[0..1], the copy of f and (with a Ritz vector) of evec
-    if (!cvec) slab_auto(*const_cast<Hubbard *>(h), want_vec ? 4 : 3);
+    // layout vectors the slab path allocates: w.buf[0..1], plus the copy of f and (with a Ritz
+    // vector) of evec unless the layout has no padding and the caller's f and evec carry them
+    if (!cvec) slab_auto(*const_cast<Hubbard *>(h), want_vec ? (slab_len_for(*h) == n ? 2 : 4) : 3);
     const bool slab = h->slab && !cvec;
     const int64_t nreal = slab ? slab_len(*h) : (cvec ? 2 * n : n);
 
@@ -303,7 +304,15 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         }
     } own;
     double *F = (double *)d_f, *ev = (double *)d_evec;
-    if (slab) {
+    // On a layout without padding (nreal == n) the caller's vectors carry layout vectors: evec
+    // holds the layout copy of f during the forward recurrence while f keeps the start vector
+    // (reference layout) for the rebuild, which then accumulates the Ritz vector in f's buffer.
+    // The solve then holds five vectors: f, evec, the two work vectors and the scratch copy.
+    const bool inplace = slab && want_vec && nreal == n;
+    if (inplace) {
+        ED_TRY(slab_from_rowmajor(*h, (const double *)d_f, (double *)d_evec, st));
+        F = (double *)d_evec;
+    } else if (slab) {
         // layout-L copies of f and evec; padding entries stay zero in every vector
         ED_TRY(dmalloc((void **)&own.p[0], sizeof(double) * (size_t)nreal));
         if (want_vec) ED_TRY(dmalloc((void **)&own.p[1], sizeof(double) * (size_t)nreal));
@@ -314,7 +323,8 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         if (want_vec) ev = own.p[1];
     }
     // evec = f keeps the start vector for the rebuild (HubMatrix_GPU.cuh:566)
-    if (want_vec) ED_HIP(hipMemcpyAsync(ev, F, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    if (want_vec && !inplace)
+        ED_HIP(hipMemcpyAsync(ev, F, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     int64_t nb;
     ED_TRY(dot_partials(nreal, F, F, w.partial, &nb, st));
     ED_TRY(finalize_sum(w.partial, nb, w.bdev, 1, st));
@@ -380,10 +390,18 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         return ED_OK;
     }
     // Ritz vector: rerun the recurrence from the saved start (HubMatrix_GPU.cuh:600-612)
-    A = F;
-    B = w.buf[0];
-    C = w.buf[1];
-    ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    if (inplace) {
+        A = w.buf[0];
+        B = w.buf[1];
+        C = (double *)d_evec;
+        ED_TRY(slab_from_rowmajor(*h, (const double *)d_f, A, st));
+        ev = (double *)d_f; // the start vector is in A now: f's buffer accumulates the Ritz vector
+    } else {
+        A = F;
+        B = w.buf[0];
+        C = w.buf[1];
+        ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+    }
     ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
     for (int i = 0; i < Nk + 1; ++i) {
         if (slab) ED_TRY(slab_lanczos_ritz(*h, A, B, C, w.bdev, w.adev, z[i], ev, i, st)); // update fused

@@ -65,6 +65,11 @@ typedef struct {
 } ed_op;
 
 const char *ed_last_error(void);
+/* Device memory the library itself holds (its own allocations on all devices: operator tables,
+ * layouts, solver work vectors, per-thread workspaces; not rocBLAS/rocSOLVER workspaces and not
+ * the caller's buffers): current and peak bytes since load or the last reset.  reset_peak != 0
+ * sets the peak to the current value after reading.  Host only. */
+int ed_memory_stats(int64_t *current_bytes, int64_t *peak_bytes, int reset_peak);
 int ed_version(void);
 int ed_device_sync(void);
 
@@ -174,8 +179,10 @@ int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const doubl
 /* The vector layout the Lanczos solver keeps internally for large real sectors: both species'
  * basis states renumbered in reverse Cuthill-McKee order of their hopping graphs (positions
  * pu, pd) and the state cut into slabs of 16 dn positions, amplitude (i_up, i_dn) at
- * ((pd >> 4)*dim_up + pu)*16 + (pd & 15); *len (>= dim_up*dim_dn) doubles per vector, padding
- * entries zero.  H.v on it (ed_hubbard_apply_slab) gives the same values bit for bit as
+ * (pd >> 4)*dim_up*16 + pu*W + (pd & 15) with W = 16 except in the last slab, which is only as
+ * wide as the remaining dn positions (rounded up to even): *len = dim_up*dim_dn for even dim_dn
+ * (else dim_up more);
+ * padding entries zero.  H.v on it (ed_hubbard_apply_slab) gives the same values bit for bit as
  * ed_hubbard_apply (HubMatrix_GPU.cuh:114-182) on the reference layout, with 16 B gathers that
  * stay in L2.  enable = 1 builds the layout tables and a scratch vector (len doubles) in the
  * handle, after which ed_lanczos_eigen on real vectors runs its recurrence on this layout
@@ -187,8 +194,9 @@ int ed_ted_time_evol(int64_t n, int complex_values, const void *d_V, const doubl
  * only reports *len (0 when there is no layout). */
 int ed_hubbard_slab_layout(ed_hubbard h, int enable, int64_t *len);
 /* The solvers' decision above, host only: 1 when a dim_up x dim_dn real sector would get the slab
- * layout with free_bytes of total_bytes free and nvec solver vectors on the layout (eigen with a
- * Ritz vector: 4; eigenvalue only or tridiagonal: 3), else 0. */
+ * layout with free_bytes of total_bytes free and nvec more layout vectors for the solver
+ * (eigen with a Ritz vector: 2 when dim_dn is even, the layout then has no padding and the
+ * caller's f and evec carry layout vectors, else 4; eigenvalue only or tridiagonal: 3), else 0. */
 int ed_slab_auto_decision(double free_bytes, double total_bytes, int64_t dim_up, int64_t dim_dn, int nvec);
 /* reference layout psi[i_up*dim_dn + i_dn] -> slab layout (padding set to zero), and back */
 int ed_slab_from_rowmajor(ed_hubbard h, const double *d_x, double *d_xl, void *stream);

@@ -15,7 +15,9 @@ def decide(free, du, dd, nvec, total=CARD):
 
 
 def slab_len(du, dd):
-    return ((dd + 15) // 16) * du * 16
+    """16-wide slabs of dn positions, the last one as wide as the remainder (rounded up to even)"""
+    r = dd - ((dd + 15) // 16 - 1) * 16
+    return ((dd + 15) // 16 - 1) * du * 16 + du * (r + (r & 1))
 
 
 def test_small_sectors_stay_on_reference_layout():
@@ -39,12 +41,22 @@ def test_18_site_ground_state_fits():
     assert decide(4 * V + 0.05 * CARD, d, d, 3) == 1
 
 
-def test_19_site_8_8_falls_back():
-    """19 sites, 8/8: V = 45.7 GB.  With f and evec resident, 197 GB are free: enough for the
-    reference layout's two work vectors, not for the slab path's five vectors."""
+def test_layout_length_has_no_padding_for_even_dims():
+    for d in (comb(18, 9), comb(19, 8), comb(12, 6), 70, 48):
+        assert slab_len(d, d) == d * d
+    assert slab_len(5, comb(8, 0) * 3) == 5 * 4        # 3 dn positions: one padding column
+    assert slab_len(7, 33) == 7 * 34
+
+
+def test_19_site_8_8():
+    """19 sites, 8/8: V = 45.7 GB.  With f and evec resident, 197 GB are free.  The slab path with
+    its own copies of f and evec (four layout vectors and the scratch) does not fit and the solver
+    stays on the reference layout (two work vectors); with the caller's f and evec carrying layout
+    vectors (two more plus the scratch: five vectors in all) it fits."""
     d = comb(19, 8)
     V = 8.0 * d * d
     free = CARD - 2 * V
     assert free > 2 * V
     assert decide(free, d, d, 4) == 0
-    assert decide(free, d, d, 3) == 0 or 4 * 8.0 * slab_len(d, d) + 0.05 * CARD <= free
+    assert decide(free, d, d, 2) == 1
+    assert 5 * V < CARD
