@@ -110,6 +110,41 @@ int lanczos_update(int64_t nreal, const double *v, double *f, const double *adev
     return ED_OK;
 }
 
+// y = -a[step]*(f/b[step]) + y ; partial sum y^2 -- k_lanczos_update with v = (1/b[step]) f, the
+// product the H.v pass forms when it scales f, so the same bits as with a stored v
+__global__ __launch_bounds__(kBB) void k_lanczos_update_vf(int64_t n, const double *__restrict__ f,
+                                                           double *__restrict__ y, const double *__restrict__ adev,
+                                                           const double *__restrict__ bdev, int step,
+                                                           double *__restrict__ partial)
+{
+    __shared__ double red[kBB / 64];
+    const double c = -adev[step], sc = 1.0 / bdev[step];
+    const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kChunk / kBB; ++k) {
+        const int64_t i = base + (int64_t)k * kBB;
+        if (i < n) {
+            const double r = fma(c, sc * f[i], y[i]);
+            y[i] = r;
+            s = fma(r, r, s);
+        }
+    }
+    s = block_sum<kBB>(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+int lanczos_update_vfree(int64_t nreal, const double *f, double *y, const double *adev, const double *bdev, int step,
+                         double *partial, int64_t *nblocks, hipStream_t st)
+{
+    const int64_t nb = blas_num_blocks(nreal);
+    if (nblocks) *nblocks = nb;
+    hipLaunchKernelGGL(k_lanczos_update_vf, dim3((unsigned)nb), dim3(kBB), 0, st, nreal, f, y, adev, bdev, step,
+                       partial);
+    ED_CHECK_LAUNCH();
+    return ED_OK;
+}
+
 // f = -a[step]*v + f ; evec = z*v + evec
 __global__ __launch_bounds__(kBB) void k_ritz_update(int64_t n, const double *__restrict__ v, double *__restrict__ f,
                                                      double *__restrict__ evec, const double *__restrict__ adev,

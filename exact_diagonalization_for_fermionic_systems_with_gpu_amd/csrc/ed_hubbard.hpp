@@ -131,6 +131,15 @@ int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const 
 int slab_lanczos_ritz(const Hubbard &h, const double *f, double *y, double *v, const double *bdev,
                       const double *adev, double zr, double *evec, int step, hipStream_t st);
 int64_t slab_num_partials(const Hubbard &h);
+// the same steps keeping no v: the previous f stands for v_{N-1} (scaled by 1 / b[step - 1] where
+// it is read) and y holds the UP partial before the result; bit-identical recurrence
+int slab_lanczos_vfree(const Hubbard &h, const double *f, const double *f_prev, double *y, const double *bdev,
+                       int step, double *partial, hipStream_t st, int64_t *nblocks);
+int slab_lanczos_ritz_vfree(const Hubbard &h, const double *f, const double *f_prev, double *y, const double *bdev,
+                            const double *adev, double zr, double *evec, int step, hipStream_t st);
+// f_next = y - a[step] f / b[step] (in y) and the partials of |f_next|^2 (lanczos_update with v = f / b)
+int lanczos_update_vfree(int64_t nreal, const double *f, double *y, const double *adev, const double *bdev, int step,
+                         double *partial, int64_t *nblocks, hipStream_t st);
 int slab_from_rowmajor(const Hubbard &h, const double *x, double *xl, hipStream_t st);
 int slab_to_rowmajor(const Hubbard &h, const double *xl, double *x, hipStream_t st);
 // builds the slab layout for a large real sector when it and nvec more layout vectors fit

@@ -236,6 +236,10 @@ __global__ __launch_bounds__(8 * TR) void k_slab_up(SlabArgs a, int64_t ntiles, 
 // f = y - a[step] v and evec += zr v, the arithmetic of ritz_update (ed_blas.hip) element for
 // element, so the rebuilt Lanczos vectors are the forward recurrence's bit for bit.
 enum { SLAB_RITZ = 16 };
+// Pass B option of the recurrence that keeps no v: w is read as f_{N-1} and scaled by
+// 1 / b[step - 1] on the fly -- the product pass A formed when it scaled x, so w is the v the
+// previous step would have written, bit for bit.
+enum { SLAB_WSCALE = 32 };
 
 // Pass B: y = zin + DN term of xt + diag * xt (+ c w); v = xt (layout L); block partial <v, y>
 template <int FLAGS, int NB, int TR>
@@ -270,6 +274,11 @@ __global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, 
         if (own) {
             zz = ld2(zin + li);
             if (FLAGS & HV_AXPY) ww = ld2(win + li);
+            if (FLAGS & SLAB_WSCALE) {
+                const double sw = 1.0 / bdev[step - 1];
+                ww.x = sw * ww.x;
+                ww.y = sw * ww.y;
+            }
         }
         s_a[ds * kSW + k][pr] = zz.x;
         s_a[ds * kSW + k + 1][pr] = zz.y;
@@ -472,6 +481,10 @@ static int slab_apply_t(const Hubbard &h, const double *x, double *z, const doub
     case HV_WRITEV | HV_AXPY | HV_DOT: ED_SLAB_B(HV_WRITEV | HV_AXPY | HV_DOT); break;
     case HV_WRITEV | SLAB_RITZ: ED_SLAB_B(HV_WRITEV | SLAB_RITZ); break;
     case HV_WRITEV | HV_AXPY | SLAB_RITZ: ED_SLAB_B(HV_WRITEV | HV_AXPY | SLAB_RITZ); break;
+    case HV_DOT: ED_SLAB_B(HV_DOT); break;
+    case HV_AXPY | HV_DOT | SLAB_WSCALE: ED_SLAB_B(HV_AXPY | HV_DOT | SLAB_WSCALE); break;
+    case SLAB_RITZ: ED_SLAB_B(SLAB_RITZ); break;
+    case HV_AXPY | SLAB_RITZ | SLAB_WSCALE: ED_SLAB_B(HV_AXPY | SLAB_RITZ | SLAB_WSCALE); break;
     default: return fail(ED_EINVAL, "slab H.v: unsupported flags %d", flags);
     }
 #undef ED_SLAB_B
@@ -504,6 +517,24 @@ int slab_lanczos(const Hubbard &h, const double *f, double *y, double *v, const 
     int flags = HV_SCALE | HV_WRITEV | (step > 0 ? HV_AXPY : 0) | (dot ? HV_DOT : 0);
     // the UP partial goes into v (overwritten by pass B with v itself), w and y share a buffer
     return slab_apply(h, f, v, w, y, v, bdev, step, flags, partial, st, nblocks);
+}
+
+// The recurrence without v (f_{N-1} stands for v_{N-1}): x = f / b[step] (layout L) -> y = H v -
+// b[step] f_prev / b[step - 1] in y, which also holds the UP partial; f_prev is not written.
+int slab_lanczos_vfree(const Hubbard &h, const double *f, const double *f_prev, double *y, const double *bdev,
+                       int step, double *partial, hipStream_t st, int64_t *nblocks)
+{
+    const int flags = HV_SCALE | HV_DOT | (step > 0 ? HV_AXPY | SLAB_WSCALE : 0);
+    return slab_apply(h, f, y, f_prev, y, nullptr, bdev, step, flags, partial, st, nblocks);
+}
+
+// one step of the Ritz-vector rebuild without v: f_{i+1} = y - a[i] v_i lands in y, evec += zr v_i
+int slab_lanczos_ritz_vfree(const Hubbard &h, const double *f, const double *f_prev, double *y, const double *bdev,
+                            const double *adev, double zr, double *evec, int step, hipStream_t st)
+{
+    if (!evec || !adev) return fail(ED_EINVAL, "slab Ritz step: null vector");
+    const int flags = HV_SCALE | SLAB_RITZ | (step > 0 ? HV_AXPY | SLAB_WSCALE : 0);
+    return slab_apply(h, f, y, f_prev, y, nullptr, bdev, step, flags, nullptr, st, nullptr, evec, adev, zr);
 }
 
 int slab_lanczos_ritz(const Hubbard &h, const double *f, double *y, double *v, const double *bdev,

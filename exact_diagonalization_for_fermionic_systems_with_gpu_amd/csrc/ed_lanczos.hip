@@ -229,6 +229,21 @@ static int lanczos_step(const Hubbard &h, bool cvec, int64_t nreal, double *A, d
     return ED_OK;
 }
 
+// One step on the slab layout without v: (A = f_N, B = f_{N-1}) -> C = f_{N+1}, a[N], b[N+1] on the
+// device.  v_N = f_N / b[N] is formed where it is read (the H.v pass and the update), exactly as
+// the step with a stored v forms it, so a, b and the vectors are the same bit for bit; the step
+// writes one vector less.  Caller's rotation: (A, B, C) <- (C, A, B).
+static int lanczos_step_vfree(const Hubbard &h, int64_t nreal, double *A, double *B, double *C, Work &w, int N,
+                              hipStream_t st)
+{
+    int64_t nb;
+    ED_TRY(slab_lanczos_vfree(h, A, B, C, w.bdev, N, w.partial, st, &nb));
+    ED_TRY(finalize_sum(w.partial, nb, w.adev + N, 0, st));
+    ED_TRY(lanczos_update_vfree(nreal, A, C, w.adev, w.bdev, N, w.partial, &nb, st));
+    ED_TRY(finalize_sum(w.partial, nb, w.bdev + N + 1, 1, st));
+    return ED_OK;
+}
+
 } // namespace ed
 
 using namespace ed;
@@ -336,12 +351,21 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     double *A = F, *B = w.buf[0], *C = w.buf[1];
     int N;
     bool stop = false;
+    const bool vfree = slab && !reorth; // reorthogonalisation works on a stored v
     for (N = 0; N < maxN && !stop; ++N) {
-        ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, slab, st));
-        double *t = A;
-        A = B;
-        B = C;
-        C = t;
+        if (vfree) {
+            ED_TRY(lanczos_step_vfree(*h, nreal, A, B, C, w, N, st));
+            double *t = B;
+            B = A;
+            A = C;
+            C = t;
+        } else {
+            ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, slab, st));
+            double *t = A;
+            A = B;
+            B = C;
+            C = t;
+        }
         if (N % every == every - 1 || N == maxN - 1) {
             ED_HIP(hipMemcpyAsync(araw.data(), w.adev, sizeof(double) * (size_t)(N + 1), hipMemcpyDeviceToHost, st));
             ED_HIP(hipMemcpyAsync(b.data(), w.bdev, sizeof(double) * (size_t)(N + 2), hipMemcpyDeviceToHost, st));
@@ -404,6 +428,14 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     }
     ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
     for (int i = 0; i < Nk + 1; ++i) {
+        if (vfree) { // update fused, no v
+            ED_TRY(slab_lanczos_ritz_vfree(*h, A, B, C, w.bdev, w.adev, z[i], ev, i, st));
+            double *t = B;
+            B = A;
+            A = C;
+            C = t;
+            continue;
+        }
         if (slab) ED_TRY(slab_lanczos_ritz(*h, A, B, C, w.bdev, w.adev, z[i], ev, i, st)); // update fused
         else {
             ED_TRY(hv_lanczos_nodot(*h, cvec, A, B, C, B, w.bdev, i, st));
@@ -470,11 +502,19 @@ extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int ns
         if ((s = finalize_sum(w.partial, nb, w.bdev, 1, st))) break;
         double *A = f, *B = w.buf[0], *C = w.buf[1];
         for (int N = 0; N < nsteps && s == ED_OK; ++N) {
-            s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, slab, st);
-            double *t = A;
-            A = B;
-            B = C;
-            C = t;
+            if (slab) {
+                s = lanczos_step_vfree(*h, nreal, A, B, C, w, N, st);
+                double *t = B;
+                B = A;
+                A = C;
+                C = t;
+            } else {
+                s = lanczos_step(*h, cvec, nreal, A, B, C, w, N, false, slab, st);
+                double *t = A;
+                A = B;
+                B = C;
+                C = t;
+            }
         }
         if (s) break;
         if (hipMemcpyAsync(a_out, w.adev, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost, st) != hipSuccess ||
