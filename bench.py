@@ -209,12 +209,16 @@ def batched_drude_line(with_cpu=False):
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import batched as BT
     gold = json.load(open(os.path.join(REPO, "tests", "golden", "lieb12_drude_oracle.json")))
     Dg = np.array(gold["D"])
-    BT.lieb12_drude_sweep(batched=True)  # warm-up (allocations, graph capture paths)
+    # panel shape of the batch kernels: autotuned once on the sweep's 208 instances (untimed)
+    tune = BT.BatchedLanczos([h for (_, _, h) in BT.lieb12_instances(4.0, 16)], True).autotune(steps=30)
+    shape = (tune["dn_rows"], tune["dn_panels"], tune["up_rows"], tune["up_panels"])
+    log(f"[bench] batch autotune: {tune}")
+    BT.lieb12_drude_sweep(batched=True, shape=shape)  # warm-up (allocations, graph capture paths)
     BT.lieb12_drude_sweep(batched=False)
     # three timed sweeps of each, alternating; the fastest of each is reported
     runs_b, runs_p = [], []
     for _ in range(3):
-        runs_b.append(BT.lieb12_drude_sweep(batched=True))
+        runs_b.append(BT.lieb12_drude_sweep(batched=True, shape=shape))
         runs_p.append(BT.lieb12_drude_sweep(batched=False))
     E0b, Db, tb, _ = min(runs_b, key=lambda r: r[2])
     E0p, Dp, tp, _ = min(runs_p, key=lambda r: r[2])
@@ -246,6 +250,7 @@ def batched_drude_line(with_cpu=False):
             "cpu_baseline": cpu,
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
             "max_abs_e0_batched_minus_plain": e0_diff, "max_abs_e0_flux_asymmetry": e0_asym,
+            "batch_shape": {k: tune[k] for k in ("dn_rows", "dn_panels", "up_rows", "up_panels")},
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 

@@ -42,6 +42,11 @@ class SlabTuning(C.Structure):
                 ("ms_per_apply", C.c_double), ("candidate_ms", C.c_double * 8)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("dn_rows", C.c_int), ("dn_panels", C.c_int), ("up_rows", C.c_int), ("up_panels", C.c_int),
+                ("steps", C.c_int), ("seconds", C.c_double), ("candidate_s", C.c_double * 18)]
+
+
 class EdError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"libedhip status {status}: {msg}")
@@ -96,6 +101,8 @@ SIGNATURES = {
     "ed_batch_create": (_i32, [_i32, _p, _i32, C.POINTER(_p)]),
     "ed_batch_destroy": (_i32, [_p]),
     "ed_batch_lowest": (_i32, [_p, C.POINTER(LanczosCfg), C.c_uint32, _i32, _pd, _pi, _pi, _p]),
+    "ed_batch_set_shape": (_i32, [_p, _i32, _i32, _i32, _i32]),
+    "ed_batch_autotune": (_i32, [_p, _i32, C.POINTER(BatchTuning), _p]),
     "ed_bond_bilinear": (_i32, [_p, _i32, _i32, _p, _p, _pd, _p]),
     "ed_site_double_occupancy": (_i32, [_p, _i32, _p, _p, _pd, _p]),
     "ed_site_density": (_i32, [_p, _i32, _p, _p, _pd, _p]),

@@ -47,6 +47,18 @@ class BatchedLanczos:
         except Exception:
             pass
 
+    def set_shape(self, dn_rows=4, dn_panels=4, up_rows=1, up_panels=4):
+        """panel shape of the batch kernels (results do not depend on it)"""
+        call("ed_batch_set_shape", self._b, dn_rows, dn_panels, up_rows, up_panels)
+
+    def autotune(self, steps=30) -> dict:
+        """time the panel shapes on a solve of `steps` steps and keep the fastest; returns the choice"""
+        from ._lib import BatchTuning
+        t = BatchTuning()
+        call("ed_batch_autotune", self._b, int(steps), C.byref(t), stream_ptr())
+        return {"dn_rows": t.dn_rows, "dn_panels": t.dn_panels, "up_rows": t.up_rows, "up_panels": t.up_panels,
+                "seconds": t.seconds, "steps": t.steps, "candidate_s": [round(v, 4) for v in t.candidate_s]}
+
     def lowest(self, max_steps=1000, tol=None, use_graph=True, seed=235):
         n = len(self.hubbards)
         cfg = LanczosCfg(max_steps, 0.0 if tol is None else tol, 10, 0)
@@ -104,14 +116,18 @@ def lieb12_instances(U=4.0, nflux=16, t=1.0, n_total=12, sectors=None, fluxes=No
     return out
 
 
-def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=True, max_steps=1000, tol=None):
-    """All 13 sectors x nflux fluxes.  Returns (E0[13, nflux], D[13], seconds, nsteps)."""
+def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=True, max_steps=1000, tol=None, shape=None):
+    """All 13 sectors x nflux fluxes.  Returns (E0[13, nflux], D[13], seconds, nsteps).  shape: the
+    batch kernels' panel shape (dn_rows, dn_panels, up_rows, up_panels), e.g. an autotuned one."""
     inst = lieb12_instances(U, nflux)
     hs = [h for (_, _, h) in inst]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if batched:
-        ev, ns, st = BatchedLanczos(hs, True).lowest(max_steps, tol, use_graph)
+        bl = BatchedLanczos(hs, True)
+        if shape is not None:
+            bl.set_shape(*shape)
+        ev, ns, st = bl.lowest(max_steps, tol, use_graph)
     else:
         ev, ns = plain_lowest(hs, True, max_steps, tol)
     torch.cuda.synchronize()

@@ -24,6 +24,7 @@
 // Each copy follows the recurrence and convergence test of ed_lanczos_eigen (eigenvalue only),
 // with its own instance's start vector (GenRand of the sector length).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -53,7 +54,8 @@ struct GGroup {
     int64_t t1_off, t1_n;    // UP tiles (GU rows x CW columns, slab-major)
     int64_t t2_off, t2_n;    // DN tiles (GR rows x CW columns)
     int64_t c_off, c_n;      // update chunks (GCH * CW elements)
-    int64_t p_off;           // partials: max(t2_n, c_n) * G
+    int64_t p_off;           // partials: max(nu_dn, c_n) * G
+    int64_t nu_dn;           // DN dot partials per copy: 4-row units x column blocks, any panel shape
 };
 
 
@@ -190,74 +192,86 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
     __syncthreads();
     const int gi = g.cbase + k;
     const bool live = k < g.ncopy && d < dd && active[gi];
-    double dot = 0.0;
-    // kp panels of GR rows share the column descriptors and the value table
-    for (int pn = 0; pn < kp && live; ++pn) {
+    // kp panels of GR rows share the column descriptors and the value table; every thread walks
+    // the panels (the per-unit sums below take barriers), threads without an element add nothing
+    for (int pn = 0; pn < kp; ++pn) {
         const int64_t u0 = (rp * kp + pn) * GR;
-        if (u0 >= du) break;
-        const int step = *dstep;
-        const double bN = bdev[(int64_t)step * ldb + gi], sc = 1.0 / bN;
-        const double scp = AXPY ? 1.0 / bdev[(int64_t)(step - 1) * ldb + gi] : 0.0;
-        const V *xv = reinterpret_cast<const V *>(x + g.voff);
-        const V *xpv = reinterpret_cast<const V *>(xp + g.voff);
-        V *yy = reinterpret_cast<V *>(y + g.voff);
-        // 32-bit element offsets inside the group (ed_batch_create checks nelem * G < 2^31):
-        // row r's amplitude of column col and copy k sits at ((rowbase[r] + col) << lg) + k
-        const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k;
-        uint32_t rowbase[GR];
+        if (u0 >= du) break; // uniform over the workgroup
+        double dotu[GR / 4];
 #pragma unroll
-        for (int r = 0; r < GR; ++r) rowbase[r] = (uint32_t)((u0 + r < du) ? u0 + r : u0) * ddu;
-        V acc[GR];
+        for (int q = 0; q < GR / 4; ++q) dotu[q] = 0.0;
+        if (live) {
+            const int step = *dstep;
+            const double bN = bdev[(int64_t)step * ldb + gi], sc = 1.0 / bN;
+            const double scp = AXPY ? 1.0 / bdev[(int64_t)(step - 1) * ldb + gi] : 0.0;
+            const V *xv = reinterpret_cast<const V *>(x + g.voff);
+            const V *xpv = reinterpret_cast<const V *>(xp + g.voff);
+            V *yy = reinterpret_cast<V *>(y + g.voff);
+            // 32-bit element offsets inside the group (ed_batch_create checks nelem * G < 2^31):
+            // row r's amplitude of column col and copy k sits at ((rowbase[r] + col) << lg) + k
+            const uint32_t ddu = (uint32_t)dd, lg = (uint32_t)g.lg, ku = (uint32_t)k;
+            uint32_t rowbase[GR];
 #pragma unroll
-        for (int r = 0; r < GR; ++r)
-            acc[r] = (g.has_up && u0 + r < du) ? yy[((rowbase[r] + (uint32_t)d) << lg) + ku] : VO<V>::zero();
-        // no branches around the gathers, so the KB x GR loads of a chunk are all in flight: a
-        // column without an element of a call loads its own amplitude (contiguous with its
-        // neighbours' and needed below anyway) and adds 0 times it, which leaves the sum
-        // unchanged bit for bit (finite x; the sums start at +0 or the pass-1 value, never -0)
-        constexpr int KB = 4;
-        const TV tzero = tv_make<TV>(0.0, 0.0);
-        for (int o0 = 0; o0 < nops; o0 += KB) {
-            uint32_t col[KB];
-            TV av[KB];
-            V xl[KB][GR];
+            for (int r = 0; r < GR; ++r) rowbase[r] = (uint32_t)((u0 + r < du) ? u0 + r : u0) * ddu;
+            V acc[GR];
 #pragma unroll
-            for (int q = 0; q < KB; ++q) {
-                const uint32_t e = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
-                const bool on = e != 0xffffffffu;
-                col[q] = on ? (e >> 8) : (uint32_t)d;
-                const TV a = gval<TV>(s_vt, g.G, on ? (e & 0xffu) : 0u, k);
-                av[q] = on ? a : tzero;
+            for (int r = 0; r < GR; ++r)
+                acc[r] = (g.has_up && u0 + r < du) ? yy[((rowbase[r] + (uint32_t)d) << lg) + ku] : VO<V>::zero();
+            // no branches around the gathers, so the KB x GR loads of a chunk are all in flight: a
+            // column without an element of a call loads its own amplitude (contiguous with its
+            // neighbours' and needed below anyway) and adds 0 times it, which leaves the sum
+            // unchanged bit for bit (finite x; the sums start at +0 or the pass-1 value, never -0)
+            constexpr int KB = 4;
+            const TV tzero = tv_make<TV>(0.0, 0.0);
+            for (int o0 = 0; o0 < nops; o0 += KB) {
+                uint32_t col[KB];
+                TV av[KB];
+                V xl[KB][GR];
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const uint32_t e = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
+                    const bool on = e != 0xffffffffu;
+                    col[q] = on ? (e >> 8) : (uint32_t)d;
+                    const TV a = gval<TV>(s_vt, g.G, on ? (e & 0xffu) : 0u, k);
+                    av[q] = on ? a : tzero;
+                }
+#pragma unroll
+                for (int q = 0; q < KB; ++q)
+#pragma unroll
+                    for (int r = 0; r < GR; ++r) xl[q][r] = xv[((rowbase[r] + col[q]) << lg) + ku];
+#pragma unroll
+                for (int q = 0; q < KB; ++q)
+#pragma unroll
+                    for (int r = 0; r < GR; ++r) acc[r] = omac(av[q], VO<V>::scale(sc, xl[q][r]), acc[r]);
             }
 #pragma unroll
-            for (int q = 0; q < KB; ++q)
-#pragma unroll
-                for (int r = 0; r < GR; ++r) xl[q][r] = xv[((rowbase[r] + col[q]) << lg) + ku];
-#pragma unroll
-            for (int q = 0; q < KB; ++q)
-#pragma unroll
-                for (int r = 0; r < GR; ++r) acc[r] = omac(av[q], VO<V>::scale(sc, xl[q][r]), acc[r]);
+            for (int r = 0; r < GR; ++r) {
+                const int64_t u = u0 + r;
+                if (u >= du) continue;
+                const uint32_t i = ((rowbase[r] + (uint32_t)d) << lg) + ku;
+                const V xd = VO<V>::scale(sc, xv[i]);
+                if (g.a.has_diag) acc[r] = VO<V>::fmr(diag_of(g.a.state_up[u], (int32_t)sd, g.a), xd, acc[r]);
+                V o = acc[r];
+                if (AXPY) o = VO<V>::fmr(-bN, VO<V>::scale(scp, xpv[i]), o);
+                yy[i] = o;
+                dotu[r / 4] += VO<V>::dotre(xd, o);
+            }
         }
+        // per-copy sums of each 4-row unit in a fixed order, thread (c, k) -> s_red[c * G + k], into
+        // partial slot (unit * ncb + cb): the same slots and values whatever GR and kp are
 #pragma unroll
-        for (int r = 0; r < GR; ++r) {
-            const int64_t u = u0 + r;
-            if (u >= du) continue;
-            const uint32_t i = ((rowbase[r] + (uint32_t)d) << lg) + ku;
-            const V xd = VO<V>::scale(sc, xv[i]);
-            if (g.a.has_diag) acc[r] = VO<V>::fmr(diag_of(g.a.state_up[u], (int32_t)sd, g.a), xd, acc[r]);
-            V o = acc[r];
-            if (AXPY) o = VO<V>::fmr(-bN, VO<V>::scale(scp, xpv[i]), o);
-            yy[i] = o;
-            dot += VO<V>::dotre(xd, o);
+        for (int q = 0; q < GR / 4; ++q) {
+            const int64_t unit = u0 / 4 + q;
+            if (unit * 4 >= du) break; // uniform
+            s_red[threadIdx.x] = dotu[q];
+            __syncthreads();
+            if (threadIdx.x < g.G) {
+                double sm = 0.0;
+                for (int cc = 0; cc < g.CW; ++cc) sm += s_red[cc * g.G + threadIdx.x];
+                partial[g.p_off + (unit * g.ncb + cb) * g.G + threadIdx.x] = sm;
+            }
+            __syncthreads();
         }
-    }
-    // per-copy block sums in a fixed order: thread (c, k) -> s_red[c * G + k]
-    s_red[threadIdx.x] = dot;
-    __syncthreads();
-    if (threadIdx.x < g.G) {
-        double s = 0.0;
-        for (int cc = 0; cc < g.CW; ++cc) s += s_red[cc * g.G + threadIdx.x];
-        partial[g.p_off + tl * g.G + threadIdx.x] = s;
     }
 }
 
@@ -313,7 +327,7 @@ __global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ 
     if (!active[gi]) return;
     const GGroup &g = gr[cgroup[gi]];
     const int k = gi - g.cbase;
-    const int64_t np = which == 0 ? g.t2_n : g.c_n;
+    const int64_t np = which == 0 ? g.nu_dn : g.c_n;
     double s = 0.0;
     for (int64_t t = threadIdx.x; t < np; t += 1024) s += partial[g.p_off + t * g.G + k];
     s = block_sum<1024>(s, red);
@@ -361,8 +375,8 @@ struct Batch {
     int64_t total_real = 0, t1 = 0, t2 = 0, tc = 0, pcap = 0;
     int maxnops_dn = 1;
     size_t lds_up = 0, lds_dn = 0; // dynamic LDS of the UP / DN kernels (largest group)
-    int gr = 4, gu = 1;            // DN / UP rows per panel (ED_BATCH_GR, ED_BATCH_GU)
-    int kp = 4, kpu = 4;           // DN / UP panels per workgroup (ED_BATCH_KP, ED_BATCH_KPU)
+    int gr = 4, gu = 1;            // DN / UP rows per panel (4 or 8 / 1 or 4)
+    int kp = 4, kpu = 4;           // DN / UP panels per workgroup (ed_batch_autotune, ed_batch_set_shape)
     ~Batch()
     {
         dfree(dgr);
@@ -391,6 +405,45 @@ static bool same_structure(const Hubbard *a, const Hubbard *b, const std::vector
     return eq(ua, ub) && eq(da, db);
 }
 
+// Tile tables of the UP and DN grids for the current panel shape (gu x kpu UP rows, gr x kp DN rows
+// per workgroup): each group's tile range, the tile -> group maps and the partial-sum slots; then
+// the group descriptors are uploaded.  Rebuilt when the shape changes.
+static int batch_tiles(Batch *B)
+{
+    int64_t t1 = 0, t2 = 0, poff = 0;
+    for (GGroup &g : B->host) {
+        const int64_t du = g.a.dim_up;
+        g.t1_off = t1;
+        g.t1_n = ((du + (int64_t)B->gu * B->kpu - 1) / ((int64_t)B->gu * B->kpu)) * g.ncb;
+        t1 += g.t1_n;
+        g.t2_off = t2;
+        g.t2_n = ((du + (int64_t)B->gr * B->kp - 1) / ((int64_t)B->gr * B->kp)) * g.ncb;
+        t2 += g.t2_n;
+        g.nu_dn = ((du + 3) / 4) * g.ncb;
+        g.p_off = poff;
+        poff += std::max(g.nu_dn, g.c_n) * g.G;
+    }
+    B->t1 = t1;
+    B->t2 = t2;
+    B->pcap = poff;
+    std::vector<int> tg1((size_t)t1), tg2((size_t)t2);
+    for (int gq = 0; gq < B->ng; ++gq) {
+        const GGroup &g = B->host[gq];
+        std::fill(tg1.begin() + g.t1_off, tg1.begin() + g.t1_off + g.t1_n, gq);
+        std::fill(tg2.begin() + g.t2_off, tg2.begin() + g.t2_off + g.t2_n, gq);
+    }
+    dfree(B->dtg1);
+    dfree(B->dtg2);
+    B->dtg1 = B->dtg2 = nullptr;
+    ED_TRY(dmalloc((void **)&B->dtg1, sizeof(int) * std::max<int64_t>(t1, 1)));
+    ED_TRY(dmalloc((void **)&B->dtg2, sizeof(int) * std::max<int64_t>(t2, 1)));
+    if (hipMemcpy(B->dtg1, tg1.data(), sizeof(int) * t1, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dtg2, tg2.data(), sizeof(int) * t2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->dgr, B->host.data(), sizeof(GGroup) * B->ng, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ED_EHIP, "batch: tile table upload failed");
+    return ED_OK;
+}
+
 } // namespace ed
 
 using namespace ed;
@@ -401,10 +454,6 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
     *out = nullptr;
     Batch *B = new Batch();
     B->n = ninst;
-    if (const char *e = getenv("ED_BATCH_GR")) B->gr = atoi(e) == 8 ? 8 : 4;
-    if (const char *e = getenv("ED_BATCH_GU")) B->gu = atoi(e) == 4 ? 4 : 1;
-    if (const char *e = getenv("ED_BATCH_KP")) B->kp = std::max(1, std::min(16, atoi(e)));
-    if (const char *e = getenv("ED_BATCH_KPU")) B->kpu = std::max(1, std::min(64, atoi(e)));
     B->cvec = complex_vectors != 0;
     std::vector<const Hubbard *> hs(ninst);
     std::vector<std::vector<OtfOp>> opu(ninst), opd(ninst);
@@ -454,7 +503,7 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
     const size_t tv = B->ctab ? 2 : 1; // doubles per value
     std::vector<double> vt;
     std::vector<int64_t> vt_up_off, vt_dn_off;
-    int64_t voff = 0, t1 = 0, t2 = 0, tc = 0, poff = 0;
+    int64_t voff = 0, tc = 0;
     B->copy_of.assign(ninst, -1);
     for (const auto &mem : groups) {
         const Hubbard *h = hs[mem[0]];
@@ -490,17 +539,9 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
         g.voff = voff;
         voff += (nreal + 31) / 32 * 32;
         g.ncb = (dd + g.CW - 1) / g.CW;
-        g.t1_off = t1;
-        g.t1_n = ((du + (int64_t)B->gu * B->kpu - 1) / ((int64_t)B->gu * B->kpu)) * g.ncb;
-        t1 += g.t1_n;
-        g.t2_off = t2;
-        g.t2_n = ((du + (int64_t)B->gr * B->kp - 1) / ((int64_t)B->gr * B->kp)) * g.ncb;
-        t2 += g.t2_n;
         g.c_off = tc;
         g.c_n = (g.nelem + (int64_t)GCH * g.CW - 1) / ((int64_t)GCH * g.CW);
         tc += g.c_n;
-        g.p_off = poff;
-        poff += std::max(g.t2_n, g.c_n) * g.G;
         B->maxnops_dn = std::max(B->maxnops_dn, h->otf.dn.nops);
         // value tables: [call][copy] x {vf, vb}
         auto table = [&](bool up) {
@@ -524,30 +565,21 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
     }
     B->ng = (int)B->host.size();
     B->total_real = voff;
-    B->t1 = t1;
-    B->t2 = t2;
     B->tc = tc;
-    B->pcap = poff;
     int s = dmalloc((void **)&B->dvt, sizeof(double) * std::max<size_t>(vt.size(), 1));
     if (!s) s = dmalloc((void **)&B->dgr, sizeof(GGroup) * B->ng);
     if (!s) s = dmalloc((void **)&B->dcgroup, sizeof(int) * B->ncopies);
-    std::vector<int> tg1((size_t)B->t1), tg2((size_t)B->t2), tgc((size_t)B->tc);
+    std::vector<int> tgc((size_t)B->tc);
     for (int gq = 0; gq < B->ng; ++gq) {
         const GGroup &g = B->host[gq];
-        std::fill(tg1.begin() + g.t1_off, tg1.begin() + g.t1_off + g.t1_n, gq);
-        std::fill(tg2.begin() + g.t2_off, tg2.begin() + g.t2_off + g.t2_n, gq);
         std::fill(tgc.begin() + g.c_off, tgc.begin() + g.c_off + g.c_n, gq);
     }
-    if (!s) s = dmalloc((void **)&B->dtg1, sizeof(int) * std::max<int64_t>(B->t1, 1));
-    if (!s) s = dmalloc((void **)&B->dtg2, sizeof(int) * std::max<int64_t>(B->t2, 1));
     if (!s) s = dmalloc((void **)&B->dtgc, sizeof(int) * std::max<int64_t>(B->tc, 1));
     if (s) {
         delete B;
         return s;
     }
-    if (hipMemcpy(B->dtg1, tg1.data(), sizeof(int) * B->t1, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(B->dtg2, tg2.data(), sizeof(int) * B->t2, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(B->dtgc, tgc.data(), sizeof(int) * B->tc, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(B->dtgc, tgc.data(), sizeof(int) * B->tc, hipMemcpyHostToDevice) != hipSuccess) {
         delete B;
         return fail(ED_EHIP, "ed_batch_create: tile table upload failed");
     }
@@ -556,10 +588,13 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
         B->host[gq].vt_dn = B->dvt + vt_dn_off[gq];
     }
     if ((!vt.empty() && hipMemcpy(B->dvt, vt.data(), sizeof(double) * vt.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(B->dgr, B->host.data(), sizeof(GGroup) * B->ng, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->dcgroup, B->cgroup.data(), sizeof(int) * B->ncopies, hipMemcpyHostToDevice) != hipSuccess) {
         delete B;
         return fail(ED_EHIP, "ed_batch_create: upload failed");
+    }
+    if ((s = batch_tiles(B))) { // also uploads the group descriptors
+        delete B;
+        return s;
     }
     *out = reinterpret_cast<ed_batch>(B);
     return ED_OK;
@@ -893,4 +928,80 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
     } while (0);
     cleanup();
     return s;
+}
+
+extern "C" int ed_batch_set_shape(ed_batch bb, int dn_rows, int dn_panels, int up_rows, int up_panels)
+{
+    Batch *B = reinterpret_cast<Batch *>(bb);
+    if (!B || (dn_rows != 4 && dn_rows != 8) || (up_rows != 1 && up_rows != 4) || dn_panels < 1 || dn_panels > 16 ||
+        up_panels < 1 || up_panels > 64)
+        return fail(ED_EINVAL, "ed_batch_set_shape: DN rows 4/8, DN panels 1-16, UP rows 1/4, UP panels 1-64");
+    B->gr = dn_rows;
+    B->kp = dn_panels;
+    B->gu = up_rows;
+    B->kpu = up_panels;
+    return batch_tiles(B);
+}
+
+// Batch autotuner: times a fixed-length batched solve (`steps` recurrence steps of every instance,
+// no convergence stop) for each panel shape and keeps the fastest.  Stage 1: DN rows (4, 8) x DN
+// panels per workgroup (1, 2, 4, 8) with the current UP shape; stage 2: UP rows (1, 4) x UP panels
+// (1, 2, 4, 8, 16) with the best DN shape.  The per-element sums and the 4-row units of the dot
+// partials do not depend on the shape, so the energies are the same bit for bit.
+extern "C" int ed_batch_autotune(ed_batch bb, int steps, ed_batch_tuning *best, void *stream)
+{
+    Batch *B = reinterpret_cast<Batch *>(bb);
+    if (!B || steps < 10) return fail(ED_EINVAL, "ed_batch_autotune: bad arguments (steps >= 10)");
+    std::vector<double> ev(B->n);
+    std::vector<int> ns(B->n), st(B->n);
+    ed_lanczos_cfg cfg{steps, 1e-300, 10, 0};
+    auto time_one = [&](double &sec) -> int {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int s = ed_batch_lowest(bb, &cfg, 235, 1, ev.data(), ns.data(), st.data(), stream);
+        sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return s;
+    };
+    double cand[18];
+    for (double &c : cand) c = -1.0;
+    double bsec = -1.0;
+    int bgr = B->gr, bkp = B->kp, bgu = B->gu, bkpu = B->kpu, ci = 0, s = ED_OK;
+    for (int gr : {4, 8})
+        for (int kp : {1, 2, 4, 8}) {
+            if (s) break;
+            if ((s = ed_batch_set_shape(bb, gr, kp, bgu, bkpu))) break;
+            double sec = 0.0;
+            if ((s = time_one(sec))) break;
+            cand[ci++] = sec;
+            if (bsec < 0.0 || sec < bsec) {
+                bsec = sec;
+                bgr = gr;
+                bkp = kp;
+            }
+        }
+    for (int gu : {1, 4})
+        for (int kpu : {1, 2, 4, 8, 16}) {
+            if (s) break;
+            if ((s = ed_batch_set_shape(bb, bgr, bkp, gu, kpu))) break;
+            double sec = 0.0;
+            if ((s = time_one(sec))) break;
+            cand[ci++] = sec;
+            if (sec < bsec) {
+                bsec = sec;
+                bgu = gu;
+                bkpu = kpu;
+            }
+        }
+    const int s2 = ed_batch_set_shape(bb, bgr, bkp, bgu, bkpu);
+    if (s) return s;
+    if (s2) return s2;
+    if (best) {
+        best->dn_rows = bgr;
+        best->dn_panels = bkp;
+        best->up_rows = bgu;
+        best->up_panels = bkpu;
+        best->seconds = bsec;
+        best->steps = steps;
+        for (int i = 0; i < 18; ++i) best->candidate_s[i] = cand[i];
+    }
+    return ED_OK;
 }

@@ -336,8 +336,16 @@ __global__ __launch_bounds__(8 * TR) void k_slab_dn(SlabArgs a, int64_t ntiles, 
         if (FLAGS & HV_WRITEV) st2(vout + li, v0, v1);
     }
     if (FLAGS & HV_DOT) {
-        const double sum = block_sum<8 * TR>(dot, s_red);
-        if (tid == 0) partial[t] = sum;
+        // one partial per 32-row unit (its four waves summed in order), at slot su ntr32 + unit:
+        // the same partials, in the same slots, for every tile height
+        const double ws = wave_sum(dot);
+        if ((tid & (kWave - 1)) == 0) s_red[tid >> 6] = ws;
+        __syncthreads();
+        const int64_t ntr32 = (a.dd + 31) / 32, unit = (p0 >> 5) + tid;
+        if (tid < TR / 32 && unit < ntr32) {
+            const double *w4 = s_red + 4 * tid;
+            partial[su * ntr32 + unit] = ((w4[0] + w4[1]) + w4[2]) + w4[3];
+        }
     }
 }
 
@@ -490,8 +498,9 @@ static int slab_apply_t(const Hubbard &h, const double *x, double *z, const doub
 #undef ED_SLAB_B
     ED_CHECK_LAUNCH();
     if (fb & HV_DOT) {
-        const int64_t nc = (tilesB + 2047) / 2048;
-        hipLaunchKernelGGL(k_slab_chunks, dim3((unsigned)nc), dim3(256), 0, st, tilesB, L.part, partial);
+        const int64_t np32 = L.nsu * ((L.dd + 31) / 32); // pass-B partials, one per 32-row unit
+        const int64_t nc = (np32 + 2047) / 2048;
+        hipLaunchKernelGGL(k_slab_chunks, dim3((unsigned)nc), dim3(256), 0, st, np32, L.part, partial);
         ED_CHECK_LAUNCH();
         if (nblocks) *nblocks = nc;
     }

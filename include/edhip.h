@@ -334,6 +334,21 @@ int ed_batch_destroy(ed_batch b);
  * the same kernels without capture.  Synchronises. */
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
+/* Panel shape of the batch kernels: DN rows per panel (4 or 8) x DN panels per workgroup (1-16),
+ * UP rows per panel (1 or 4) x UP panels per workgroup (1-64); default 4 x 4, 1 x 4.  The
+ * eigenvalues do not depend on it, bit for bit (same per-element sums; the dot partials are per
+ * 4-row unit whatever the shape).  Host; rebuilds the tile tables. */
+int ed_batch_set_shape(ed_batch b, int dn_rows, int dn_panels, int up_rows, int up_panels);
+/* Autotuner (no counterpart in the reference): times a solve of `steps` (>= 10) recurrence steps of
+ * every instance for each shape -- DN rows {4, 8} x DN panels {1, 2, 4, 8}, then UP rows {1, 4} x
+ * UP panels {1, 2, 4, 8, 16} with the best DN shape -- and keeps the fastest.  best (may be NULL)
+ * gets the choice, its time and every candidate's time in the order tried.  Synchronises. */
+typedef struct {
+    int dn_rows, dn_panels, up_rows, up_panels, steps;
+    double seconds;
+    double candidate_s[18];
+} ed_batch_tuning;
+int ed_batch_autotune(ed_batch b, int steps, ed_batch_tuning *best, void *stream);
 
 /* ---------------- adjoint / Hellmann-Feynman --------------------------------------- */
 

@@ -81,3 +81,25 @@ def test_full_sweep_all_instances():
     epi, _ = BT.plain_lowest(hs, False)
     for e, nu in zip(epi, (3, 5, 6)):
         assert abs(E0b[nu, 8] - e) <= 1e-10 * max(1.0, abs(e)), (nu, E0b[nu, 8], e)
+
+
+def test_batch_shapes_bitwise_and_autotune():
+    """Every panel shape of the batch kernels gives the same energies bit for bit (per-element sums
+    in call order, dot partials per 4-row unit), and the autotuner keeps one of them."""
+    inst = BT.lieb12_instances(U=4.0, nflux=16, fluxes=[1, 5])
+    hs = [h for (_, _, h) in inst]
+    bl = BT.BatchedLanczos(hs, True)
+    ref = None
+    for shape in ((4, 4, 1, 4), (8, 1, 4, 16), (4, 8, 1, 1), (8, 2, 1, 8)):
+        bl.set_shape(*shape)
+        ev, ns, st = bl.lowest(use_graph=True)
+        assert np.all(st == 0)
+        if ref is None:
+            ref = (ev, ns)
+        else:
+            assert np.array_equal(ns, ref[1]), shape
+            assert np.array_equal(ev.view(np.uint64), ref[0].view(np.uint64)), shape
+    t = bl.autotune(steps=20)
+    assert t["dn_rows"] in (4, 8) and t["up_rows"] in (1, 4) and t["seconds"] > 0
+    ev, ns, st = bl.lowest(use_graph=True)
+    assert np.array_equal(ev.view(np.uint64), ref[0].view(np.uint64))

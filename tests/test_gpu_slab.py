@@ -218,6 +218,17 @@ def test_slab_shapes_bitwise_and_autotune():
                 yl.fill_(float("nan"))
                 H.apply_slab(xl, yl)
                 assert torch.equal(H.from_slab(yl), yref), (tr, nb, order)
+    # the fused Lanczos dot (one partial per 32-row unit) is the same for every shape: the
+    # recurrence coefficients agree bit for bit
+    coef = []
+    f0 = torch.from_numpy(O.random_vector(H.dim)).to(DEV)
+    for tr, nb, order in ((32, 8, 8), (64, 16, 0), (64, 8, 4)):
+        H.slab_set_shape(tr, nb, order)
+        lz = Lanczos(H.dimUP, H.dimDN)
+        lz.tridiagonal(H, f0, 20)
+        coef.append(np.array(lz.a + lz.b))
+    for c in coef[1:]:
+        assert np.array_equal(c.view(np.uint64), coef[0].view(np.uint64))
     best = H.slab_autotune(xl, yl, reps=1)
     assert best["tile_rows"] in (32, 64) and best["gathers_in_flight"] in (8, 16)
     assert best["tile_order"] in (0, 2, 4, 8, 16) and best["ms_per_apply"] > 0
