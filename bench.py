@@ -35,17 +35,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per H.v launch from the committed rocprofv3 PMC summary (or None)."""
+def read_traffic(shape):
+    """HBM bytes per H.v launch for the slab tile shape (tile_rows, gathers_in_flight, tile_order)
+    from the committed rocprofv3 PMC summary (None when that shape was not measured)."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     vals = {}
     for line in open(TRAFFIC_FILE):
         if "=" in line and not line.lstrip().startswith("#"):
             k, v = line.split("=", 1)
-            vals[k.strip()] = v.strip()
+            vals[k.strip().replace(" ", "")] = v.strip()
+    key = "hbm_bytes_per_launch[%d,%d,%d]" % tuple(shape)
     try:
-        return float(vals["hbm_bytes_per_launch"])
+        return float(vals[key])
     except (KeyError, ValueError):
         return None
 
@@ -78,11 +80,12 @@ def cpu_baseline(H_or, x_host, dim_dn, budget_s=15.0):
     return 16.0 * elems / dt2 / 1e9, rows2, dt2, O.num_threads()
 
 
-def ground_state_line(H, hq, x, y, N, Q, reorth=False):
+def ground_state_line(H, hq, x, y, N, Q, reorth=False, slab_shape=None):
     """Lanczos ground state (reference convergence test) of the benchmark Hamiltonian from the
     seed-235 start, Ritz vector rebuilt by recomputation, then S_vN of sites 0..N/2-1.  reorth:
     a second Gram-Schmidt pass of every f against v_N (DGKS), the reorthogonalisation that three
-    resident vectors allow."""
+    resident vectors allow.  slab_shape: the slab H.v launch shape (tile rows, gathers in flight,
+    tile order) the H.v autotuner chose."""
     import ctypes as C
     import numpy as np
     import torch
@@ -91,6 +94,8 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     lz = Lanczos(hq.count, hq.count, reorth=reorth)
     lz.random_vector(x)
     H.slab_layout(True)  # the recurrence runs on the slab layout (start and Ritz vectors converted)
+    if slab_shape is not None:
+        H.slab_set_shape(*slab_shape)
     torch.cuda.synchronize()
     cur, peak = C.c_int64(), C.c_int64()
     call("ed_memory_stats", C.byref(cur), C.byref(peak), 1)
@@ -98,6 +103,9 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     e0 = lz.eigen(H, x, y)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # the omega monitor's estimate of the largest |v_j . v_k| over the run (Simon's recurrence on
+    # the run's a and b): semi-orthogonality is <= sqrt(eps) = 1.5e-8
+    lz_omega = lz.so_stats()["omega_max"]
     call("ed_memory_stats", C.byref(cur), C.byref(peak), 0)
     D = hq.count * hq.count
     # one forward step: the fixed-length recurrence (ed_lanczos_tridiagonal, same step kernels
@@ -122,15 +130,62 @@ def ground_state_line(H, hq, x, y, N, Q, reorth=False):
     torch.cuda.empty_cache()
     log(f"[bench] ground state E0 {e0:.15f} in {lz.nsteps} steps, {t1 - t0:.1f} s")
     s = np.zeros(1)
+    call("ed_entanglement_stats", None, None, None, 1)
+    torch.cuda.synchronize()
+    t_e = time.perf_counter()
     call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(y), s.ctypes.data_as(C.POINTER(C.c_double)),
          stream_ptr())
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    log(f"[bench] S_vN({N // 2}|{N - N // 2}) {s[0]:.15f} in {t2 - t1:.1f} s")
-    return {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "reorth": bool(reorth),
-            "ms_per_forward_step": round(ms_step, 2), "peak_device_gb": round(peak_bytes / 1e9, 2),
-            "peak_device_vectors": round(peak_bytes / (8.0 * D), 2),
-            "S_vN": float(s[0]), "entropy_s": round(t2 - t1, 2)}
+    log(f"[bench] S_vN({N // 2}|{N - N // 2}) {s[0]:.15f} in {t2 - t_e:.1f} s")
+    call("ed_entanglement_stats", None, None, None, 1)
+    # the same entropy with every block on one stream: each Gram kernel then runs alone, so its
+    # event-timed duration is its own (on the worker streams the Gram overlaps other blocks'
+    # eigenvalue kernels and its events also count their time)
+    s1 = np.zeros(1)
+    prev = os.environ.get("ED_EE_PARALLEL_MIN")
+    os.environ["ED_EE_PARALLEL_MIN"] = str(1 << 40)
+    try:
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(y),
+             s1.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+    finally:
+        if prev is None:
+            del os.environ["ED_EE_PARALLEL_MIN"]
+        else:
+            os.environ["ED_EE_PARALLEL_MIN"] = prev
+    log(f"[bench] S_vN on one stream {s1[0]:.15f} in {t4 - t3:.1f} s")
+    out = {"E0": e0, "lanczos_steps": lz.nsteps, "lanczos_s": round(t1 - t0, 2), "reorth": bool(reorth),
+           "ms_per_forward_step": round(ms_step, 2), "peak_device_gb": round(peak_bytes / 1e9, 2),
+           "peak_device_vectors": round(peak_bytes / (8.0 * D), 2),
+           "S_vN": float(s[0]), "entropy_s": round(t2 - t_e, 2), "entropy_one_stream_s": round(t4 - t3, 2),
+           "S_vN_one_stream_minus_workers": float(s1[0] - s[0]),
+           "omega_max_estimate": lz_omega}
+    out.update(rho_a_roofline())
+    return out
+
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def rho_a_roofline():
+    """MFMA roofline of the rho_A Gram kernels of the entanglement calls since the last reset
+    (library statistics: SYRK FLOPs over the kernels' event-timed durations) and the eigenvalue
+    stage's kernel time.  Meaningful when the blocks ran on one stream."""
+    import ctypes as C
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call
+    g_ms, g_fl, e_ms = C.c_double(), C.c_double(), C.c_double()
+    call("ed_entanglement_stats", C.byref(g_ms), C.byref(g_fl), C.byref(e_ms), 1)
+    tf = g_fl.value / (g_ms.value * 1e-3) / 1e12 if g_ms.value > 0 else 0.0
+    log(f"[bench] rho_A Gram: {g_fl.value:.4e} FLOPs in {g_ms.value:.1f} ms of kernels = {tf:.1f} TFLOP/s "
+        f"({tf / FP64_MFMA_PEAK_TFLOPS:.3f} of FP64 MFMA peak); eigenvalue stage {e_ms.value / 1e3:.2f} s of kernels")
+    return {"rho_a_roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": FP64_MFMA_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+                               "gram_flops": g_fl.value, "gram_ms": round(g_ms.value, 2)},
+            "eigenvalue_stage_s": round(e_ms.value / 1e3, 2)}
 
 
 def kagome12_line(with_cpu=True):
@@ -176,8 +231,9 @@ def kagome12_line(with_cpu=True):
 
 def entropy_only_line(N=18, Q=9, seed=5):
     """S_vN(N/2 | N/2) of a seeded random normalized state of the N-site, Q/Q sector: the rho_A
-    blocks (FP64 MFMA Gram) and their eigenvalues alone, to profile them without a ground state.
-    The block sizes, and so the Gram FLOPs (2.789e13 at 18 sites), do not depend on the state."""
+    blocks (FP64 MFMA Gram) and their eigenvalues alone, to profile them without a ground state,
+    every block on one stream (no worker threads).  The block sizes, and so the Gram FLOPs
+    (2.789e13 at 18 sites), do not depend on the state."""
     import ctypes as C
     import numpy as np
     import torch
@@ -191,14 +247,18 @@ def entropy_only_line(N=18, Q=9, seed=5):
     psi /= torch.linalg.norm(psi)
     s = np.zeros(1)
     torch.cuda.synchronize()
+    call("ed_entanglement_stats", None, None, None, 1)
+    os.environ["ED_EE_PARALLEL_MIN"] = str(1 << 40)  # one stream: the Gram kernels run alone
     t0 = time.perf_counter()
     call("ed_entanglement_entropy_vn", N, Q, Q, (1 << (N // 2)) - 1, 0, dptr(psi), s.ctypes.data_as(C.POINTER(C.c_double)),
          stream_ptr())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     log(f"[bench] S_vN({N // 2}|{N - N // 2}) of a random state {s[0]:.15f} in {t1 - t0:.1f} s")
-    return {"S_vN": float(s[0]), "entropy_s": round(t1 - t0, 2), "dim": D,
-            "workload": f"random_state_N{N}_nup{Q}_ndn{Q}_cut{N // 2}"}
+    out = {"S_vN": float(s[0]), "entropy_s": round(t1 - t0, 2), "dim": D,
+           "workload": f"random_state_N{N}_nup{Q}_ndn{Q}_cut{N // 2}"}
+    out.update(rho_a_roofline())
+    return out
 
 
 def batched_drude_line(with_cpu=False):
@@ -254,14 +314,14 @@ def batched_drude_line(with_cpu=False):
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
-def ensemble_line(n_real, ncv=24):
+def ensemble_line(n_real, ncv=24, workers=8):
     """14-site open chain, 7/7, U = 4, W = 8: E0 of all realisations in one batched launch, then per
     realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i, eight realisations at
     a time on separate streams.  ncv = 24 basis vectors for the 4 lowest (32 realisations: 16, 20,
     24, 32, 40 gave 0.90, 1.00, 1.01, 0.93, 0.82 realisations/s; 12 does not converge for all)."""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
     res = DS.run_ensemble(n_real=n_real, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=ncv, tol=1e-10,
-                          log=lambda m: log("[bench] " + m), with_sref=False, workers=8)
+                          log=lambda m: log("[bench] " + m), with_sref=False, workers=workers)
     log(f"[bench] disorder ensemble: {n_real} realisations in {res.seconds:.2f} s "
         f"(batched E0 of all: {res.seconds_batched_e0:.2f} s)")
     return {"value": round(res.realisations_per_s, 3), "unit": "realisations/s", "realisations": n_real,
@@ -325,6 +385,9 @@ def main():
                     help="with --ground-state: a DGKS pass of each new Lanczos vector against v_N")
     ap.add_argument("--entropy-only", action="store_true",
                     help="only S_vN(9|9) of a seeded random 18-site 9/9 state (rho_A Gram + block eigenvalues)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the disorder-ensemble line (--ensemble realisations, --workers threads)")
+    ap.add_argument("--workers", type=int, default=8, help="host threads of the disorder-ensemble line")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -338,6 +401,9 @@ def main():
 
     if args.drude_only:
         print(json.dumps({"batched_drude": batched_drude_line()}), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps({"disorder_ensemble": ensemble_line(args.ensemble, workers=args.workers)}), flush=True)
         return
     if args.entropy_only:
         print(json.dumps({"entropy": entropy_only_line()}), flush=True)
@@ -446,7 +512,7 @@ def main():
         torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         log(f"[bench] copy reference: {16 * D} bytes moved by one copy kernel")
-    traffic = read_traffic()
+    traffic = read_traffic((slab_tuning["tile_rows"], slab_tuning["gathers_in_flight"], slab_tuning["tile_order"]))
     out = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -490,13 +556,15 @@ def main():
                                          f"of {hq.count} x {hq.count} ({secs:.1f} s), same 16 B/element count"}
         del x_host
     if args.ground_state or not (args.no_extras or args.no_ground_state):
-        out["ground_state"] = ground_state_line(H, hq, x, y, N, Q, reorth=args.reorth)
+        out["ground_state"] = ground_state_line(
+            H, hq, x, y, N, Q, reorth=args.reorth,
+            slab_shape=(slab_tuning["tile_rows"], slab_tuning["gathers_in_flight"], slab_tuning["tile_order"]))
     if not args.no_extras:
         del x, y
         torch.cuda.empty_cache()
         out["batched_drude"] = batched_drude_line(with_cpu=not args.no_cpu)
         out["kagome12_ground_state"] = kagome12_line(with_cpu=not args.no_cpu)
-        out["disorder_ensemble"] = ensemble_line(args.ensemble)
+        out["disorder_ensemble"] = ensemble_line(args.ensemble, workers=args.workers)
         out["krylov_quench"] = krylov_line()
     print(json.dumps(out), flush=True)
 

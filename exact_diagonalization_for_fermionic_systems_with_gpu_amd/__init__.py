@@ -8,10 +8,11 @@ missing.
 """
 from .fermi_operator import H_TBq, count1, count1_bt_sites, lowmem_basis  # noqa: F401
 from .hubmatrix import (Hubbard, Krylov, Lanczos, Thrust_Lin, compute_sigma_omega,  # noqa: F401
-                        correlator_continued_fraction, cusolverDsyevd, cusolverZheevd, drude_weight_Jx, tED,
-                        tridiag_eigen)
+                        correlator_continued_fraction, cusolverDsyevd, cusolverZheevd, drude_weight_Jx,
+                        lanczos_orthogonality, tED, tridiag_eigen)
 from . import lattices  # noqa: F401
 
 __all__ = ["H_TBq", "Hubbard", "Krylov", "Lanczos", "Thrust_Lin", "compute_sigma_omega",
-           "correlator_continued_fraction", "cusolverDsyevd", "cusolverZheevd", "drude_weight_Jx", "tED",
+           "correlator_continued_fraction", "cusolverDsyevd", "cusolverZheevd", "drude_weight_Jx",
+           "lanczos_orthogonality", "tED",
            "tridiag_eigen", "lattices", "count1", "count1_bt_sites", "lowmem_basis"]

@@ -92,6 +92,8 @@ SIGNATURES = {
     "ed_lanczos_eigen": (_i32, [_p, _i32, C.POINTER(LanczosCfg), _p, _pd, _p, _pd, _pd, _pi, _p]),
     "ed_lanczos_tridiagonal": (_i32, [_p, _i32, _i32, _p, _pd, _pd, _p]),
     "ed_tridiag_eigen": (_i32, [_i32, _pd, _pd, _pd, _pd]),
+    "ed_lanczos_orthogonality": (_i32, [_i32, _pd, _pd, _pd]),
+    "ed_lanczos_so_stats": (_i32, [_pi, _pi, _pi, _pd]),
     "ed_krylov_tridiagonal": (_i32, [_p, _p, _i32, _d, _pd, _pd, _pi, _p]),
     "ed_krylov_time_evol": (_i32, [_p, _p, _i32, _pd, _pd, _pd, _pd, _d, _p, _p]),
     "ed_tridiag_lowest": (_i32, [_i32, _pd, _pd, _pd, _pd]),
@@ -113,6 +115,7 @@ SIGNATURES = {
     "ed_entanglement_entropy": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _pi64, _p]),
     "ed_entanglement_entropy_vn": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _p]),
     "ed_entanglement_spectrum": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _i64, _pi64, _p]),
+    "ed_entanglement_stats": (_i32, [_pd, _pd, _pd, _i32]),
 }
 
 _LIB = None

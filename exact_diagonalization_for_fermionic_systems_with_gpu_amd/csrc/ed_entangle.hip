@@ -331,6 +331,15 @@ static EePool *ee_pool(int dev)
     return p;
 }
 
+// kernel-time statistics of the block spectra, summed over calls until reset: the Gram kernels'
+// durations and their SYRK FLOPs, and the durations of the eigenvalue (and SVD) stage, each
+// measured by events on the stream that ran it
+struct EeStats {
+    double gram_ms = 0.0, gram_flops = 0.0, eig_ms = 0.0;
+};
+static std::mutex g_stats_mu;
+static EeStats g_stats;
+
 // spectra of all blocks: lambda (rho eigenvalues, clipped at 0) and sigma (Schmidt values)
 static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const void *d_psi, hipStream_t st,
                          std::vector<double> &lambda, std::vector<double> &sigma, bool want_sigma)
@@ -464,6 +473,22 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 return;
             }
             double *rho = dR + (size_t)wi * (bR / sizeof(double)), *E = dE + (size_t)wi * (bE / sizeof(double));
+            // timing events of this worker's blocks: before the Gram, after it, after the solvers
+            std::vector<hipEvent_t> tev;
+            double flops = 0.0;
+            struct TevGuard {
+                std::vector<hipEvent_t> &v;
+                ~TevGuard()
+                {
+                    for (hipEvent_t e : v) (void)hipEventDestroy(e);
+                }
+            } tev_guard{tev};
+            auto mark = [&]() {
+                hipEvent_t e = nullptr;
+                if (hipEventCreate(&e) != hipSuccess) return false;
+                tev.push_back(e);
+                return hipEventRecord(e, ws) == hipSuccess;
+            };
             for (int q = next.fetch_add(1); q < njobs; q = next.fetch_add(1)) {
                 const int b = order[q];
                 const BlockGeo &G = P.geo[jobs[b]];
@@ -471,11 +496,16 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 double *A = dM + NC * G.off;
                 const int ntile = (int)((k + TT - 1) / TT);
                 const int64_t nblk = (int64_t)ntile * (ntile + 1) / 2;
+                if (!mark()) {
+                    wfail(wi, "entanglement: timing event failed (%d)", 0);
+                    return;
+                }
+                flops += (cplx ? 4.0 : 1.0) * (double)k * (double)(k + 1) * (double)K;
                 if (cplx)
                     hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
                 else
                     hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
-                if (hipGetLastError() != hipSuccess) {
+                if (hipGetLastError() != hipSuccess || !mark()) {
                     wfail(wi, "entanglement: gram launch failed (%d)", 0);
                     return;
                 }
@@ -491,7 +521,13 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                     wfail(wi, "entanglement: rocsolver syevd failed (%d)", (int)rs);
                     return;
                 }
-                if (!want_sigma) continue;
+                if (!want_sigma) {
+                    if (!mark()) {
+                        wfail(wi, "entanglement: timing event failed (%d)", 0);
+                        return;
+                    }
+                    continue;
+                }
                 // Schmidt values: SVD of the block (column-major K x k, ld K); A is overwritten
                 if (cplx)
                     rs = rocsolver_zgesvd(hw, rocblas_svect_none, rocblas_svect_none, (rocblas_int)K, (rocblas_int)k,
@@ -505,7 +541,28 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                     wfail(wi, "entanglement: rocsolver gesvd failed (%d)", (int)rs);
                     return;
                 }
+                if (!mark()) {
+                    wfail(wi, "entanglement: timing event failed (%d)", 0);
+                    return;
+                }
             }
+            if (tev.empty()) return;
+            if (hipEventSynchronize(tev.back()) != hipSuccess) {
+                wfail(wi, "entanglement: worker stream failed (%d)", 0);
+                return;
+            }
+            double gms = 0.0, ems = 0.0;
+            for (size_t j = 0; j + 2 < tev.size(); j += 3) {
+                float a = 0.0f, c = 0.0f;
+                (void)hipEventElapsedTime(&a, tev[j], tev[j + 1]);
+                (void)hipEventElapsedTime(&c, tev[j + 1], tev[j + 2]);
+                gms += a;
+                ems += c;
+            }
+            std::lock_guard<std::mutex> lk(g_stats_mu);
+            g_stats.gram_ms += gms;
+            g_stats.gram_flops += flops;
+            g_stats.eig_ms += ems;
         };
         if (nw > 1 && !pool->warm[cplx ? 1 : 0][want_sigma ? 1 : 0]) {
             // Before the first worker threads on this device: rocBLAS's kernels are loaded and each
@@ -617,5 +674,15 @@ extern "C" int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t mas
     for (double l : lam)
         if (l > 0.0) svn -= l * std::log(l);
     *s_vn = svn;
+    return ED_OK;
+}
+
+extern "C" int ed_entanglement_stats(double *gram_ms, double *gram_flops, double *eig_ms, int reset)
+{
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    if (gram_ms) *gram_ms = g_stats.gram_ms;
+    if (gram_flops) *gram_flops = g_stats.gram_flops;
+    if (eig_ms) *eig_ms = g_stats.eig_ms;
+    if (reset) g_stats = EeStats();
     return ED_OK;
 }

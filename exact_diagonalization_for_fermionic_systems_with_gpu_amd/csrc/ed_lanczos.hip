@@ -22,107 +22,21 @@
 
 #include "ed_common.hpp"
 #include "ed_hubbard.hpp"
+#include "ed_tridiag.hpp"
 
 namespace ed {
 
 // ---- symmetric tridiagonal eigen-solvers (host) ------------------------------------
 
-// number of eigenvalues below x (Sturm sequence of the LDL^T pivots)
-static int sturm_below(int n, const double *d, const double *e, double x)
-{
-    int c = 0;
-    double q = d[0] - x;
-    if (q < 0) ++c;
-    for (int i = 1; i < n; ++i) {
-        if (q == 0.0) q = std::numeric_limits<double>::min();
-        q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
-        if (q < 0) ++c;
-    }
-    return c;
-}
+// the lowest eigenpair: Sturm bisection and inverse iteration (ed_tridiag.hpp, shared with the
+// batch engine's device-side convergence test)
+static double bisect_kth(int n, const double *d, const double *e, int k) { return tri_bisect_kth(n, d, e, k); }
 
-static double bisect_kth(int n, const double *d, const double *e, int k)
-{
-    double lo = d[0], hi = d[0];
-    for (int i = 0; i < n; ++i) {
-        const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
-        lo = std::fmin(lo, d[i] - r);
-        hi = std::fmax(hi, d[i] + r);
-    }
-    const double pad = 1e-12 * (hi - lo + 1.0);
-    lo -= pad;
-    hi += pad;
-    for (int it = 0; it < 256; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (!(mid > lo && mid < hi)) break;
-        if (sturm_below(n, d, e, mid) > k) hi = mid;
-        else lo = mid;
-    }
-    return 0.5 * (lo + hi);
-}
-
-// inverse iteration on T - lam I, elimination with partial pivoting; z normalised, sign
-// such that the largest component is positive
 static void inverse_iteration(int n, const double *d, const double *e, double lam, double *z)
 {
-    if (n == 1) {
-        z[0] = 1.0;
-        return;
-    }
-    std::vector<double> lo(n), dg(n), up(n), up2(n, 0.0), x(n, 1.0);
-    std::vector<char> sw(n, 0);
-    double scale = std::fabs(lam);
-    for (int i = 0; i < n; ++i) scale = std::fmax(scale, std::fabs(d[i]));
-    const double tiny = std::numeric_limits<double>::epsilon() * (scale + 1e-300);
-    for (int i = 0; i < n; ++i) {
-        dg[i] = d[i] - lam;
-        lo[i] = i < n - 1 ? e[i] : 0.0;
-        up[i] = i < n - 1 ? e[i] : 0.0;
-    }
-    for (int i = 0; i < n - 1; ++i) {
-        if (std::fabs(dg[i]) >= std::fabs(lo[i])) {
-            if (dg[i] == 0.0) dg[i] = tiny;
-            const double m = lo[i] / dg[i];
-            lo[i] = m;
-            dg[i + 1] -= m * up[i];
-        } else {
-            const double m = dg[i] / lo[i];
-            dg[i] = lo[i];
-            lo[i] = m;
-            const double t = up[i];
-            up[i] = dg[i + 1];
-            dg[i + 1] = t - m * dg[i + 1];
-            if (i < n - 2) {
-                up2[i] = up[i + 1];
-                up[i + 1] = -m * up[i + 1];
-            }
-            sw[i] = 1;
-        }
-    }
-    if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
-    for (int sweep = 0; sweep < 3; ++sweep) {
-        for (int i = 0; i < n - 1; ++i) {
-            if (!sw[i]) {
-                x[i + 1] -= lo[i] * x[i];
-            } else {
-                const double t = x[i];
-                x[i] = x[i + 1];
-                x[i + 1] = t - lo[i] * x[i];
-            }
-        }
-        x[n - 1] /= dg[n - 1];
-        x[n - 2] = (x[n - 2] - up[n - 2] * x[n - 1]) / dg[n - 2];
-        for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - up[i] * x[i + 1] - up2[i] * x[i + 2]) / dg[i];
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += x[i] * x[i];
-        s = 1.0 / std::sqrt(s);
-        for (int i = 0; i < n; ++i) x[i] *= s;
-    }
-    int imax = 0;
-    for (int i = 1; i < n; ++i)
-        if (std::fabs(x[i]) > std::fabs(x[imax])) imax = i;
-    const double sg = x[imax] < 0 ? -1.0 : 1.0;
-    for (int i = 0; i < n; ++i) z[i] = sg * x[i];
+    std::vector<double> ws(5 * (size_t)n);
+    std::vector<char> sw((size_t)n);
+    tri_inverse_iteration(n, d, e, lam, z, ws.data(), sw.data());
 }
 
 // implicit QL with Wilkinson-type shifts, Golub & Van Loan sec. 8.3; with z (n x n, column j =
@@ -177,6 +91,66 @@ static void ql_eig(int n, double *d, double *e /* n entries, e[n-1] scratch */, 
         } while (m != l);
     }
 }
+
+// ---- orthogonality monitor ---------------------------------------------------------
+// Estimates w_{j,k} = v_j . v_k of the Lanczos vectors from a and b alone (the omega recurrence
+// of H. D. Simon, "The Lanczos algorithm with partial reorthogonalization", Math. Comp. 42 (1984)
+// 115-142).  Taking v_k . (H v_j) - v_j . (H v_k) = 0 with H v_j = b[j] v_{j-1} + a[j] v_j +
+// b[j+1] v_{j+1} gives, for k < j,
+//   b[j+1] w_{j+1,k} = b[k+1] w_{j,k+1} + (a[k] - a[j]) w_{j,k} + b[k] w_{j,k-1} - b[j] w_{j-1,k}
+// plus the rounding of the two steps, modelled as in that paper by eps (b[k+1] + b[j+1]) times a
+// normal deviate of standard deviation 0.3 (a fixed pseudo-random sequence, so the estimate is
+// reproducible); w_{j+1,j} = eps |T| / b[j+1] with |T| the running max_k |a[k]| + b[k] + b[k+1],
+// and w_{j,j} = 1.
+struct OmegaMonitor {
+    std::vector<double> prev, cur{1.0}, next;
+    int j = 0; // cur holds row j
+    double tnorm = 0.0;
+    uint64_t rng = 0x9e3779b97f4a7c15ull;
+    double normal() // Box-Muller over splitmix64 uniforms
+    {
+        auto u01 = [&]() {
+            uint64_t z = (rng += 0x9e3779b97f4a7c15ull);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            z ^= z >> 31;
+            return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        const double u = u01(), v = u01();
+        return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+    }
+    // row j+1 from a[0..j], b[0..j+1]; returns max_{k<=j} |w_{j+1,k}|
+    double step(const double *a, const double *b)
+    {
+        const double eps = std::numeric_limits<double>::epsilon();
+        tnorm = std::fmax(tnorm, std::fabs(a[j]) + (j > 0 ? b[j] : 0.0) + b[j + 1]);
+        const double bj1 = b[j + 1] > 0.0 ? b[j + 1] : std::numeric_limits<double>::min();
+        next.assign((size_t)j + 2, 0.0);
+        double mx = 0.0;
+        for (int k = 0; k < j; ++k) {
+            double t = b[k + 1] * cur[k + 1] + (a[k] - a[j]) * cur[k];
+            if (k > 0) t += b[k] * cur[k - 1];
+            t -= b[j] * prev[k];
+            t += 0.3 * eps * (b[k + 1] + b[j + 1]) * normal();
+            next[k] = t / bj1;
+            mx = std::fmax(mx, std::fabs(next[k]));
+        }
+        next[j] = eps * tnorm / bj1;
+        next[j + 1] = 1.0;
+        mx = std::fmax(mx, next[j]);
+        prev.swap(cur);
+        cur.swap(next);
+        ++j;
+        return mx;
+    }
+    // after the two newest vectors were orthogonalised against every direction the loss lies in
+    void reset_recent()
+    {
+        const double eps = std::numeric_limits<double>::epsilon();
+        for (int k = 0; k + 1 < j; ++k) cur[k] = eps;
+        for (int k = 0; k + 1 < j - 1; ++k) prev[k] = eps;
+    }
+};
 
 // ---- Lanczos workspace -------------------------------------------------------------
 struct Work {
@@ -244,9 +218,175 @@ static int lanczos_step_vfree(const Hubbard &h, int64_t nreal, double *A, double
     return ED_OK;
 }
 
+// ---- selective orthogonalisation (Parlett & Scott) -----------------------------------
+// B. N. Parlett and D. S. Scott, "The Lanczos algorithm with selective orthogonalization",
+// Math. Comp. 33 (1979) 217-238.  The Lanczos vectors lose orthogonality only along Ritz vectors
+// that have converged ("good" ones: b_{N+1} |s_N| <= sqrt(eps) |T|, Paige's theorem), and once a
+// Ritz vector is good the loss along it regrows within a step, so every new f is orthogonalised
+// against all good Ritz vectors (on the device, no host round trip).  The Ritz analysis that finds
+// new good vectors runs at the convergence-test cadence; a vector found good there is also
+// removed from v_N.  Forming a Ritz vector needs the Lanczos basis, which is kept on the device;
+// the Ritz vector of the result is then V z (no second recurrence).
+
+__global__ __launch_bounds__(256) void k_basis_combine(int64_t n, int nk, const double *__restrict__ V,
+                                                       const double *__restrict__ s, double *__restrict__ y,
+                                                       int accumulate)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double acc = accumulate ? y[i] : 0.0;
+    for (int k = 0; k < nk; ++k) acc = fma(s[k], V[(size_t)k * (size_t)n + (size_t)i], acc);
+    y[i] = acc;
+}
+
+// i y of an interleaved complex vector
+__global__ __launch_bounds__(256) void k_times_i(int64_t ncplx, const double *__restrict__ y, double *__restrict__ iy)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ncplx) return;
+    iy[2 * i] = -y[2 * i + 1];
+    iy[2 * i + 1] = y[2 * i];
+}
+
+struct SelOrth {
+    static constexpr int kChunk = 16;
+    int64_t n = 0; // doubles per vector
+    bool cvec = false;
+    std::vector<double *> chunks; // v_0, v_1, ... kChunk per allocation
+    std::vector<double *> Y;      // unit good Ritz vectors (with i y after each, complex vectors)
+    std::vector<double> theta;    // their Ritz values
+    double *coef = nullptr;       // device: purge coefficients, then Ritz coordinates
+    int coef_cap = 0;
+    int analyses = 0, purges = 0;
+    ~SelOrth()
+    {
+        for (double *c : chunks) dfree(c);
+        for (double *y : Y) dfree(y);
+        dfree(coef);
+    }
+    double *vec(int k) { return chunks[(size_t)k / kChunk] + (size_t)(k % kChunk) * (size_t)n; }
+    int store(int k, const double *v, hipStream_t st)
+    {
+        while ((int)chunks.size() * kChunk <= k) {
+            double *c = nullptr;
+            if (dmalloc((void **)&c, sizeof(double) * (size_t)n * kChunk) != ED_OK)
+                return fail(ED_ENOMEM, "ed_lanczos_eigen: selective reorthogonalisation keeps the Lanczos basis on "
+                                       "the device; %d vectors of %lld doubles do not fit",
+                            (int)(chunks.size() + 1) * kChunk, (long long)n);
+            chunks.push_back(c);
+        }
+        ED_HIP(hipMemcpyAsync(vec(k), v, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        return ED_OK;
+    }
+    int reserve_coef(int m)
+    {
+        if (m <= coef_cap) return ED_OK;
+        dfree(coef);
+        coef = nullptr;
+        coef_cap = 0;
+        ED_TRY(dmalloc((void **)&coef, sizeof(double) * (size_t)m));
+        coef_cap = m;
+        return ED_OK;
+    }
+    // y = sum_{k<=M} z_k v_k
+    int combine(int M, const double *z, double *y, hipStream_t st)
+    {
+        ED_TRY(reserve_coef(std::max(M + 1, 2 * (int)Y.size() + 2)));
+        ED_HIP(hipMemcpyAsync(coef, z, sizeof(double) * (size_t)(M + 1), hipMemcpyHostToDevice, st));
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        for (int k0 = 0; k0 <= M; k0 += kChunk) {
+            const int nk = std::min(kChunk, M + 1 - k0);
+            hipLaunchKernelGGL(k_basis_combine, dim3(grid), dim3(256), 0, st, n, nk, (const double *)vec(k0),
+                               (const double *)(coef + k0), y, k0 > 0 ? 1 : 0);
+            ED_CHECK_LAUNCH();
+        }
+        // the host copy of z must outlive the upload
+        ED_HIP(hipStreamSynchronize(st));
+        return ED_OK;
+    }
+    // x -= (y . x) y for every good Ritz vector in turn; the partials of |x|^2 of the result
+    // (nb = 0 when there is none)
+    int purge(double *x, double *partial, int64_t *nb, hipStream_t st)
+    {
+        *nb = 0;
+        ED_TRY(reserve_coef(2 * (int)Y.size() + 2));
+        for (size_t q = 0; q < Y.size(); ++q) {
+            int64_t nd;
+            ED_TRY(dot_partials(n, Y[q], x, partial, &nd, st));
+            ED_TRY(finalize_sum(partial, nd, coef + q, 0, st));
+            ED_TRY(lanczos_update(n, Y[q], x, coef, (int)q, partial, nb, st));
+        }
+        return ED_OK;
+    }
+    // Ritz analysis of T (a[0..M], b[1..M]) with residual norm b[M+1]: good pairs not yet kept are
+    // formed from the basis and kept; *found = number of new ones
+    int analyse(int M, const double *a, const double *b, hipStream_t st, int *found)
+    {
+        const size_t before = Y.size();
+        ++analyses;
+        const double eps = std::numeric_limits<double>::epsilon();
+        double tn = 1e-300;
+        for (int k = 0; k <= M; ++k) tn = std::fmax(tn, std::fabs(a[k]) + (k > 0 ? b[k] : 0.0) + b[k + 1]);
+        std::vector<double> d(a, a + M + 1), e((size_t)M + 1, 0.0), z((size_t)M + 1);
+        for (int i = 0; i < M; ++i) e[i] = b[i + 1];
+        ql_eig(M + 1, d.data(), e.data(), nullptr);
+        std::sort(d.begin(), d.end());
+        for (int i = 0; i <= M; ++i) {
+            const double th = d[i];
+            bool known = false;
+            for (double t : theta) known = known || std::fabs(t - th) <= std::sqrt(eps) * tn;
+            if (known) continue;
+            inverse_iteration(M + 1, a, b + 1, th, z.data());
+            if (!(std::fabs(b[M + 1] * z[M]) <= std::sqrt(eps) * tn)) continue;
+            double *y = nullptr;
+            ED_TRY(dmalloc((void **)&y, sizeof(double) * (size_t)n));
+            Y.push_back(y);
+            theta.push_back(th);
+            ED_TRY(combine(M, z.data(), y, st));
+            ED_TRY(ed_normalize(n, 0, y, nullptr, st));
+            if (cvec) {
+                double *iy = nullptr;
+                ED_TRY(dmalloc((void **)&iy, sizeof(double) * (size_t)n));
+                Y.push_back(iy);
+                theta.push_back(th);
+                hipLaunchKernelGGL(k_times_i, dim3((unsigned)((n / 2 + 255) / 256)), dim3(256), 0, st, n / 2,
+                                   (const double *)y, iy);
+                ED_CHECK_LAUNCH();
+            }
+        }
+        *found = (int)(Y.size() - before);
+        return ED_OK;
+    }
+};
+
+// statistics of the calling thread's last ed_lanczos_eigen with selective reorthogonalisation
+struct SoStats {
+    int good = 0, analyses = 0, purges = 0;
+    double omega_max = 0.0;
+};
+static thread_local SoStats t_so_stats;
+
 } // namespace ed
 
 using namespace ed;
+
+extern "C" int ed_lanczos_so_stats(int *good, int *analyses, int *purges, double *omega_max)
+{
+    if (good) *good = t_so_stats.good;
+    if (analyses) *analyses = t_so_stats.analyses;
+    if (purges) *purges = t_so_stats.purges;
+    if (omega_max) *omega_max = t_so_stats.omega_max;
+    return ED_OK;
+}
+
+extern "C" int ed_lanczos_orthogonality(int n, const double *a, const double *b, double *omega_max)
+{
+    if (n < 1 || !a || !b || !omega_max) return fail(ED_EINVAL, "ed_lanczos_orthogonality: bad arguments");
+    OmegaMonitor om;
+    omega_max[0] = 0.0;
+    for (int j = 1; j < n; ++j) omega_max[j] = om.step(a, b);
+    return ED_OK;
+}
 
 extern "C" int ed_tridiag_lowest(int n, const double *d, const double *e, double *lambda, double *z)
 {
@@ -296,9 +436,12 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
     const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
     const int every = cfg && cfg->check_every > 0 ? cfg->check_every : 10;
-    const bool reorth = cfg && cfg->reorth;
+    const int rmode = cfg ? cfg->reorth : 0;
+    if (rmode < 0 || rmode > 2) return fail(ED_EINVAL, "ed_lanczos_eigen: reorth must be 0, 1 or 2");
+    const bool reorth = rmode == 1, selective = rmode == 2;
     hipStream_t st = stream_of(stream);
     const int64_t n = h->args.dim_up * h->args.dim_dn;
+    t_so_stats = SoStats();
     // real sectors with a slab layout (ed_hubbard_slab_layout, or built here for sectors of 2^27
     // amplitudes and more) run the whole recurrence on it: the start vector is converted once and
     // the Ritz vector converted back at the end
@@ -323,7 +466,7 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     // holds the layout copy of f during the forward recurrence while f keeps the start vector
     // (reference layout) for the rebuild, which then accumulates the Ritz vector in f's buffer.
     // The solve then holds five vectors: f, evec, the two work vectors and the scratch copy.
-    const bool inplace = slab && want_vec && nreal == n;
+    const bool inplace = slab && want_vec && nreal == n && !selective;
     if (inplace) {
         ED_TRY(slab_from_rowmajor(*h, (const double *)d_f, (double *)d_evec, st));
         F = (double *)d_evec;
@@ -351,7 +494,10 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     double *A = F, *B = w.buf[0], *C = w.buf[1];
     int N;
     bool stop = false;
-    const bool vfree = slab && !reorth; // reorthogonalisation works on a stored v
+    const bool vfree = slab && !reorth && !selective; // reorthogonalisation works on a stored v
+    SelOrth so;
+    so.n = nreal;
+    so.cvec = cvec;
     for (N = 0; N < maxN && !stop; ++N) {
         if (vfree) {
             ED_TRY(lanczos_step_vfree(*h, nreal, A, B, C, w, N, st));
@@ -361,10 +507,18 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
             C = t;
         } else {
             ED_TRY(lanczos_step(*h, cvec, nreal, A, B, C, w, N, reorth, slab, st));
+            if (selective) ED_TRY(so.store(N, C, st)); // v_N
             double *t = A;
             A = B;
             B = C;
             C = t;
+        }
+        if (selective && !so.Y.empty()) {
+            // f_{N+1} (in A) against the good Ritz vectors; b[N+1] recomputed
+            int64_t nbk;
+            ED_TRY(so.purge(A, w.partial, &nbk, st));
+            ED_TRY(finalize_sum(w.partial, nbk, w.bdev + N + 1, 1, st));
+            ++so.purges;
         }
         if (N % every == every - 1 || N == maxN - 1) {
             ED_HIP(hipMemcpyAsync(araw.data(), w.adev, sizeof(double) * (size_t)(N + 1), hipMemcpyDeviceToHost, st));
@@ -376,6 +530,21 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
                 for (int k = 0; k <= N; ++k) a[k] = araw[k] + dl[k];
             else
                 for (int k = 0; k <= N; ++k) a[k] = araw[k];
+            if (selective && b[N + 1] > 0.0 && std::isfinite(a[N])) {
+                int found = 0;
+                ED_TRY(so.analyse(N, a.data(), b.data(), st, &found));
+                if (found) {
+                    // v_N (in B) and f_{N+1} (in A) against every good Ritz vector; the kept copy
+                    // of v_N follows
+                    int64_t nbk;
+                    ED_TRY(so.purge(B, w.partial, &nbk, st));
+                    ED_TRY(so.purge(A, w.partial, &nbk, st));
+                    ED_TRY(finalize_sum(w.partial, nbk, w.bdev + N + 1, 1, st));
+                    ED_HIP(hipMemcpyAsync(so.vec(N), B, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
+                    ED_HIP(hipMemcpyAsync(&b[N + 1], w.bdev + N + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+                    ED_HIP(hipStreamSynchronize(st));
+                }
+            }
             int M = N; // last usable step
             for (int k = processed; k <= N; ++k) {
                 if (!(b[k + 1] > 0.0) || !std::isfinite(a[k])) { // breakdown / invariant subspace
@@ -404,6 +573,15 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     }
     if (lastN < 0) return fail(ED_EINVAL, "ed_lanczos_eigen: no steps run");
     const int Nk = lastN;
+    if (selective) {
+        t_so_stats.good = (int)so.theta.size() / (cvec ? 2 : 1);
+        t_so_stats.analyses = so.analyses;
+        t_so_stats.purges = so.purges;
+    } else if (Nk >= 1) {
+        // the monitor's estimate of the loss of orthogonality over the run
+        OmegaMonitor om;
+        for (int j = 1; j <= Nk; ++j) t_so_stats.omega_max = std::fmax(t_so_stats.omega_max, om.step(a.data(), b.data()));
+    }
 
     if (!want_vec) {
         ED_HIP(hipStreamSynchronize(st));
@@ -413,8 +591,11 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         if (status != ED_OK) return fail(status, "ed_lanczos_eigen: convergence test not met in %d steps", maxN);
         return ED_OK;
     }
-    // Ritz vector: rerun the recurrence from the saved start (HubMatrix_GPU.cuh:600-612)
-    if (inplace) {
+    // Ritz vector: with the basis kept, V z; otherwise rerun the recurrence from the saved start
+    // (HubMatrix_GPU.cuh:600-612)
+    if (selective) {
+        ED_TRY(so.combine(Nk, z.data(), ev, st));
+    } else if (inplace) {
         A = w.buf[0];
         B = w.buf[1];
         C = (double *)d_evec;
@@ -426,8 +607,8 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
         C = w.buf[1];
         ED_HIP(hipMemcpyAsync(A, ev, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, st));
     }
-    ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
-    for (int i = 0; i < Nk + 1; ++i) {
+    if (!selective) ED_HIP(hipMemsetAsync(ev, 0, sizeof(double) * (size_t)nreal, st));
+    for (int i = 0; !selective && i < Nk + 1; ++i) {
         if (vfree) { // update fused, no v
             ED_TRY(slab_lanczos_ritz_vfree(*h, A, B, C, w.bdev, w.adev, z[i], ev, i, st));
             double *t = B;

@@ -38,11 +38,16 @@ class EEmatHubAB_GPU:
         return out[0], out[1], nsv.value
 
     def spectrum(self, psi: torch.Tensor) -> np.ndarray:
-        """all eigenvalues of rho_A, descending"""
+        """all eigenvalues of rho_A, descending (one solve: rho_A has at most 4^min(|A|, |B|)
+        eigenvalues, the number of (up, dn) configurations of the smaller side)"""
+        if psi.numel() != self.dimUP * self.dimDN:
+            raise ValueError("psi has the wrong length")
+        nA = len(self.Asite)
+        cap = 4 ** min(nA, self.N - nA)
+        lam = np.zeros(cap)
         n = C.c_int64()
         call("ed_entanglement_spectrum", self.N, self.Qup, self.Qdn, self.maskA, 1 if psi.is_complex() else 0,
-             dptr(psi), None, 0, C.byref(n), stream_ptr())
-        lam = np.zeros(max(1, n.value))
-        call("ed_entanglement_spectrum", self.N, self.Qup, self.Qdn, self.maskA, 1 if psi.is_complex() else 0,
-             dptr(psi), lam.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n), stream_ptr())
+             dptr(psi), lam.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n), stream_ptr())
+        if n.value > cap:
+            raise RuntimeError("entanglement spectrum longer than its bound")
         return lam[: n.value]

@@ -252,18 +252,41 @@ class Thrust_Lin:
         return w
 
 
+_REORTH = {False: 0, True: 1, 0: 0, 1: 1, 2: 2, "none": 0, "dgks": 1, "selective": 2}
+
+
+def lanczos_orthogonality(a, b) -> np.ndarray:
+    """omega[j] = estimated max_{k<j} |v_j . v_k| of a Lanczos run without reorthogonalisation,
+    from its a[0..n-1] and b[0..n-1] (b[0] the start norm): the omega recurrence (Simon 1984),
+    ed_lanczos_orthogonality.  Host only."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n = len(a)
+    if n < 1 or len(b) < n:
+        raise ValueError("lanczos_orthogonality: need len(b) >= len(a) >= 1")
+    out = np.zeros(n)
+    P = C.POINTER(C.c_double)
+    call("ed_lanczos_orthogonality", n, a.ctypes.data_as(P), b.ctypes.data_as(P), out.ctypes.data_as(P))
+    return out
+
+
 class Lanczos(Thrust_Lin):
-    """Lanczos<T> (HubMatrix_GPU.cuh:477-613).  a, b are host lists; b starts as [0.0]."""
+    """Lanczos<T> (HubMatrix_GPU.cuh:477-613).  a, b are host lists; b starts as [0.0].
+    reorth: 0 / False (the reference's recurrence), 1 / True / "dgks" (a second Gram-Schmidt pass
+    of f against v_N), 2 / "selective" (Parlett-Scott selective orthogonalisation against the
+    converged Ritz vectors, the Lanczos basis kept on the device)."""
 
     def __init__(self, dimUP: int, dimDN: int, max_steps: int = 1000, tol: float | None = None,
-                 check_every: int = 10, reorth: bool = False):
+                 check_every: int = 10, reorth=False):
         super().__init__(dimUP, dimDN)
         self.a: list[float] = []
         self.b: list[float] = [0.0]
         self.max_steps = int(max_steps)
         self.tol = 0.01 * math.sqrt(np.finfo(float).eps) if tol is None else float(tol)
         self.check_every = int(check_every)
-        self.reorth = bool(reorth)
+        if reorth not in _REORTH:
+            raise ValueError(f"reorth must be one of {sorted(map(str, _REORTH))}")
+        self.reorth = _REORTH[reorth]
         self.nsteps = 0
         self.converged = False
 
@@ -304,6 +327,12 @@ class Lanczos(Thrust_Lin):
         if st == ED_ENOCONV:
             raise EdError(st, f"Lanczos convergence test not met in {self.max_steps} steps (E0={ev.value})")
         return ev.value
+
+    def so_stats(self) -> dict:
+        """selective-orthogonalisation statistics of this thread's last eigen() (ed_lanczos_so_stats)"""
+        g, an, pu, om = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        call("ed_lanczos_so_stats", C.byref(g), C.byref(an), C.byref(pu), C.byref(om))
+        return {"good_ritz_vectors": g.value, "analyses": an.value, "purges": pu.value, "omega_max": om.value}
 
     def tridiagonal(self, Hub: Hubbard, f: torch.Tensor, maxN: int = 200):
         """Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): appends maxN a's and b's."""

@@ -260,7 +260,13 @@ typedef struct {
     int max_steps;   /* maxN (reference 1000) */
     double tol;      /* tolconv (reference 0.01*sqrt(eps)) */
     int check_every; /* convergence test cadence (reference 10) */
-    int reorth;      /* 1: second Gram-Schmidt pass of f against v_N (DGKS correction) */
+    int reorth;      /* 0: none (the reference); 1: second Gram-Schmidt pass of f against v_N
+                      * (DGKS correction); 2: selective orthogonalisation (Parlett-Scott): the
+                      * Lanczos basis is kept on the device, a Ritz analysis at every convergence
+                      * check keeps the converged ("good") Ritz vectors, and every new f is
+                      * orthogonalised against them (v_N too when one is found); the Ritz vector
+                      * is then V z, with no second recurrence.  ED_ENOMEM when the basis does
+                      * not fit. */
 } ed_lanczos_cfg;
 
 /* Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613).  d_f: start vector in (any norm),
@@ -276,6 +282,20 @@ int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cf
  * the slab layout when the operator has one, or gets one as in ed_lanczos_eigen. */
 int ed_lanczos_tridiagonal(ed_hubbard h, int complex_vectors, int nsteps, const void *d_f, double *a_out,
                            double *b_out, void *stream);
+
+/* Statistics of the calling thread's last ed_lanczos_eigen (no counterpart in the reference):
+ * with reorth = 2, the good Ritz vectors kept, the Ritz analyses and the steps whose f was
+ * orthogonalised against them; otherwise omega_max, the omega monitor's estimate of the largest
+ * |v_j . v_k| of the run (ed_lanczos_orthogonality over its a and b).  Any pointer may be NULL. */
+int ed_lanczos_so_stats(int *good, int *analyses, int *purges, double *omega_max);
+
+/* Orthogonality monitor of a Lanczos run (no counterpart in the reference, whose recurrence
+ * has no reorthogonalisation, HubMatrix_GPU.cuh:573-582): from a[0..n-1] and b[0..n-1] (b[0] the
+ * start norm, b[k] coupling v_{k-1} and v_k) the omega recurrence (Simon 1984) estimates
+ * omega_max[j] = max_{k<j} |v_j . v_k| for j = 0..n-1, assuming no reorthogonalisation.
+ * Semi-orthogonality, omega <= sqrt(eps), keeps the tridiagonal's Ritz values accurate to
+ * working precision.  Host only. */
+int ed_lanczos_orthogonality(int n, const double *a, const double *b, double *omega_max);
 
 /* lowest eigenpair of a symmetric tridiagonal (replaces LAPACKf77::dsted, MAGMA_LAPACK.h:296-326,
  * for the Lanczos test): d[n], e[n-1]; z[n] (may be NULL) gets the unit eigenvector. */
@@ -404,6 +424,12 @@ int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t maskA, int com
 /* all eigenvalues of rho_A, descending, into lambda_out (capacity cap); *nlambda = total count */
 int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                              double *lambda_out, int64_t cap, int64_t *nlambda, void *stream);
+/* Kernel-time statistics of the three calls above, summed over calls since the last reset (no
+ * counterpart in the reference: a measurement aid).  *gram_ms: the rho_A Gram kernels' durations,
+ * *gram_flops: their SYRK FLOPs (k (k+1) K per real k x K block, 4x that for complex), *eig_ms:
+ * the durations of the per-block eigenvalue (and SVD) stage; each duration is timed by events on
+ * the stream the work ran on.  Any pointer may be NULL; reset != 0 zeroes the sums after reading. */
+int ed_entanglement_stats(double *gram_ms, double *gram_flops, double *eig_ms, int reset);
 
 #ifdef __cplusplus
 }

@@ -57,3 +57,51 @@ def test_kagome18y_first_100_lanczos_steps(layout, monkeypatch):
     from scipy.linalg import eigvalsh_tridiagonal
     theta = eigvalsh_tridiagonal(a, b[1:n], select="i", select_range=(0, 0))[0]
     assert abs(theta - gold["ritz_lowest"]) < 1e-10 * abs(gold["ritz_lowest"]), (theta, gold["ritz_lowest"])
+
+
+GOLD_E0 = os.path.join(os.path.dirname(__file__), "golden", "kagome18y_e0_oracle.json")
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD_E0), reason="fixture not generated")
+def test_kagome18y_ground_state_and_entropy():
+    """The plain method at GPU scale, end to end: the converged Lanczos ground state
+    (Lanczos<T>::eigen, HubMatrix_GPU.cuh:561-613, reference convergence test, Ritz vector rebuilt
+    by recomputation) of the 18-site model on the slab layout the solver builds by itself, against
+    the oracle's converged E0 (tests/golden/kagome18y_e0_oracle.json) to 1e-10 with the same step
+    count; then the 9|9 entanglement spectrum (entangle_entropy.h:231-353) at full size, checked by
+    what a pure state fixes: sum lambda = 1, lambda >= 0, S(sites 0-8) = S(sites 9-17)."""
+    gold = json.load(open(GOLD_E0))
+    print(f"kagome18y 9/9 U=8 ground state at D = 2.36e9 (oracle E0 {gold['E0']:.15f}, {gold['nsteps']} steps)",
+          flush=True)
+    hq = H_TBq(18, 9)
+    hq.add_ops(L.hubbard_ops(L.KAGOME18Y, t=-1.0))
+    H = Hubbard(hq, hq, 8.0)
+    f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(f)
+    lz = Lanczos(H.dimUP, H.dimDN)
+    lz.random_vector(f)
+    e0 = lz.eigen(H, f, psi)
+    assert H.slab_len_query() == H.dim  # the solver ran on the slab layout (no padding at 48,620)
+    H.slab_layout(False)
+    print(f"E0 {e0:.15f} in {lz.nsteps} steps", flush=True)
+    assert lz.nsteps == gold["nsteps"]
+    assert abs(e0 - gold["E0"]) <= 1e-10 * abs(gold["E0"]), (e0, gold["E0"])
+    assert abs(float(torch.linalg.vector_norm(psi)) - 1.0) <= 1e-12
+    hpsi = H.apply(psi)  # f's buffer is free: the residual of the Ritz pair
+    res = float(torch.linalg.vector_norm(hpsi - e0 * psi))
+    del hpsi, f
+    torch.cuda.empty_cache()
+    print(f"|H psi - E0 psi| = {res:.2e}", flush=True)
+    assert res <= 1e-6 * abs(e0)
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd.entangle import EEmatHubAB_GPU
+    lam = EEmatHubAB_GPU(list(range(9)), 9, hq, hq).spectrum(psi)
+    assert len(lam) == 4 ** 9 and lam.min() >= 0.0
+    assert abs(lam.sum() - 1.0) <= 1e-12
+    s_a = -float(np.sum(lam[lam > 0] * np.log(lam[lam > 0])))
+    import ctypes as C
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
+    sb = (C.c_double * 1)()
+    call("ed_entanglement_entropy_vn", 18, 9, 9, ((1 << 18) - 1) ^ ((1 << 9) - 1), 0, dptr(psi), sb, stream_ptr())
+    s_b = sb[0]
+    print(f"S_vN(0-8) {s_a:.15f}, S_vN(9-17) {s_b:.15f}", flush=True)
+    assert abs(s_a - s_b) <= 1e-10
