@@ -1,8 +1,9 @@
 """Batched ground states and the twist-flux Drude-weight sweep.
 
-BatchedLanczos solves many independent Hubbard instances together (ed_batch_*, one launch
-per kernel for all instances, ten-step hipGraph replays); the plain path runs the same
-instances one after another through Lanczos.eigen without a Ritz vector.  Both follow the
+BatchedLanczos solves many independent Hubbard instances together (ed_batch_*: one launch
+per kernel for all instances, the convergence test on the device every ten steps, fifty-step
+hipGraph replays); the plain path runs the same instances one after another through
+Lanczos.eigen without a Ritz vector.  Both follow the
 reference's Lanczos<T>::eigen (MBL_test/include/HubMatrix_GPU.cuh:561-613) per instance.
 
 Drude weight from the flux curvature (Kohn): for every particle-number sector s of the

@@ -18,9 +18,13 @@
 //     offsets, binary search for the owner);
 //   * each copy keeps its own Lanczos scalars, active flag and deterministic reductions
 //     (per-tile, per-copy partials; one finalize block per copy);
-//   * the step number and the active flags live in device memory, so ten steps (the
-//     reference's convergence-check cadence) are captured once in a hipGraph per buffer
-//     rotation phase and replayed; the host reads a/b only at the checks.
+//   * the step number, the active flags and the convergence test live on the device: after
+//     every ten steps (the reference's check cadence) a check kernel runs the test of every
+//     active copy (Sturm bisection and inverse iteration of its tridiagonal, the same code as
+//     the host test of ed_lanczos_eigen, ed_tridiag.hpp) and switches converged copies off, so
+//     a graph of several ten-step chunks with their checks is captured once per buffer rotation
+//     phase and replayed; the host only reads the count of copies still running between
+//     replays.
 // Each copy follows the recurrence and convergence test of ed_lanczos_eigen (eigenvalue only),
 // with its own instance's start vector (GenRand of the sector length).
 #include <algorithm>
@@ -28,12 +32,12 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
-#include <thread>
 #include <vector>
 
 #include "ed_common.hpp"
 #include "ed_hubbard.hpp"
 #include "ed_hv_otf_body.hpp"
+#include "ed_tridiag.hpp"
 
 namespace ed {
 
@@ -95,15 +99,16 @@ __device__ __forceinline__ uint32_t gdesc(const OtfOp &op, int o, uint32_t s, ui
 // stay in the XCD's L2 while the slab is processed.
 template <typename V, typename TV, int GU>
 __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
-                                              const int *__restrict__ active, const double *__restrict__ bdev,
-                                              int ldb, const int *__restrict__ dstep, const double *x, double *y, int kp)
+                                              const int *__restrict__ active, const int *__restrict__ gact,
+                                              const double *__restrict__ bdev, int ldb,
+                                              const int *__restrict__ dstep, const double *x, double *y, int kp)
 {
     extern __shared__ double s_upv[]; // value table, TV [nops_up][G][2]
     __shared__ uint32_t s_desc[GUMAX][64];
     __shared__ int s_cnt[GUMAX];
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
-    if (T >= ntiles) return;
+    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     if (!g.has_up) return;
     TV *s_vt = reinterpret_cast<TV *>(s_upv);
@@ -168,15 +173,16 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
 // no v_{N-1} term.
 template <typename V, typename TV, bool AXPY, int GR>
 __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
-                                              const int *__restrict__ active, const double *__restrict__ bdev,
-                                              int ldb, const int *__restrict__ dstep, const double *x,
-                                              const double *xp, double *y, double *partial, int kp)
+                                              const int *__restrict__ active, const int *__restrict__ gact,
+                                              const double *__restrict__ bdev, int ldb,
+                                              const int *__restrict__ dstep, const double *x, const double *xp,
+                                              double *y, double *partial, int kp)
 {
     extern __shared__ double s_dnv[]; // value table TV [nops_dn][G][2], then descriptors [nops_dn][CW]
     __shared__ double s_red[256];
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
-    if (T >= ntiles) return;
+    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     TV *s_vt = reinterpret_cast<TV *>(s_dnv);
     uint32_t *s_gd = reinterpret_cast<uint32_t *>(s_vt + g.a.dn.nops * g.G * 2);
@@ -277,14 +283,14 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
 
 // f = -a_k v + f with v = x / b_N over GCH * CW elements x G copies; per-copy partials of |f|^2
 __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles,
-                                                  const int *__restrict__ active, const double *__restrict__ adev,
-                                                  const double *__restrict__ bdev, int lda,
-                                                  const int *__restrict__ dstep, int cvec, const double *x,
+                                                  const int *__restrict__ active, const int *__restrict__ gact,
+                                                  const double *__restrict__ adev, const double *__restrict__ bdev,
+                                                  int lda, const int *__restrict__ dstep, int cvec, const double *x,
                                                   double *f, double *partial)
 {
     __shared__ double s_red[256];
     const int64_t T = blockIdx.x;
-    if (T >= ntiles) return;
+    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     const int64_t ch = T - g.c_off;
     const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
@@ -606,22 +612,116 @@ extern "C" int ed_batch_destroy(ed_batch bb)
     return ED_OK;
 }
 
+// convergence state of one copy (device)
+struct CopyCheck {
+    double anorm, ev;
+    int processed, last, conv, pad;
+};
+
+// The convergence test of every active copy after a chunk (ed_lanczos_eigen's test, HubMatrix_GPU.cuh:
+// 588-597): the copy's a[0..N-1], b[0..N] gathered from the step-major rows, anorm updated over the
+// new steps, the lowest eigenvalue of T by a 64-point multisection of the Gershgorin interval (each
+// round keeps the pair of neighbouring points where the Sturm count turns positive, until the two
+// ends are adjacent doubles: the bracket the host's bisection ends on, ed_tridiag.hpp), then its
+// vector by the host's inverse iteration; a copy that passes (or meets an invariant subspace, or
+// has run max_steps) is switched off, and its group's count of running copies drops.  One
+// wavefront per copy; the copy's a, b and the inverse iteration's five work arrays (the
+// eigenvector overwrites the last) and flags sit in LDS (LDS: 7 ldz doubles + ldz bytes) or in
+// ws / wsf (the same per copy).
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_g_check(int nc, int ld, const double *__restrict__ adev,
+                                                const double *__restrict__ bdev, const int *__restrict__ dchk,
+                                                int chunk, int *active, const int *__restrict__ cgroup, int *gact,
+                                                double tol, int max_steps, CopyCheck *cc, double *ws, char *wsf,
+                                                int ldz, int *remaining)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double s_chk[];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    if (c >= nc || !active[c]) return; // uniform over the wavefront
+    const int Ne = *dchk + chunk, Nl = Ne - 1; // steps done at the end of the chunk checked
+    double *a = LDS ? s_chk : ws + (size_t)c * 7 * ldz, *b = a + ldz, *w5 = b + ldz;
+    char *sw = LDS ? reinterpret_cast<char *>(s_chk + 7 * (size_t)ldz) : wsf + (size_t)c * ldz;
+    for (int k = lane; k <= Nl; k += 64) a[k] = adev[(int64_t)k * ld + c];
+    for (int k = lane; k <= Ne; k += 64) b[k] = bdev[(int64_t)k * ld + c];
+    __syncthreads();
+    CopyCheck st = cc[c];
+    int M = Nl;
+    for (int k = st.processed; k <= Nl; ++k)
+        if (!(b[k + 1] > 0.0) || !isfinite(a[k])) {
+            M = k;
+            break;
+        }
+    for (int k = st.processed; k <= M; ++k) {
+        if (k == 0) st.anorm = fabs(a[0] + b[1]);
+        else st.anorm = fmax(st.anorm, b[k] + fabs(a[k] + b[k + 1]));
+    }
+    st.processed = M + 1;
+    const int n = M + 1;
+    const double *d = a, *e = b + 1;
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
+    }
+    const double pad = 1e-12 * (hi - lo + 1.0);
+    lo -= pad;
+    hi += pad;
+    for (int round = 0; round < 64; ++round) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break; // adjacent doubles (uniform)
+        const double w = hi - lo;
+        const double xj = lo + w * ((double)(lane + 1) / 65.0);
+        const unsigned long long above = __ballot(tri_sturm_below(n, d, e, xj) > 0);
+        if (above == 0ull) {
+            lo = lo + w * (64.0 / 65.0);
+        } else {
+            const int j = __ffsll((long long)above) - 1;
+            const double xh = lo + w * ((double)(j + 1) / 65.0), xl = j > 0 ? lo + w * ((double)j / 65.0) : lo;
+            lo = xl;
+            hi = xh;
+        }
+    }
+    if (lane != 0) return;
+    const double lam = 0.5 * (lo + hi);
+    double *z = w5 + 4 * (size_t)n; // the iteration's own vector, overwritten in place
+    tri_inverse_iteration(n, d, e, lam, z, w5, sw);
+    st.ev = lam;
+    st.last = M;
+    bool stop = false, conv = false;
+    if (M < Nl || !(b[M + 1] > 0.0)) stop = conv = true;
+    else if (fabs(b[M + 1] * z[M]) < tol * st.anorm && fabs(z[0]) > tol * st.anorm) stop = conv = true;
+    else if (Ne >= max_steps) stop = true;
+    if (conv) st.conv = 1;
+    cc[c] = st;
+    if (stop) {
+        active[c] = 0;
+        atomicSub(&gact[cgroup[c]], 1);
+        atomicSub(remaining, 1);
+    }
+}
+
+// the checks' own step count (the step kernels' counter runs ahead while a check runs beside them)
+__global__ void k_g_chk_bump(int *dchk, int chunk) { dchk[0] += chunk; }
+
 // one step: x = f_N, xp = f_{N-1}, y <- f_{N+1}
 template <typename V, typename TV>
 static int enqueue_step(const Batch &B, const double *x, const double *xp, double *y, double *partial,
-                        const int *active, double *adev, double *bdev, int ld, int *dstep, bool first, hipStream_t st)
+                        const int *active, const int *gact, double *adev, double *bdev, int ld, int *dstep,
+                        bool first, hipStream_t st)
 {
     const int64_t per1 = (B.t1 + 7) / 8, per2 = (B.t2 + 7) / 8;
     if (B.gu == 4)
         hipLaunchKernelGGL((k_g_up<V, TV, 4>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, bdev, ld, dstep, x, y, B.kpu);
+                           B.t1, per1, active, gact, bdev, ld, dstep, x, y, B.kpu);
     else
         hipLaunchKernelGGL((k_g_up<V, TV, 1>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, bdev, ld, dstep, x, y, B.kpu);
+                           B.t1, per1, active, gact, bdev, ld, dstep, x, y, B.kpu);
     const size_t sh = B.lds_dn;
 #define ED_GDN(AX, R_)                                                                                        \
     hipLaunchKernelGGL((k_g_dn<V, TV, AX, R_>), dim3((unsigned)(per2 * 8)), dim3(256), sh, st, B.dgr, B.dtg2, B.t2, \
-                       per2, active, bdev, ld, dstep, x, xp, y, partial, B.kp)
+                       per2, active, gact, bdev, ld, dstep, x, xp, y, partial, B.kp)
     if (B.gr == 8) {
         if (first) ED_GDN(false, 8);
         else ED_GDN(true, 8);
@@ -632,8 +732,8 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
 #undef ED_GDN
     hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 0, adev,
                        ld, dstep);
-    hipLaunchKernelGGL(k_g_update, dim3((unsigned)B.tc), dim3(256), 0, st, B.dgr, B.dtgc, B.tc, active, adev, bdev, ld,
-                       dstep, B.cvec ? 1 : 0, x, y, partial);
+    hipLaunchKernelGGL(k_g_update, dim3((unsigned)B.tc), dim3(256), 0, st, B.dgr, B.dtgc, B.tc, active, gact, adev, bdev,
+                       ld, dstep, B.cvec ? 1 : 0, x, y, partial);
     hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 1, bdev,
                        ld, dstep);
     hipLaunchKernelGGL(k_g_bump, dim3(1), dim3(1), 0, st, dstep);
@@ -663,38 +763,55 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
         if (ev) (void)hipEventDestroy(ev);
         return fail(ED_EHIP, "ed_batch_lowest: stream setup failed");
     }
-    const int nc = B->ncopies, ld = maxN + 3;
+    const int nc = B->ncopies, ld = maxN + 3, ldz = maxN + 3;
+    // the check kernel keeps its arrays in LDS up to max_steps of about 1070
+    constexpr size_t kCheckLdsMax = 60 * 1024;
+    const size_t chk_lds = (7 * sizeof(double) + 1) * (size_t)ldz;
     double *buf[3] = {nullptr, nullptr, nullptr}, *partial = nullptr, *adev = nullptr, *bdev = nullptr,
-           *tmp = nullptr;
-    int *active = nullptr, *dstep = nullptr;
-    // pinned host staging for every transfer of the solve (a, b readback; b_0, flags, step upload)
-    double *pa = nullptr, *pb = nullptr, *dpart = nullptr; // dpart: start-vector norm partials + result
+           *tmp = nullptr, *ws = nullptr;
+    int *active = nullptr, *dstep = nullptr, *dremaining = nullptr, *dchk = nullptr, *gact = nullptr;
+    // the convergence checks run on a side stream, each beside the next chunk's steps
+    hipStream_t side = nullptr;
+    hipEvent_t ev_chunk = nullptr, ev_check = nullptr, ev_count[2] = {nullptr, nullptr};
+    char *wsf = nullptr;
+    CopyCheck *cc = nullptr;
+    double *dpart = nullptr; // start-vector norm partials + result
     int64_t npart = 0;
-    int *pact = nullptr;
-    hipStream_t fs = nullptr; // flag uploads while a chunk runs
-    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    int *pact = nullptr; // pinned: active flags, step counter, copies still running
+    // graphs of kGraphChunks chunks (and of one chunk, for the tail) per buffer rotation phase
+    constexpr int kGraphChunks = 5;
+    hipGraphExec_t gexec[2][3] = {};
+    hipGraph_t graph[2][3] = {};
     int s = ED_OK;
     auto cleanup = [&] {
         (void)hipStreamSynchronize(st);
-        if (fs) {
-            (void)hipStreamSynchronize(fs);
-            (void)hipStreamDestroy(fs);
-        }
-        for (int k = 0; k < 3; ++k) {
-            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
-            if (graph[k]) (void)hipGraphDestroy(graph[k]);
-            dfree(buf[k]);
-        }
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 3; ++k) {
+                if (gexec[q][k]) (void)hipGraphExecDestroy(gexec[q][k]);
+                if (graph[q][k]) (void)hipGraphDestroy(graph[q][k]);
+            }
+        for (int k = 0; k < 3; ++k) dfree(buf[k]);
         dfree(partial);
         dfree(adev);
         dfree(bdev);
         dfree(active);
         dfree(dstep);
+        dfree(dremaining);
+        dfree(dchk);
+        dfree(gact);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (ev_chunk) (void)hipEventDestroy(ev_chunk);
+        if (ev_check) (void)hipEventDestroy(ev_check);
+        for (hipEvent_t e : ev_count)
+            if (e) (void)hipEventDestroy(e);
         dfree(tmp);
         dfree(dpart);
-        if (pa) (void)hipHostFree(pa);
-        if (pb) (void)hipHostFree(pb);
+        dfree(ws);
+        dfree(wsf);
+        dfree(cc);
         if (pact) (void)hipHostFree(pact);
         (void)hipStreamDestroy(st);
         (void)hipEventDestroy(ev);
@@ -709,17 +826,24 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             (s = dmalloc((void **)&adev, sizeof(double) * (size_t)nc * ld)) ||
             (s = dmalloc((void **)&bdev, sizeof(double) * (size_t)nc * ld)) ||
             (s = dmalloc((void **)&active, sizeof(int) * nc)) || (s = dmalloc((void **)&dstep, sizeof(int) * 2)) ||
+            (s = dmalloc((void **)&dremaining, sizeof(int))) || (s = dmalloc((void **)&dchk, sizeof(int))) ||
+            (s = dmalloc((void **)&gact, sizeof(int) * (size_t)std::max(B->ng, 1))) ||
             (s = dmalloc((void **)&tmp, sizeof(double) * (size_t)maxe * (B->cvec ? 2 : 1))) ||
-            (s = dmalloc((void **)&dpart, sizeof(double) * (size_t)(npart + 1))))
+            (s = dmalloc((void **)&dpart, sizeof(double) * (size_t)(npart + 1))) ||
+            (s = dmalloc((void **)&ws, sizeof(double) * 7 * (size_t)nc * ldz)) ||
+            (s = dmalloc((void **)&wsf, (size_t)nc * ldz)) ||
+            (s = dmalloc((void **)&cc, sizeof(CopyCheck) * (size_t)nc)))
             break;
-        if (hipHostMalloc((void **)&pa, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&pb, sizeof(double) * (size_t)nc * ld, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(2 * nc + 2), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(nc + 3 + B->ng), hipHostMallocDefault) != hipSuccess) {
             s = fail(ED_ENOMEM, "ed_batch_lowest: pinned staging allocation failed");
             break;
         }
-        if (hipStreamCreateWithFlags(&fs, hipStreamNonBlocking) != hipSuccess) {
-            s = fail(ED_EHIP, "ed_batch_lowest: stream creation failed");
+        if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_chunk, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_check, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_count[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_count[1], hipEventDisableTiming) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_batch_lowest: stream setup failed");
             break;
         }
         // start vectors: each copy gets the normalised GenRand vector of its sector length (the
@@ -743,31 +867,30 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                                g.ncopy, NC, (const double *)tmp, buf[0] + g.voff);
         }
         if (s) break;
-        // host copies per copy (row c: a_0.., b_0..); on the device a and b are step-major
-        std::vector<double> hb(nc * (size_t)ld, 0.0), ha(nc * (size_t)ld, 0.0);
-        std::vector<int> hact(nc, 0);
-        for (int c = 0; c < nc; ++c) hact[c] = B->inst_of[c] >= 0 ? 1 : 0;
         if (hipGetLastError() != hipSuccess) {
             s = fail(ED_EHIP, "ed_batch_lowest: start vectors failed");
             break;
         }
-        for (int c = 0; c < nc; ++c) pact[c] = hact[c];
-        pact[2 * nc] = pact[2 * nc + 1] = 0;
+        int running = 0;
+        int *pgact = pact + nc + 3; // running copies per group
+        for (int q = 0; q < B->ng; ++q) pgact[q] = 0;
+        for (int c = 0; c < nc; ++c) {
+            pact[c] = B->inst_of[c] >= 0 ? 1 : 0;
+            running += pact[c];
+            pgact[B->cgroup[c]] += pact[c];
+        }
+        pact[nc] = pact[nc + 1] = 0;
+        pact[nc + 2] = running;
         if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(dstep, pact + 2 * nc, sizeof(int) * 2, hipMemcpyHostToDevice, st) ||
-            hipStreamSynchronize(st)) {
+            hipMemcpyAsync(dstep, pact + nc, sizeof(int) * 2, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(dremaining, pact + nc + 2, sizeof(int), hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(dchk, pact + nc, sizeof(int), hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(gact, pgact, sizeof(int) * (size_t)B->ng, hipMemcpyHostToDevice, st) ||
+            hipMemsetAsync(cc, 0, sizeof(CopyCheck) * (size_t)nc, st) || hipStreamSynchronize(st)) {
             s = fail(ED_EHIP, "ed_batch_lowest: upload failed");
             break;
         }
 
-        std::vector<int> last(nc, -1), done(nc, 0), conv(nc, 0), processed(nc, 0), todo;
-        std::vector<char> stopnow(nc, 0);
-        std::vector<double> anorm(nc, 0.0), z, ev_c(nc, 0.0);
-        int remaining = 0;
-        for (int c = 0; c < nc; ++c) {
-            if (B->inst_of[c] >= 0) ++remaining;
-            else done[c] = 1;
-        }
         int phase = 0, N = 0;
         auto enqueue = [&](int ph, int n0, int nstep) -> int {
             for (int k = 0; k < nstep; ++k) {
@@ -776,155 +899,108 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
                 const double *X = buf[(ph + k) % 3], *XP = buf[(ph + k + 2) % 3];
                 double *Y = buf[(ph + k + 1) % 3];
                 const bool first = (n0 + k) == 0;
-                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, adev, bdev, nc,
-                                                                     dstep, first, st)
-                                           : enqueue_step<cplx, double>(*B, X, XP, Y, partial, active, adev, bdev,
-                                                                       nc, dstep, first, st))
-                                : enqueue_step<double, double>(*B, X, XP, Y, partial, active, adev, bdev, nc, dstep,
-                                                               first, st);
+                int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, gact, adev, bdev,
+                                                                     nc, dstep, first, st)
+                                           : enqueue_step<cplx, double>(*B, X, XP, Y, partial, active, gact, adev,
+                                                                       bdev, nc, dstep, first, st))
+                                : enqueue_step<double, double>(*B, X, XP, Y, partial, active, gact, adev, bdev, nc,
+                                                               dstep, first, st);
                 if (r) return r;
             }
             return ED_OK;
         };
-        // one chunk runs ahead of the check: the GPU steps chunk k + 1 while the host checks
-        // chunk k; a copy found converged there is switched off during chunk k + 1
-        hipEvent_t evr[2] = {nullptr, nullptr};
-        for (int k = 0; k < 2 && !s; ++k)
-            if (hipEventCreateWithFlags(&evr[k], hipEventDisableTiming) != hipSuccess)
-                s = fail(ED_EHIP, "ed_batch_lowest: event creation failed");
-        int qN[2] = {0, 0}, qslot[2] = {0, 0}, nq = 0, slot = 0, Nc = 0;
-        auto launch_chunk = [&]() -> int {
-            const int chunk = std::min(every, maxN - N);
-            // the first ten steps contain the step-0 kernel: run them eagerly; later phases
-            // are captured once and replayed
-            if (use_graph && chunk == every && N > 0) {
-                if (!gexec[phase]) {
-                    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
-                        return fail(ED_EHIP, "ed_batch_lowest: capture begin failed");
-                    int r = enqueue(phase, N, chunk);
-                    hipError_t e = hipStreamEndCapture(st, &graph[phase]);
-                    if (r || e != hipSuccess ||
-                        hipGraphInstantiate(&gexec[phase], graph[phase], nullptr, nullptr, 0) != hipSuccess)
-                        return r ? r : fail(ED_EHIP, "ed_batch_lowest: graph capture failed");
-                }
-                if (hipGraphLaunch(gexec[phase], st) != hipSuccess)
-                    return fail(ED_EHIP, "ed_batch_lowest: graph launch failed");
-            } else if (int r = enqueue(phase, N, chunk)) {
-                return r;
+        // nch chunks of ten steps (the last one shorter at max_steps); after each, the
+        // device-side convergence test runs on the side stream while the next chunk steps on (a
+        // copy found converged is switched off from the chunk after; a kernel that still sees the
+        // old flag does wasted but harmless work: a copy's kernels touch only its own vectors,
+        // partial slots and a/b rows, and the test reads only the rows of finished steps).  The
+        // main stream waits for the last test.  Advances N and the rotation phase.
+        auto enqueue_chunks = [&](int nch) -> int {
+            for (int q = 0; q < nch && N < maxN; ++q) {
+                const int chunk = std::min(every, maxN - N);
+                if (int r = enqueue(phase, N, chunk)) return r;
+                ED_HIP(hipEventRecord(ev_chunk, st));
+                ED_HIP(hipStreamWaitEvent(side, ev_chunk, 0));
+                if (chk_lds <= kCheckLdsMax)
+                    hipLaunchKernelGGL(k_g_check<true>, dim3((unsigned)nc), dim3(64), chk_lds, side, nc, nc,
+                                       (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
+                                       (const int *)B->dcgroup, gact, tol, maxN, cc, ws, wsf, ldz, dremaining);
+                else
+                    hipLaunchKernelGGL(k_g_check<false>, dim3((unsigned)nc), dim3(64), 0, side, nc, nc,
+                                       (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
+                                       (const int *)B->dcgroup, gact, tol, maxN, cc, ws, wsf, ldz, dremaining);
+                hipLaunchKernelGGL(k_g_chk_bump, dim3(1), dim3(1), 0, side, dchk, chunk);
+                ED_CHECK_LAUNCH();
+                N += chunk;
+                phase = (phase + chunk) % 3;
             }
-            const int N0 = N;
-            N += chunk;
-            phase = (phase + chunk) % 3;
-            // the a rows N0..N-1 and b rows N0+1..N this chunk wrote, one contiguous range each
-            if (hipMemcpyAsync(pa + (size_t)N0 * nc, adev + (size_t)N0 * nc, sizeof(double) * nc * chunk,
-                               hipMemcpyDeviceToHost, st) ||
-                hipMemcpyAsync(pb + (size_t)(N0 + 1) * nc, bdev + (size_t)(N0 + 1) * nc,
-                               sizeof(double) * nc * chunk, hipMemcpyDeviceToHost, st) ||
-                hipEventRecord(evr[slot], st) != hipSuccess)
-                return fail(ED_EHIP, "ed_batch_lowest: readback failed");
-            qN[nq] = N;
-            qslot[nq] = slot;
-            ++nq;
-            slot ^= 1;
+            ED_HIP(hipEventRecord(ev_check, side));
+            ED_HIP(hipStreamWaitEvent(st, ev_check, 0));
             return ED_OK;
         };
-        while (!s) {
-            if (remaining > 0 && N < maxN && nq < 2 && (s = launch_chunk())) break;
-            if (nq == 0) break;
-            const int Ne = qN[0];
-            if (hipEventSynchronize(evr[qslot[0]]) != hipSuccess) {
-                s = fail(ED_EHIP, "ed_batch_lowest: readback wait failed");
+        // the first chunk holds the step-0 kernel: eagerly.  Then graphs of kGraphChunks chunks
+        // (one chunk near max_steps), captured once per phase and replayed, one submission ahead
+        // of the host: after each, the count of copies still running is copied into a pinned slot,
+        // and the host reads it once the next submission is queued.
+        auto submit = [&]() -> int {
+            const int left = (maxN - N) / every; // whole chunks left
+            const int q = left >= kGraphChunks ? 0 : 1, nch = q == 0 ? kGraphChunks : 1;
+            if (!use_graph || left == 0) return enqueue_chunks(left == 0 ? 1 : nch);
+            if (!gexec[q][phase]) {
+                const int N0 = N, ph0 = phase;
+                if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                    return fail(ED_EHIP, "ed_batch_lowest: capture begin failed");
+                const int r = enqueue_chunks(nch);
+                const hipError_t e = hipStreamEndCapture(st, &graph[q][ph0]);
+                N = N0;
+                phase = ph0;
+                if (r || e != hipSuccess ||
+                    hipGraphInstantiate(&gexec[q][ph0], graph[q][ph0], nullptr, nullptr, 0) != hipSuccess)
+                    return r ? r : fail(ED_EHIP, "ed_batch_lowest: graph capture failed");
+            }
+            if (hipGraphLaunch(gexec[q][phase], st) != hipSuccess)
+                return fail(ED_EHIP, "ed_batch_lowest: graph launch failed");
+            N += nch * every;
+            phase = (phase + nch * every) % 3;
+            return ED_OK;
+        };
+        // pinned slots pact[nc], pact[nc + 1] (free after the uploads) take the running counts
+        auto post_count = [&](int slot) -> int {
+            ED_HIP(hipMemcpyAsync(pact + nc + slot, dremaining, sizeof(int), hipMemcpyDeviceToHost, st));
+            ED_HIP(hipEventRecord(ev_count[slot], st));
+            return ED_OK;
+        };
+        if ((s = enqueue_chunks(1)) || (s = post_count(0))) break;
+        int pending[2] = {0, 0}, np = 1, next = 1;
+        bool stop = false;
+        while (np > 0 && !s) {
+            if (!stop && N < maxN) {
+                if ((s = submit()) || (s = post_count(next))) break;
+                pending[np++] = next;
+                next ^= 1;
+            }
+            const int slot = pending[0];
+            pending[0] = pending[1];
+            --np;
+            if (hipEventSynchronize(ev_count[slot]) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_batch_lowest: readback failed");
                 break;
             }
-            qN[0] = qN[1];
-            qslot[0] = qslot[1];
-            --nq;
-            for (int k = Nc; k < Ne; ++k)
-                for (int c = 0; c < nc; ++c) {
-                    ha[(size_t)c * ld + k] = pa[(size_t)k * nc + c];
-                    hb[(size_t)c * ld + k + 1] = pb[(size_t)(k + 1) * nc + c];
-                }
-            Nc = Ne;
-            // convergence check of every active copy (same test as ed_lanczos_eigen), the
-            // copies split over host threads (the Sturm bisection dominates: ~60 passes of M)
-            auto check_copy = [&](int c, std::vector<double> &zc) {
-                const double *a = &ha[(size_t)c * ld], *b = &hb[(size_t)c * ld];
-                const int Nl = Ne - 1;
-                int M = Nl;
-                for (int k = processed[c]; k <= Nl; ++k)
-                    if (!(b[k + 1] > 0.0) || !std::isfinite(a[k])) {
-                        M = k;
-                        break;
-                    }
-                for (int k = processed[c]; k <= M; ++k) {
-                    if (k == 0) anorm[c] = std::fabs(a[0] + b[1]);
-                    else anorm[c] = std::fmax(anorm[c], b[k] + std::fabs(a[k] + b[k + 1]));
-                }
-                processed[c] = M + 1;
-                zc.assign(M + 1, 0.0);
-                double lam;
-                ed_tridiag_lowest(M + 1, a, b + 1, &lam, zc.data());
-                ev_c[c] = lam;
-                last[c] = M;
-                bool stop = false;
-                if (M < Nl || !(b[M + 1] > 0.0)) stop = true;
-                else if (std::fabs(b[M + 1] * zc[M]) < tol * anorm[c] && std::fabs(zc[0]) > tol * anorm[c]) stop = true;
-                stopnow[c] = stop ? 1 : 0;
-            };
-            todo.clear();
-            for (int c = 0; c < nc; ++c)
-                if (!done[c]) todo.push_back(c);
-            const int nth = (int)std::min<size_t>(8, (todo.size() + 15) / 16);
-            if (nth <= 1) {
-                for (int c : todo) check_copy(c, z);
-            } else {
-                std::vector<std::thread> th;
-                for (int t = 0; t < nth; ++t)
-                    th.emplace_back([&, t] {
-                        std::vector<double> zc;
-                        for (size_t i = t; i < todo.size(); i += nth) check_copy(todo[i], zc);
-                    });
-                for (auto &x : th) x.join();
-            }
-            bool changed = false;
-            for (int c : todo)
-                if (stopnow[c]) {
-                    done[c] = conv[c] = 1;
-                    hact[c] = 0;
-                    --remaining;
-                    changed = true;
-                }
-            if (changed) { // also when all have stopped: the chunk in flight then skips them
-                // converged copies are switched off at once, on a second stream, while the
-                // chunk that runs ahead is in flight: the kernels it has not started yet skip
-                // them.  A flag only goes from 1 to 0, and a copy's kernels touch only its own
-                // vectors, partial slots and a/b rows, so a kernel that still sees the old
-                // flag does wasted but harmless work.
-                if (hipStreamSynchronize(fs) != hipSuccess) {
-                    s = fail(ED_EHIP, "ed_batch_lowest: flag stream failed");
-                    break;
-                }
-                for (int c = 0; c < nc; ++c) pact[c] = hact[c];
-                if (hipMemcpyAsync(active, pact, sizeof(int) * nc, hipMemcpyHostToDevice, fs)) {
-                    s = fail(ED_EHIP, "ed_batch_lowest: active upload failed");
-                    break;
-                }
-            }
-            if (remaining == 0) break;
+            if (pact[nc + slot] <= 0) stop = true;
         }
-        for (int k = 0; k < 2; ++k)
-            if (evr[k]) {
-                (void)hipStreamSynchronize(st);
-                (void)hipEventDestroy(evr[k]);
-            }
         if (s) break;
+        std::vector<CopyCheck> hc((size_t)nc);
+        if (hipMemcpyAsync(hc.data(), cc, sizeof(CopyCheck) * (size_t)nc, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            s = fail(ED_EHIP, "ed_batch_lowest: result readback failed");
+            break;
+        }
         for (int i = 0; i < B->n; ++i) {
             const int c = B->copy_of[i];
-            evals[i] = ev_c[c];
-            if (nsteps) nsteps[i] = last[c] + 1;
-            if (status) status[i] = conv[c] ? ED_OK : ED_ENOCONV;
+            evals[i] = hc[c].ev;
+            if (nsteps) nsteps[i] = hc[c].last + 1;
+            if (status) status[i] = hc[c].conv ? ED_OK : ED_ENOCONV;
         }
-        if (hipStreamSynchronize(st) != hipSuccess) s = fail(ED_EHIP, "ed_batch_lowest: final sync failed");
     } while (0);
     cleanup();
     return s;

@@ -342,11 +342,15 @@ int ed_hubbard_apply_host(ed_hubbard h, int complex_vectors, const void *x, void
 
 /* Many independent instances (sectors, twist fluxes, disorder realisations), each built by
  * ed_hubbard_create_ops, solved together: every Lanczos step of all active instances is one
- * launch per kernel over a shared tile list, and ten steps (the convergence-check cadence)
- * are captured in a hipGraph and replayed.  Each instance runs the recurrence and
- * convergence test of Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613) from the seed-`seed`
- * start vector of its own length; only eigenvalues are produced.  No counterpart in the
- * reference, whose drivers loop over instances (e.g. drude_v2_cal_Kagome_randomHop.cu:129-175). */
+ * launch per kernel over a shared tile list; after every ten steps (the convergence-check
+ * cadence) a device kernel runs the convergence test of every active instance (the host test's
+ * code, ed_tridiag.hpp) beside the next ten steps and switches converged instances off.  Fifty
+ * steps with their tests are captured in one hipGraph (per buffer rotation phase) and replayed;
+ * between replays the host reads only the number of instances still running.  Each instance
+ * runs the recurrence and convergence test of Lanczos<T>::eigen (HubMatrix_GPU.cuh:561-613) from
+ * the seed-`seed` start vector of its own length; only eigenvalues are produced.  No
+ * counterpart in the reference, whose drivers loop over instances (e.g.
+ * drude_v2_cal_Kagome_randomHop.cu:129-175). */
 typedef struct ed_batch_s *ed_batch;
 int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vectors, ed_batch *out);
 int ed_batch_destroy(ed_batch b);
