@@ -314,20 +314,23 @@ def batched_drude_line(with_cpu=False):
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
-def ensemble_line(n_real, ncv=24, workers=8):
+def ensemble_line(n_real, ncv=24, workers=1, method="lanczos"):
     """14-site open chain, 7/7, U = 4, W = 8: E0 of all realisations in one batched launch, then per
-    realisation the 4 lowest eigenvalues, the 7|7 entropy S_vN and dE0/d eps_i, eight realisations at
-    a time on separate streams.  ncv = 24 basis vectors for the 4 lowest (32 realisations: 16, 20,
-    24, 32, 40 gave 0.90, 1.00, 1.01, 0.93, 0.82 realisations/s; 12 does not converge for all)."""
+    realisation the 4 lowest eigenvalues and the ground state (method "lanczos": one Lanczos run
+    with selective orthogonalisation, ground state refined by a plain run; "restart": the
+    thick-restart solver with ncv = 24 basis vectors) and dE0/d eps_i, `workers` realisations at a
+    time on separate streams; then the 7|7 entropy S_vN of every ground state, each block size of
+    rho_A solved for all realisations in one batched call."""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import disorder as DS
     res = DS.run_ensemble(n_real=n_real, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=ncv, tol=1e-10,
-                          log=lambda m: log("[bench] " + m), with_sref=False, workers=workers)
-    log(f"[bench] disorder ensemble: {n_real} realisations in {res.seconds:.2f} s "
+                          log=lambda m: log("[bench] " + m), with_sref=False, workers=workers, method=method)
+    log(f"[bench] disorder ensemble ({method}, {workers} worker(s)): {n_real} realisations in {res.seconds:.2f} s "
         f"(batched E0 of all: {res.seconds_batched_e0:.2f} s)")
     return {"value": round(res.realisations_per_s, 3), "unit": "realisations/s", "realisations": n_real,
             "batched_e0_s": round(res.seconds_batched_e0, 3),
             "max_abs_e0_batched_minus_eigs": float(abs(res.e0_batched - res.evals[:, 0]).max()),
-            "ncv": ncv, "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
+            "method": method, "workers": workers, "ncv": ncv if method == "restart" else None,
+            "workload": "chain14_open_nup7_ndn7_U4_W8_nev4"}
 
 
 def krylov_line(max_steps=1000, dt=0.02):
@@ -387,7 +390,9 @@ def main():
                     help="only S_vN(9|9) of a seeded random 18-site 9/9 state (rho_A Gram + block eigenvalues)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the disorder-ensemble line (--ensemble realisations, --workers threads)")
-    ap.add_argument("--workers", type=int, default=8, help="host threads of the disorder-ensemble line")
+    ap.add_argument("--workers", type=int, default=1, help="host threads of the disorder-ensemble line")
+    ap.add_argument("--ensemble-method", default="lanczos", choices=["lanczos", "restart"],
+                    help="eigensolver of the disorder-ensemble line")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -403,7 +408,8 @@ def main():
         print(json.dumps({"batched_drude": batched_drude_line()}), flush=True)
         return
     if args.ensemble_only:
-        print(json.dumps({"disorder_ensemble": ensemble_line(args.ensemble, workers=args.workers)}), flush=True)
+        print(json.dumps({"disorder_ensemble": ensemble_line(args.ensemble, workers=args.workers,
+                                                             method=args.ensemble_method)}), flush=True)
         return
     if args.entropy_only:
         print(json.dumps({"entropy": entropy_only_line()}), flush=True)
@@ -564,7 +570,7 @@ def main():
         torch.cuda.empty_cache()
         out["batched_drude"] = batched_drude_line(with_cpu=not args.no_cpu)
         out["kagome12_ground_state"] = kagome12_line(with_cpu=not args.no_cpu)
-        out["disorder_ensemble"] = ensemble_line(args.ensemble, workers=args.workers)
+        out["disorder_ensemble"] = ensemble_line(args.ensemble, workers=args.workers, method=args.ensemble_method)
         out["krylov_quench"] = krylov_line()
     print(json.dumps(out), flush=True)
 

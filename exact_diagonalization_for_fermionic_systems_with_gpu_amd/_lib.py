@@ -90,6 +90,7 @@ SIGNATURES = {
     "ed_scal": (_i32, [_i64, _i32, _d, _p, _p]),
     "ed_normalize": (_i32, [_i64, _i32, _p, _pd, _p]),
     "ed_lanczos_eigen": (_i32, [_p, _i32, C.POINTER(LanczosCfg), _p, _pd, _p, _pd, _pd, _pi, _p]),
+    "ed_lanczos_lowest": (_i32, [_p, _i32, C.POINTER(LanczosCfg), _p, _i32, _pd, _p, _pi, _p]),
     "ed_lanczos_tridiagonal": (_i32, [_p, _i32, _i32, _p, _pd, _pd, _p]),
     "ed_tridiag_eigen": (_i32, [_i32, _pd, _pd, _pd, _pd]),
     "ed_lanczos_orthogonality": (_i32, [_i32, _pd, _pd, _pd]),
@@ -114,6 +115,7 @@ SIGNATURES = {
     "ed_continued_fraction": (_i32, [_i32, _pd, _pd, _d, _d, _i32, _pd, _pd]),
     "ed_entanglement_entropy": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _pi64, _p]),
     "ed_entanglement_entropy_vn": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _p]),
+    "ed_entanglement_entropy_vn_batch": (_i32, [_i32, _i32, _i32, _i64, _i32, _i32, _p, _pd, _p]),
     "ed_entanglement_spectrum": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _i64, _pi64, _p]),
     "ed_entanglement_stats": (_i32, [_pd, _pd, _pd, _i32]),
 }

@@ -88,8 +88,11 @@ constexpr int TT = 64, TK = 16;
 
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_gram_mfma(int64_t k, int64_t K, const double *__restrict__ A,
-                                                   double *__restrict__ rho, int ntile)
+                                                   double *__restrict__ rho, int ntile, int64_t sA, int64_t sR)
 {
+    // blockIdx.y: the state of a batch (block matrices sA, rho matrices sR doubles apart)
+    A += (int64_t)blockIdx.y * sA;
+    rho += (int64_t)blockIdx.y * sR;
     // map blockIdx -> (ti <= tj)
     int64_t b = blockIdx.x, ti = 0;
     {
@@ -502,9 +505,11 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 }
                 flops += (cplx ? 4.0 : 1.0) * (double)k * (double)(k + 1) * (double)K;
                 if (cplx)
-                    hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
+                    hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile,
+                                       (int64_t)0, (int64_t)0);
                 else
-                    hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile);
+                    hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, ws, k, K, A, rho, ntile,
+                                       (int64_t)0, (int64_t)0);
                 if (hipGetLastError() != hipSuccess || !mark()) {
                     wfail(wi, "entanglement: gram launch failed (%d)", 0);
                     return;
@@ -674,6 +679,133 @@ extern "C" int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t mas
     for (double l : lam)
         if (l > 0.0) svn -= l * std::log(l);
     *s_vn = svn;
+    return ED_OK;
+}
+
+// S_vN of many states of one sector and cut: the blocks of rho_A have the same sizes in every
+// state, so each block size is solved once for all states of a chunk (one Gram launch over the
+// chunk, one strided-batched rocSOLVER dsyevd/zheevd) instead of once per state and block.  The
+// per-state arithmetic is that of ed_entanglement_entropy_vn: the same gather, Gram kernel and
+// block order of the sum; only the eigenvalue routine is rocSOLVER's batched variant.
+extern "C" int ed_entanglement_entropy_vn_batch(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors,
+                                                int nstates, const void *const *d_psis, double *s_vn, void *stream)
+{
+    if (!s_vn || !d_psis || nstates < 1) return fail(ED_EINVAL, "ed_entanglement_entropy_vn_batch: bad arguments");
+    if (N < 2 || N > 31 || Q_up < 0 || Q_dn < 0 || Q_up > N || Q_dn > N)
+        return fail(ED_EINVAL, "ed_entanglement_entropy_vn_batch: bad sector");
+    for (int i = 0; i < nstates; ++i)
+        if (!d_psis[i]) return fail(ED_EINVAL, "ed_entanglement_entropy_vn_batch: null state %d", i);
+    const uint32_t full = (1u << N) - 1u, mA = (uint32_t)maskA & full;
+    if (mA == 0 || mA == full) return fail(ED_EINVAL, "entanglement: A must be a proper subset of the sites");
+    const bool cplx = complex_vectors != 0;
+    const int NC = cplx ? 2 : 1;
+    hipStream_t st = stream_of(stream);
+    Plan P;
+    ED_TRY(make_plan(N, Q_up, Q_dn, mA, P));
+    const int64_t dim_up = (int64_t)P.cu.size() / 3, dim_dn = (int64_t)P.cd.size() / 3;
+    std::vector<int> jobs;
+    std::vector<int64_t> woff;
+    int64_t maxk = 1, nvals = 0, rtot = 0; // rtot: doubles of every block's rho, packed
+    std::vector<int64_t> roff;
+    for (int g = 0; g < (int)P.geo.size(); ++g) {
+        const BlockGeo &G = P.geo[g];
+        if (G.rows == 0 || G.cols == 0) continue;
+        maxk = std::max(maxk, G.rows);
+        jobs.push_back(g);
+        woff.push_back(nvals);
+        roff.push_back(rtot);
+        nvals += G.rows;
+        rtot += NC * G.rows * G.rows;
+    }
+    const int njobs = (int)jobs.size();
+    rocblas_handle h = solver_handle();
+    if (!h) return fail(ED_EHIP, "entanglement: rocblas handle creation failed");
+    rocblas_set_stream(h, st);
+    // states per chunk: every block matrix and rho of a chunk resident, within half the free memory
+    size_t free_b = 0, total_b = 0;
+    ED_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double per_state = 8.0 * ((double)NC * (double)P.total + (double)rtot + 2.0 * (double)nvals + (double)maxk) +
+                             sizeof(rocblas_int) * (double)njobs;
+    const int sb = (int)std::max<double>(1.0, std::min<double>((double)nstates, 0.5 * (double)free_b / per_state));
+    static thread_local Arena t_eeb_arena;
+    Scratch S(t_eeb_arena, st);
+    const size_t bM = Scratch::round(sizeof(double) * NC * (size_t)P.total * sb),
+                 bR = Scratch::round(sizeof(double) * (size_t)rtot * sb),
+                 bW = Scratch::round(sizeof(double) * (size_t)nvals * sb),
+                 bE = Scratch::round(sizeof(double) * (size_t)maxk * sb),
+                 bU = Scratch::round(sizeof(int32_t) * P.cu.size()), bD = Scratch::round(sizeof(int32_t) * P.cd.size()),
+                 bG = Scratch::round(sizeof(BlockGeo) * P.geo.size()),
+                 bI = Scratch::round(sizeof(rocblas_int) * (size_t)njobs * sb);
+    ED_TRY(S.reserve(bM + bR + bW + bE + bU + bD + bG + bI));
+    double *dM = (double *)S.take(bM), *dR = (double *)S.take(bR), *dW = (double *)S.take(bW),
+           *dE = (double *)S.take(bE);
+    int32_t *dcu = (int32_t *)S.take(bU), *dcd = (int32_t *)S.take(bD);
+    BlockGeo *dgeo = (BlockGeo *)S.take(bG);
+    rocblas_int *dinfo = (rocblas_int *)S.take(bI);
+    if (hipMemcpyAsync(dcu, P.cu.data(), sizeof(int32_t) * P.cu.size(), hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(dcd, P.cd.data(), sizeof(int32_t) * P.cd.size(), hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(dgeo, P.geo.data(), sizeof(BlockGeo) * P.geo.size(), hipMemcpyHostToDevice, st))
+        return fail(ED_EHIP, "entanglement: upload failed");
+    const int64_t D = dim_up * dim_dn;
+    const unsigned grid = (unsigned)((D + 255) / 256);
+    std::vector<double> wv((size_t)nvals * sb);
+    std::vector<rocblas_int> info((size_t)njobs * sb);
+    for (int c0 = 0; c0 < nstates; c0 += sb) {
+        const int nc = std::min(sb, nstates - c0);
+        for (int i = 0; i < nc; ++i) {
+            double *Mi = dM + (size_t)NC * (size_t)P.total * i;
+            if (cplx)
+                hipLaunchKernelGGL(k_gather_blocks<true>, dim3(grid), dim3(256), 0, st, dim_up, dim_dn, dcu, dcd,
+                                   P.nbd, dgeo, (const double *)d_psis[c0 + i], Mi);
+            else
+                hipLaunchKernelGGL(k_gather_blocks<false>, dim3(grid), dim3(256), 0, st, dim_up, dim_dn, dcu, dcd,
+                                   P.nbd, dgeo, (const double *)d_psis[c0 + i], Mi);
+        }
+        ED_CHECK_LAUNCH();
+        for (int b = 0; b < njobs; ++b) {
+            const BlockGeo &G = P.geo[jobs[b]];
+            const int64_t k = G.rows, K = G.cols;
+            const int ntile = (int)((k + TT - 1) / TT);
+            const int64_t nblk = (int64_t)ntile * (ntile + 1) / 2;
+            double *rho = dR + (size_t)roff[b] * sb; // nc matrices of NC k^2 doubles
+            const int64_t sA = NC * P.total, sR = NC * k * k;
+            if (cplx)
+                hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk, (unsigned)nc), dim3(256), 0, st, k, K,
+                                   (const double *)(dM + NC * G.off), rho, ntile, sA, sR);
+            else
+                hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk, (unsigned)nc), dim3(256), 0, st, k, K,
+                                   (const double *)(dM + NC * G.off), rho, ntile, sA, sR);
+            ED_CHECK_LAUNCH();
+            // eigenvalues of state i's block into dW[i * nvals + woff[b] ...]
+            const rocblas_status rs =
+                cplx ? rocsolver_zheevd_strided_batched(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k,
+                                                        (rocblas_double_complex *)rho, (rocblas_int)k, k * k,
+                                                        dW + woff[b], nvals, dE, maxk, dinfo + (size_t)b * sb,
+                                                        (rocblas_int)nc)
+                     : rocsolver_dsyevd_strided_batched(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k, rho,
+                                                        (rocblas_int)k, k * k, dW + woff[b], nvals, dE, maxk,
+                                                        dinfo + (size_t)b * sb, (rocblas_int)nc);
+            if (rs != rocblas_status_success)
+                return fail(ED_EHIP, "entanglement: rocsolver batched syevd failed (%d)", (int)rs);
+        }
+        ED_HIP(hipMemcpyAsync(wv.data(), dW, sizeof(double) * (size_t)nvals * nc, hipMemcpyDeviceToHost, st));
+        ED_HIP(hipMemcpyAsync(info.data(), dinfo, sizeof(rocblas_int) * (size_t)njobs * sb, hipMemcpyDeviceToHost, st));
+        ED_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < njobs; ++b)
+            for (int i = 0; i < nc; ++i)
+                if (info[(size_t)b * sb + i] != 0)
+                    return fail(ED_EHIP, "entanglement: batched syevd info %d (state %d)", (int)info[(size_t)b * sb + i],
+                                c0 + i);
+        for (int i = 0; i < nc; ++i) {
+            // block order, as the blocks are listed in the plan (ed_entanglement_entropy_vn's sum)
+            double svn = 0.0;
+            for (int64_t j = 0; j < nvals; ++j) {
+                const double l = wv[(size_t)i * nvals + j];
+                if (l > 0.0) svn -= l * std::log(l);
+            }
+            s_vn[c0 + i] = svn;
+        }
+    }
     return ED_OK;
 }
 

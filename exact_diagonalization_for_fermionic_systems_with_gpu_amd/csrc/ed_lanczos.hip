@@ -22,6 +22,7 @@
 
 #include "ed_common.hpp"
 #include "ed_hubbard.hpp"
+#include "ed_thread.hpp"
 #include "ed_tridiag.hpp"
 
 namespace ed {
@@ -248,28 +249,62 @@ __global__ __launch_bounds__(256) void k_times_i(int64_t ncplx, const double *__
     iy[2 * i + 1] = y[2 * i];
 }
 
+// The basis chunks and good Ritz vectors come from a grow-only pool of the calling thread, reused by
+// its next solve (a solve with a few hundred vectors would otherwise map and unmap gigabytes per
+// call); released when the thread exits (not during process teardown, ed_thread.hpp).
+struct SoPool {
+    std::vector<double *> chunk; // basis chunks
+    std::vector<size_t> chunk_cap;
+    std::vector<double *> yv;    // Ritz vectors
+    std::vector<size_t> yv_cap;
+    ~SoPool()
+    {
+        if (process_exiting()) return;
+        for (double *c : chunk) dfree(c);
+        for (double *y : yv) dfree(y);
+    }
+    // the i-th buffer of list v (capacities cap) with room for `doubles` doubles
+    static int get(std::vector<double *> &v, std::vector<size_t> &cap, size_t i, size_t doubles, double **out)
+    {
+        if (v.size() <= i) {
+            v.resize(i + 1, nullptr);
+            cap.resize(i + 1, 0);
+        }
+        if (cap[i] < doubles) {
+            dfree(v[i]);
+            v[i] = nullptr;
+            cap[i] = 0;
+            ED_TRY(dmalloc((void **)&v[i], sizeof(double) * doubles));
+            cap[i] = doubles;
+        }
+        *out = v[i];
+        return ED_OK;
+    }
+};
+static thread_local SoPool t_so_pool;
+
 struct SelOrth {
     static constexpr int kChunk = 16;
     int64_t n = 0; // doubles per vector
     bool cvec = false;
-    std::vector<double *> chunks; // v_0, v_1, ... kChunk per allocation
+    std::vector<double *> chunks; // v_0, v_1, ... kChunk per allocation (t_so_pool's)
     std::vector<double *> Y;      // unit good Ritz vectors (with i y after each, complex vectors)
     std::vector<double> theta;    // their Ritz values
     double *coef = nullptr;       // device: purge coefficients, then Ritz coordinates
     int coef_cap = 0;
+    // Ritz values above theta_cap are not kept (the nev-lowest solve sets it to its nev-th lowest
+    // Ritz value: copies of converged eigenvalues further up stay up there and do not disturb the
+    // lowest ones, so the basis need only stay orthogonal to the converged vectors at the low end)
+    double theta_cap = std::numeric_limits<double>::infinity();
     int analyses = 0, purges = 0;
-    ~SelOrth()
-    {
-        for (double *c : chunks) dfree(c);
-        for (double *y : Y) dfree(y);
-        dfree(coef);
-    }
+    ~SelOrth() { dfree(coef); }
     double *vec(int k) { return chunks[(size_t)k / kChunk] + (size_t)(k % kChunk) * (size_t)n; }
+    int new_y(double **y) { return SoPool::get(t_so_pool.yv, t_so_pool.yv_cap, Y.size(), (size_t)n, y); }
     int store(int k, const double *v, hipStream_t st)
     {
         while ((int)chunks.size() * kChunk <= k) {
             double *c = nullptr;
-            if (dmalloc((void **)&c, sizeof(double) * (size_t)n * kChunk) != ED_OK)
+            if (SoPool::get(t_so_pool.chunk, t_so_pool.chunk_cap, chunks.size(), (size_t)n * kChunk, &c) != ED_OK)
                 return fail(ED_ENOMEM, "ed_lanczos_eigen: selective reorthogonalisation keeps the Lanczos basis on "
                                        "the device; %d vectors of %lld doubles do not fit",
                             (int)(chunks.size() + 1) * kChunk, (long long)n);
@@ -333,20 +368,21 @@ struct SelOrth {
         std::sort(d.begin(), d.end());
         for (int i = 0; i <= M; ++i) {
             const double th = d[i];
+            if (th > theta_cap) break; // ascending
             bool known = false;
             for (double t : theta) known = known || std::fabs(t - th) <= std::sqrt(eps) * tn;
             if (known) continue;
             inverse_iteration(M + 1, a, b + 1, th, z.data());
             if (!(std::fabs(b[M + 1] * z[M]) <= std::sqrt(eps) * tn)) continue;
             double *y = nullptr;
-            ED_TRY(dmalloc((void **)&y, sizeof(double) * (size_t)n));
+            ED_TRY(new_y(&y));
             Y.push_back(y);
             theta.push_back(th);
             ED_TRY(combine(M, z.data(), y, st));
             ED_TRY(ed_normalize(n, 0, y, nullptr, st));
             if (cvec) {
                 double *iy = nullptr;
-                ED_TRY(dmalloc((void **)&iy, sizeof(double) * (size_t)n));
+                ED_TRY(new_y(&iy));
                 Y.push_back(iy);
                 theta.push_back(th);
                 hipLaunchKernelGGL(k_times_i, dim3((unsigned)((n / 2 + 255) / 256)), dim3(256), 0, st, n / 2,
@@ -425,18 +461,20 @@ extern "C" int ed_tridiag_eigen(int n, const double *d, const double *e, double 
     return ED_OK;
 }
 
-extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f,
-                                double *eval, void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream)
+// The driver of ed_lanczos_eigen (nev = 1) and ed_lanczos_lowest (nev >= 1, selective
+// orthogonalisation): eval[0..nev-1] get the nev lowest Ritz values of the last test, the Ritz
+// vector is the lowest one's.
+static int lanczos_eigen_impl(const Hubbard *h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, int nev,
+                              double *eval, void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream)
 {
-    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
-    if (!h || !d_f || !eval || d_f == d_evec) return fail(ED_EINVAL, "ed_lanczos_eigen: bad arguments");
+    if (!h || !d_f || !eval || d_f == d_evec || nev < 1) return fail(ED_EINVAL, "ed_lanczos_eigen: bad arguments");
     const bool want_vec = d_evec != nullptr;
     const bool cvec = complex_vectors != 0;
     if (h->cplx_tables && !cvec) return fail(ED_EINVAL, "ed_lanczos_eigen: complex tables need complex vectors");
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
     const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
     const int every = cfg && cfg->check_every > 0 ? cfg->check_every : 10;
-    const int rmode = cfg ? cfg->reorth : 0;
+    const int rmode = nev > 1 ? 2 : (cfg ? cfg->reorth : 0);
     if (rmode < 0 || rmode > 2) return fail(ED_EINVAL, "ed_lanczos_eigen: reorth must be 0, 1 or 2");
     const bool reorth = rmode == 1, selective = rmode == 2;
     hipStream_t st = stream_of(stream);
@@ -531,6 +569,13 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
             else
                 for (int k = 0; k <= N; ++k) a[k] = araw[k];
             if (selective && b[N + 1] > 0.0 && std::isfinite(a[N])) {
+                if (nev > 1 && N + 1 >= nev) {
+                    // keep only good Ritz vectors up to the nev-th lowest Ritz value (with a margin)
+                    double tn = 0.0;
+                    for (int k = 0; k <= N; ++k) tn = std::fmax(tn, std::fabs(a[k]) + b[k + 1] + (k ? b[k] : 0.0));
+                    so.theta_cap = bisect_kth(N + 1, a.data(), b.data() + 1, nev - 1) +
+                                   std::sqrt(std::numeric_limits<double>::epsilon()) * tn;
+                }
                 int found = 0;
                 ED_TRY(so.analyse(N, a.data(), b.data(), st, &found));
                 if (found) {
@@ -562,10 +607,24 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
             ed_tridiag_lowest(M + 1, a.data(), b.data() + 1, &lam, z.data());
             *eval = lam;
             lastN = M;
+            // nev > 1: the test of every one of the nev lowest Ritz pairs of T_{M+1}; with the
+            // Lanczos vectors kept semi-orthogonal (selective orthogonalisation) T has no spurious
+            // copies of converged eigenvalues, so its k-th lowest Ritz value stands for the k-th
+            // eigenvalue of H
+            bool rest = true;
+            const int nk = std::min(nev, M + 1);
+            for (int k = 1; k < nk; ++k) {
+                eval[k] = bisect_kth(M + 1, a.data(), b.data() + 1, k);
+                std::vector<double> zk((size_t)M + 1);
+                inverse_iteration(M + 1, a.data(), b.data() + 1, eval[k], zk.data());
+                rest = rest && std::fabs(b[M + 1] * zk[M]) < tol * anorm;
+            }
+            for (int k = nk; k < nev; ++k) eval[k] = std::numeric_limits<double>::quiet_NaN();
             if (M < N || !(b[M + 1] > 0.0)) {
                 status = ED_OK; // exact invariant subspace: T_{M+1} eigenvalues are exact
                 stop = true;
-            } else if (std::fabs(b[M + 1] * z[M]) < tol * anorm && std::fabs(z[0]) > tol * anorm) {
+            } else if (nk == nev && rest && std::fabs(b[M + 1] * z[M]) < tol * anorm &&
+                       std::fabs(z[0]) > tol * anorm) {
                 status = ED_OK;
                 stop = true;
             }
@@ -642,6 +701,39 @@ extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lan
     if (nsteps) *nsteps = Nk + 1;
     if (status != ED_OK) return fail(status, "ed_lanczos_eigen: convergence test not met in %d steps", maxN);
     return ED_OK;
+}
+
+extern "C" int ed_lanczos_eigen(ed_hubbard hh, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f,
+                                double *eval, void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream)
+{
+    return lanczos_eigen_impl(reinterpret_cast<const Hubbard *>(hh), complex_vectors, cfg, d_f, 1, eval, d_evec,
+                              a_out, b_out, nsteps, stream);
+}
+
+extern "C" int ed_lanczos_lowest(ed_hubbard hh, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, int nev,
+                                 double *evals, void *d_evec, int *nsteps, void *stream)
+{
+    if (nev < 1 || nev > 64) return fail(ED_EINVAL, "ed_lanczos_lowest: nev must be 1..64");
+    const Hubbard *h = reinterpret_cast<const Hubbard *>(hh);
+    int n1 = 0;
+    const int s = lanczos_eigen_impl(h, complex_vectors, cfg, d_f, nev, evals, d_evec, nullptr, nullptr, &n1, stream);
+    if (nsteps) *nsteps = n1;
+    if (s != ED_OK || !d_evec) return s;
+    // The purges keep the basis semi-orthogonal, which leaves the eigenvalues accurate to working
+    // precision but the Ritz vector V s_0 off by about sqrt(eps) along the purged directions.  A
+    // plain Lanczos run (no reorthogonalisation) started from that vector, with the same test and
+    // its Ritz vector rebuilt by recompute, removes that: its residual bounds the error.
+    ed_lanczos_cfg c2{};
+    if (cfg) c2 = *cfg;
+    c2.reorth = 0;
+    const bool cvec = complex_vectors != 0;
+    const int64_t nreal = h->args.dim_up * h->args.dim_dn * (cvec ? 2 : 1);
+    ED_HIP(hipMemcpyAsync(d_f, d_evec, sizeof(double) * (size_t)nreal, hipMemcpyDeviceToDevice, stream_of(stream)));
+    double e0 = 0.0;
+    int n2 = 0;
+    const int s2 = lanczos_eigen_impl(h, complex_vectors, &c2, d_f, 1, &e0, d_evec, nullptr, nullptr, &n2, stream);
+    if (nsteps) *nsteps = n1 + n2;
+    return s2;
 }
 
 extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int nsteps, const void *d_f,

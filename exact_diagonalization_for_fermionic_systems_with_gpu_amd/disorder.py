@@ -5,10 +5,13 @@ potential Make_N(i, eps_i) per species).
 For each seeded realisation eps_i ~ Uniform[-W, W] (the same potential for both species;
 numpy PCG64 seeded with seed0 + r) this module computes
   * E0 of every realisation in one batched Lanczos launch (ed_batch_lowest),
-  * the nev lowest eigenvalues and Ritz vectors (ed_eigs_lowest, the cuSpectra solver role),
+  * the nev lowest eigenvalues and the ground state (the cuSpectra solver role): by default one
+    Lanczos run per realisation with selective orthogonalisation (ed_lanczos_lowest, method
+    "lanczos"), or the thick-restart solver (ed_eigs_lowest, method "restart"),
   * the half-chain entanglement entropy S_vN of the ground state from the particle-number blocks
-    of rho_A, and optionally the reference's S_ref = -sum sigma ln sigma (per-block SVDs,
-    ed_entanglement_entropy; S_vN alone: ed_entanglement_entropy_vn),
+    of rho_A -- for all realisations at once, each block size solved for every state in one
+    batched call (ed_entanglement_entropy_vn_batch) -- and optionally the reference's
+    S_ref = -sum sigma ln sigma (per-block SVDs, ed_entanglement_entropy, per realisation),
   * the Hellmann-Feynman gradient dE0/d eps_i = <n_{i up}> + <n_{i dn}> (ed_site_density).
 """
 from __future__ import annotations
@@ -23,7 +26,8 @@ import torch
 from ._lib import call, dptr, stream_ptr
 from .batched import BatchedLanczos
 from .fermi_operator import H_TBq
-from .hubmatrix import Hubbard
+from .hubmatrix import Hubbard, Lanczos
+from .entangle import entropy_vn_batch
 from .spectra import HubbardOp, SymEigsSolver
 from . import lattices as L
 
@@ -86,9 +90,16 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
-                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1) -> EnsembleResult:
+                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
+                 max_steps=2000) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched E0 launch, each worker
-    thread on its own stream."""
+    thread on its own stream.  method "lanczos": ed_lanczos_lowest (at most max_steps steps; tol
+    is its test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart":
+    the thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).
+    Without S_ref the entropies are taken after the solves, all realisations in one batched
+    call."""
+    if method not in ("lanczos", "restart"):
+        raise ValueError("run_ensemble: method is 'lanczos' or 'restart'")
     eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -101,21 +112,35 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
     S = np.zeros((n_real, 2))
     g = np.zeros((n_real, L_sites))
 
+    psis = [None] * n_real  # ground states kept for the batched entropy (without S_ref)
+
     def solve(r, stream):
         # the library releases the GIL in every call; each worker thread drives its own stream
         with torch.cuda.stream(stream):
             H = hs[r]
-            es = SymEigsSolver(HubbardOp(H), nev, ncv)
-            es.init()
-            es.compute("SmallestAlge", maxit, tol)
-            if es.info() != "Successful":
-                raise RuntimeError(f"realisation {r}: eigensolver did not converge")
-            evals[r] = es.eigenvalues()
-            psi = es.eigenvectors()[0]
-            S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, with_sref)
+            if method == "restart":
+                es = SymEigsSolver(HubbardOp(H), nev, ncv)
+                es.init()
+                es.compute("SmallestAlge", maxit, tol)
+                if es.info() != "Successful":
+                    raise RuntimeError(f"realisation {r}: eigensolver did not converge")
+                evals[r] = es.eigenvalues()
+                psi = es.eigenvectors()[0]
+                del es
+            else:
+                f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+                psi = torch.empty_like(f)
+                lz = Lanczos(H.dimUP, H.dimDN, max_steps=max_steps, tol=tol)
+                lz.random_vector(f)
+                evals[r] = lz.lowest(H, f, nev, psi)
+                del f
+            if with_sref:
+                S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, True)
+            else:
+                psis[r] = psi
             g[r] = site_density(H, psi).sum(0)
             stream.synchronize()
-            del es, psi
+            del psi
             if log is not None:
                 log(f"realisation {r}: E0 {evals[r, 0]:.12f}, {time.perf_counter() - t0:.2f} s elapsed")
 
@@ -139,5 +164,9 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
         with ThreadPoolExecutor(workers) as ex:
             for f in [ex.submit(task, r) for r in range(1, n_real)]:
                 f.result()
+    if not with_sref:
+        S[:, 0] = entropy_vn_batch(L_sites, n_up, n_dn, (1 << (L_sites // 2)) - 1, psis)
+        S[:, 1] = np.nan
+        del psis[:]
     torch.cuda.synchronize()
     return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb)

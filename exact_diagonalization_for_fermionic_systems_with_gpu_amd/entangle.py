@@ -51,3 +51,23 @@ class EEmatHubAB_GPU:
         if n.value > cap:
             raise RuntimeError("entanglement spectrum longer than its bound")
         return lam[: n.value]
+
+
+def entropy_vn_batch(N: int, Q_up: int, Q_dn: int, maskA: int, states) -> np.ndarray:
+    """S_vN of every state in `states` (tensors of one sector, all real or all complex) for the cut
+    maskA, each block size of rho_A solved for all states at once
+    (ed_entanglement_entropy_vn_batch)."""
+    states = list(states)
+    if not states:
+        return np.zeros(0)
+    cplx = states[0].is_complex()
+    for s in states:
+        if s.is_complex() != cplx or not s.is_contiguous() or not s.is_cuda:
+            raise ValueError("entropy_vn_batch: states must be contiguous CUDA tensors of one dtype")
+        if s.dtype not in (torch.float64, torch.complex128) or s.numel() != states[0].numel():
+            raise ValueError("entropy_vn_batch: float64/complex128 states of one length")
+    ptrs = (C.c_void_p * len(states))(*[s.data_ptr() for s in states])
+    out = np.zeros(len(states))
+    call("ed_entanglement_entropy_vn_batch", N, Q_up, Q_dn, maskA, 1 if cplx else 0, len(states), ptrs,
+         out.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
+    return out

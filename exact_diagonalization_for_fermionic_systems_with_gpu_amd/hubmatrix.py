@@ -328,6 +328,34 @@ class Lanczos(Thrust_Lin):
             raise EdError(st, f"Lanczos convergence test not met in {self.max_steps} steps (E0={ev.value})")
         return ev.value
 
+    def lowest(self, Hub: Hubbard, f: torch.Tensor, nev: int, evec: torch.Tensor | None = None) -> np.ndarray:
+        """The nev lowest eigenvalues (ascending) by one Lanczos run with selective
+        orthogonalisation (ed_lanczos_lowest: the SymEigsSolver "SmallestAlge" role,
+        TotalSpin/include/SymEigsSolver.cuh:332-359); evec (optional) gets the lowest one's Ritz
+        vector.  f is clobbered.  Raises EdError(ED_ENOCONV) if the test is not met for all nev."""
+        if evec is None:
+            Hub._check(f)
+        else:
+            Hub._check(f, evec)
+        if not 1 <= nev <= 64:
+            raise ValueError("lowest: nev must be 1..64")
+        cfg = LanczosCfg(self.max_steps, self.tol, self.check_every, 2)
+        ev = (C.c_double * nev)()
+        ns = C.c_int()
+        from ._lib import lib
+        st = lib().ed_lanczos_lowest(Hub.handle, _is_complex(f), C.byref(cfg), _raw(f), int(nev), ev, _raw(evec),
+                                     C.byref(ns), stream_ptr())
+        self.nsteps = ns.value
+        self.converged = st == 0
+        vals = np.array(ev[:])
+        self.eval = float(vals[0])
+        if st not in (0, ED_ENOCONV):
+            from ._lib import check
+            check(st)
+        if st == ED_ENOCONV:
+            raise EdError(st, f"Lanczos test of the {nev} lowest not met in {self.max_steps} steps ({vals})")
+        return vals
+
     def so_stats(self) -> dict:
         """selective-orthogonalisation statistics of this thread's last eigen() (ed_lanczos_so_stats)"""
         g, an, pu, om = C.c_int(), C.c_int(), C.c_int(), C.c_double()

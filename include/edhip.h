@@ -277,6 +277,22 @@ typedef struct {
 int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, double *eval,
                      void *d_evec, double *a_out, double *b_out, int *nsteps, void *stream);
 
+/* The nev lowest eigenvalues of H by one Lanczos run with selective orthogonalisation (the role
+ * of cuSpectra::SymEigsSolver's "SmallestAlge" solve, TotalSpin/include/SymEigsSolver.cuh:332-359,
+ * which restarts a 2 nev + 1 vector Krylov basis; here one recurrence runs on, its vectors kept
+ * semi-orthogonal against the converged Ritz vectors as with reorth = 2, so its tridiagonal has no
+ * spurious copies and its k-th lowest Ritz value stands for the k-th eigenvalue).  Every
+ * check_every steps each of the nev lowest Ritz pairs (theta_k, s_k) of T must pass
+ * |b_{N+1} s_k[N]| < tol * anorm (ed_lanczos_eigen's test, applied to all nev); the lowest also
+ * needs |s_0[0]| > tol * anorm.  evals[nev] ascending; d_evec (may be NULL) gets the normalised
+ * ground state: V s_0, refined by a plain Lanczos run from it (ed_lanczos_eigen with the same
+ * test, Ritz vector rebuilt), since the purges leave V s_0 off by about sqrt(eps); *nsteps counts
+ * the steps of both runs.  cfg->reorth is ignored; nev 1..64.  The basis is kept on the device:
+ * ED_ENOMEM when it does not fit.  Returns ED_ENOCONV (with the last Ritz values) when a test is
+ * not met by max_steps.  Synchronises the stream. */
+int ed_lanczos_lowest(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, int nev,
+                      double *evals, void *d_evec, int *nsteps, void *stream);
+
 /* Lanczos<T>::tridiagonal (HubMatrix_GPU.cuh:541-559): nsteps fixed steps from d_f
  * (not modified); a_out[nsteps], b_out[nsteps+1] with b_out[0] = |f|.  Real vectors run on
  * the slab layout when the operator has one, or gets one as in ed_lanczos_eigen. */
@@ -425,6 +441,15 @@ int ed_entanglement_entropy(int N, int Q_up, int Q_dn, int64_t maskA, int comple
  * cost for blocks of a few hundred rows and more) */
 int ed_entanglement_entropy_vn(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                                double *s_vn, void *stream);
+/* S_vN of nstates states of the same sector and cut (no counterpart in the reference, whose drivers
+ * take one state at a time): d_psis is a host array of nstates device pointers, s_vn (host)
+ * gets one entropy per state.  Each block size of rho_A is solved once for a chunk of states (one
+ * Gram launch, one strided-batched rocSOLVER eigenvalue call) instead of once per state; a
+ * state's entropy is ed_entanglement_entropy_vn's sum over the same block eigenvalues (rocSOLVER's
+ * batched routine may round them differently in the last bits).  Chunks hold at most half the
+ * free device memory.  Synchronises the stream. */
+int ed_entanglement_entropy_vn_batch(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, int nstates,
+                                     const void *const *d_psis, double *s_vn, void *stream);
 /* all eigenvalues of rho_A, descending, into lambda_out (capacity cap); *nlambda = total count */
 int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                              double *lambda_out, int64_t cap, int64_t *nlambda, void *stream);

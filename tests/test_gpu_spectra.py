@@ -105,9 +105,13 @@ def test_eigs_kagome12_vs_lanczos_and_arpack():
     assert np.allclose(ev, wa, rtol=1e-10, atol=0), (ev, wa)
 
 
-def test_disorder_ensemble_vs_dense_chain8():
-    print("disorder ensemble chain8 x4", flush=True)
-    res = DS.run_ensemble(n_real=4, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=4, ncv=16, tol=1e-13)
+@pytest.mark.parametrize("method", ["lanczos", "restart"])
+def test_disorder_ensemble_vs_dense_chain8(method):
+    print(f"disorder ensemble chain8 x4 ({method})", flush=True)
+    # the Lanczos test scales with |T| (about 3 |E0| here), the restart solver's with |theta|: S_ref
+    # needs the ground state to ~1e-11, so the Lanczos run gets the tighter bar
+    tol = 1e-14 if method == "lanczos" else 1e-13
+    res = DS.run_ensemble(n_real=4, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=4, ncv=16, tol=tol, method=method)
     for r in range(4):
         ops = DS.chain_disorder_ops(8, res.eps[r])
         Ho = O.Hubbard(O.Species(8, 4, ops), O.Species(8, 4, ops), 4.0)
@@ -150,7 +154,7 @@ def test_disorder_gradient_vs_finite_differences():
 def test_chain14_against_fixture():
     gold = json.load(open(GOLD))
     print("disorder chain14 7/7 x2 vs fixture", flush=True)
-    res = DS.run_ensemble(n_real=2, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=20, tol=1e-13)
+    res = DS.run_ensemble(n_real=2, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=20, tol=1e-14)
     for rec in gold["realisations"]:
         r = rec["r"]
         assert np.allclose(res.eps[r], rec["eps"], rtol=0, atol=0)
@@ -170,3 +174,89 @@ def test_disorder_ensemble_concurrent_workers_match_serial():
     assert np.allclose(a.evals, b.evals, rtol=1e-12, atol=0)
     assert np.allclose(a.entropy, b.entropy, rtol=0, atol=1e-10)
     assert np.allclose(a.grad_eps, b.grad_eps, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["kagome12_u4", "chain8_disorder", "chain10_twist"])
+def test_lanczos_lowest_vs_dense(case):
+    """ed_lanczos_lowest: the nev lowest eigenvalues from one recurrence with selective
+    orthogonalisation against the dense spectrum of the oracle's H (no spurious copies of
+    converged eigenvalues), and its ground-state vector against the dense one."""
+    print(f"lanczos lowest {case}", flush=True)
+    cplx = False
+    if case == "kagome12_u4":
+        # 12-site Kagome at 5/5 with random hopping r_i in [0.8, 1.2] (no spatial symmetry, so no
+        # exactly degenerate levels within the sector; dim 792^2 = 627k)
+        r = np.random.default_rng(5).uniform(0.8, 1.2, 12)
+        ops, N, nu, nd, U = L.hubbard_ops(L.KAGOME12, r=r), 12, 5, 5, 4.0
+    elif case == "chain8_disorder":
+        ops, N, nu, nd, U = DS.chain_disorder_ops(8, DS.realisation_eps(7, 8, 8.0)), 8, 4, 4, 4.0
+    else:
+        # periodic 8-site ring with a twist of 0.7 on the closing bond (complex128) and a random
+        # on-site potential (no translation symmetry, so no degenerate levels within the sector)
+        eps = np.random.default_rng(9).uniform(-1.0, 1.0, 8)
+        ops = [(L.OP_HOP, i, i + 1, -1.0) for i in range(7)] + [(L.OP_HOP_PHASE, 7, 0, -1.0, 0.7)]
+        ops += [(L.OP_DENSITY, i, i, float(eps[i])) for i in range(8)]
+        N, nu, nd, U = 8, 3, 3, 2.0
+        cplx = True
+    hu = H_TBq(N, nu, complex_tables=cplx)
+    hu.add_ops(ops)
+    hd = H_TBq(N, nd, complex_tables=cplx)
+    hd.add_ops(ops)
+    Hg = Hubbard(hu, hd, U)
+    Ho = O.Hubbard(O.Species(N, nu, ops, complex_tables=cplx), O.Species(N, nd, ops, complex_tables=cplx), U)
+    nev = 4
+    dt = torch.complex128 if cplx else torch.float64
+    f = torch.empty(Hg.dim, dtype=dt, device="cuda")
+    psi = torch.empty_like(f)
+    lz = Lanczos(Hg.dimUP, Hg.dimDN, max_steps=3000, tol=1e-12)
+    lz.random_vector(f)
+    ev = lz.lowest(Hg, f, nev, psi)
+    if Ho.dim <= 6000:
+        w, V = np.linalg.eigh(_dense(Ho))
+        ref = w[:nev]
+        v0 = V[:, 0]
+    else:
+        # too large for a dense reference: the thick-restart solver (itself checked against ARPACK
+        # over the oracle's H.v in test_eigs_kagome12_vs_lanczos_and_arpack) at a tight tolerance
+        es = SymEigsSolver(HubbardOp(Hg), nev, 24)
+        es.init()
+        es.compute("SmallestAlge", 1000, 1e-13)
+        assert es.info() == "Successful"
+        ref = es.eigenvalues()
+        v0 = es.eigenvectors()[0].cpu().numpy()
+    assert np.allclose(ev, ref, rtol=1e-10, atol=0), (ev, ref)
+    ov = abs(np.vdot(v0, psi.cpu().numpy()))
+    assert abs(ov - 1.0) < 1e-8, ov
+    assert lz.nsteps < 3000
+
+
+def test_entropy_vn_batch_matches_per_state():
+    """S_vN of many states at once (one batched eigenvalue call per block size) against the
+    per-state entry point, real and complex, on the 7|7 cut of the 14-site chain (64 blocks up to
+    1225 rows) and a 5|5 cut of a 10-site sector."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import entangle as EN
+    print("entropy vn batch", flush=True)
+    for (N, nu, nd, cplx, ns) in [(14, 7, 7, False, 5), (10, 5, 4, True, 3)]:
+        dim = math.comb(N, nu) * math.comb(N, nd)
+        g = torch.Generator(device="cpu").manual_seed(11 + N)
+        states = []
+        for _ in range(ns):
+            if cplx:
+                x = torch.randn(dim, 2, generator=g, dtype=torch.float64)
+                x = torch.view_as_complex(x).contiguous()
+            else:
+                x = torch.randn(dim, generator=g, dtype=torch.float64)
+            x = x / torch.linalg.vector_norm(x)
+            states.append(x.cuda())
+        mask = (1 << (N // 2)) - 1
+        sb = EN.entropy_vn_batch(N, nu, nd, mask, states)
+        for i, x in enumerate(states):
+            out = np.zeros(1)
+            import ctypes as C
+            from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
+            call("ed_entanglement_entropy_vn", N, nu, nd, mask, int(cplx), dptr(x),
+                 out.ctypes.data_as(C.POINTER(C.c_double)), stream_ptr())
+            assert abs(sb[i] - out[0]) < 1e-12 * max(1.0, abs(out[0])), (N, i, sb[i], out[0])
+        # the oracle on one state
+        lam, _ = EE.block_spectra(N, nu, nd, mask, states[0].cpu().numpy())
+        assert abs(sb[0] - EE.entropy_from_lambda(lam)) < 1e-10
