@@ -118,6 +118,7 @@ SIGNATURES = {
     "ed_entanglement_entropy_vn_batch": (_i32, [_i32, _i32, _i32, _i64, _i32, _i32, _p, _pd, _p]),
     "ed_entanglement_spectrum": (_i32, [_i32, _i32, _i32, _i64, _i32, _p, _pd, _i64, _pi64, _p]),
     "ed_entanglement_stats": (_i32, [_pd, _pd, _pd, _i32]),
+    "ed_sym_eigvals": (_i32, [_i64, _p, _i64, _i32, _p, _p]),
 }
 
 _LIB = None

@@ -26,6 +26,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -35,6 +36,18 @@
 #include "ed_thread.hpp"
 
 namespace ed {
+
+// real rho_A blocks of this many rows and more take the two-stage eigenvalue routine
+// (ed_symeig.hip) instead of rocSOLVER's one-stage dsyevd
+constexpr int64_t kTwoStageMin = 1024;
+int sym_eigvals_two_stage(rocblas_handle h, hipStream_t st, int64_t n, double *A, int64_t lda, bool upper_only,
+                          double *w);
+struct SymGroup { // ed_symeig.hip
+    int64_t n, lda;
+    int count;
+    double *A, *w;
+};
+int sym_eigvals_groups(rocblas_handle h, hipStream_t st, const std::vector<SymGroup> &groups, bool upper_only);
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -462,10 +475,104 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
             s = fail(ED_EHIP, "entanglement: event record failed");
             break;
         }
+        // Real blocks of kTwoStageMin rows and more (S_vN only): every one's rho resident at once,
+        // the blocks of one size reduced together (strided-batched panels and updates) and the
+        // bulge chasing and bisection of all of them in shared launches (ed_symeig.hip), on st.
+        // The other blocks go to the workers below.
+        if (!cplx && !want_sigma) {
+            std::map<int64_t, std::vector<int>> big; // rows -> jobs
+            double rho_doubles = 0.0, w_doubles = 0.0;
+            for (int b = 0; b < njobs; ++b) {
+                const int64_t k = P.geo[jobs[b]].rows;
+                if (k < kTwoStageMin) continue;
+                big[k].push_back(b);
+                rho_doubles += (double)k * (double)k;
+                w_doubles += (double)k;
+            }
+            size_t free_b = 0, total_b = 0;
+            if (!big.empty() && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                8.0 * (rho_doubles + w_doubles) < 0.7 * (double)free_b) {
+                static thread_local Arena t_eeg_arena;
+                Scratch SG(t_eeg_arena, st);
+                if ((s = SG.reserve(Scratch::round(sizeof(double) * (size_t)rho_doubles) +
+                                    Scratch::round(sizeof(double) * (size_t)w_doubles))))
+                    break;
+                double *rho_all = (double *)SG.take(sizeof(double) * (size_t)rho_doubles);
+                double *w_all = (double *)SG.take(sizeof(double) * (size_t)w_doubles);
+                std::vector<SymGroup> groups;
+                std::vector<std::pair<int, double *>> wslot; // job -> its eigenvalues in w_all
+                hipEvent_t te[3] = {nullptr, nullptr, nullptr};
+                for (hipEvent_t &e : te)
+                    if (hipEventCreate(&e) != hipSuccess) s = fail(ED_EHIP, "entanglement: event creation failed");
+                struct TeGuard {
+                    hipEvent_t *e;
+                    ~TeGuard()
+                    {
+                        for (int i = 0; i < 3; ++i)
+                            if (e[i]) (void)hipEventDestroy(e[i]);
+                    }
+                } te_guard{te};
+                if (s) break;
+                double flops = 0.0;
+                (void)hipEventRecord(te[0], st);
+                double *rp = rho_all, *wp = w_all;
+                for (auto &kv : big) {
+                    const int64_t k = kv.first;
+                    SymGroup g{k, k, (int)kv.second.size(), rp, wp};
+                    for (int b : kv.second) {
+                        const BlockGeo &G = P.geo[jobs[b]];
+                        const int ntile = (int)((k + TT - 1) / TT);
+                        const int64_t nblk = (int64_t)ntile * (ntile + 1) / 2;
+                        hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(256), 0, st, k, G.cols,
+                                           (const double *)(dM + G.off), rp, ntile, (int64_t)0, (int64_t)0);
+                        flops += (double)k * (double)(k + 1) * (double)G.cols;
+                        wslot.push_back({b, wp});
+                        rp += k * k;
+                        wp += k;
+                    }
+                    groups.push_back(g);
+                }
+                if (hipGetLastError() != hipSuccess) {
+                    s = fail(ED_EHIP, "entanglement: gram launch failed");
+                    break;
+                }
+                (void)hipEventRecord(te[1], st);
+                if ((s = sym_eigvals_groups(h, st, groups, true))) break;
+                (void)hipEventRecord(te[2], st);
+                for (auto &bw : wslot) {
+                    const int64_t k = P.geo[jobs[bw.first]].rows;
+                    if (hipMemcpyAsync(dWall + woff[bw.first], bw.second, sizeof(double) * (size_t)k,
+                                       hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                        hipMemsetAsync(dinfo + bw.first, 0, sizeof(rocblas_int), st) != hipSuccess) {
+                        s = fail(ED_EHIP, "entanglement: eigenvalue copy failed");
+                        break;
+                    }
+                }
+                if (s) break;
+                if (hipEventSynchronize(te[2]) != hipSuccess) {
+                    s = fail(ED_EHIP, "entanglement: two-stage eigenvalues failed");
+                    break;
+                }
+                float gms = 0.0f, ems = 0.0f;
+                (void)hipEventElapsedTime(&gms, te[0], te[1]);
+                (void)hipEventElapsedTime(&ems, te[1], te[2]);
+                {
+                    std::lock_guard<std::mutex> lk(g_stats_mu);
+                    g_stats.gram_ms += gms;
+                    g_stats.gram_flops += flops;
+                    g_stats.eig_ms += ems;
+                }
+                std::vector<int> rest;
+                for (int b : order)
+                    if (P.geo[jobs[b]].rows < kTwoStageMin) rest.push_back(b);
+                order.swap(rest);
+            }
+        }
         // the blocks go to the workers largest first; each worker runs Gram, syevd (and gesvd)
         // on its own stream, rho buffer and rocSOLVER handle, and leaves the values in its
         // block's slot of dWall / dSall
         std::atomic<int> next(0);
+        const int nwork = (int)order.size();
         auto worker = [&](int wi, hipStream_t ws, rocblas_handle hw) {
             if (nw > 1 && hipSetDevice(dev) != hipSuccess) {
                 wfail(wi, "entanglement: worker device selection failed (%d)", 0);
@@ -492,7 +599,7 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                 tev.push_back(e);
                 return hipEventRecord(e, ws) == hipSuccess;
             };
-            for (int q = next.fetch_add(1); q < njobs; q = next.fetch_add(1)) {
+            for (int q = next.fetch_add(1); q < nwork; q = next.fetch_add(1)) {
                 const int b = order[q];
                 const BlockGeo &G = P.geo[jobs[b]];
                 const int64_t k = G.rows, K = G.cols;
@@ -514,12 +621,19 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
                     wfail(wi, "entanglement: gram launch failed (%d)", 0);
                     return;
                 }
-                rocblas_status rs;
+                rocblas_status rs = rocblas_status_success;
                 if (cplx)
                     rs = rocsolver_zheevd(hw, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k,
                                           (rocblas_double_complex *)rho, (rocblas_int)k, dWall + woff[b], E,
                                           dinfo + b);
-                else
+                else if (k >= kTwoStageMin) {
+                    // large real blocks: two-stage reduction (ed_symeig.hip); its info slot stays 0
+                    if (hipMemsetAsync(dinfo + b, 0, sizeof(rocblas_int), ws) != hipSuccess ||
+                        sym_eigvals_two_stage(hw, ws, k, rho, k, true, dWall + woff[b]) != ED_OK) {
+                        wfail(wi, "entanglement: two-stage eigenvalues failed (%d)", 0);
+                        return;
+                    }
+                } else
                     rs = rocsolver_dsyevd(hw, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)k, rho,
                                           (rocblas_int)k, dWall + woff[b], E, dinfo + b);
                 if (rs != rocblas_status_success) {

@@ -453,6 +453,14 @@ int ed_entanglement_entropy_vn_batch(int N, int Q_up, int Q_dn, int64_t maskA, i
 /* all eigenvalues of rho_A, descending, into lambda_out (capacity cap); *nlambda = total count */
 int ed_entanglement_spectrum(int N, int Q_up, int Q_dn, int64_t maskA, int complex_vectors, const void *d_psi,
                              double *lambda_out, int64_t cap, int64_t *nlambda, void *stream);
+/* All eigenvalues (ascending, into d_w[n]) of a real symmetric n x n matrix d_A (column-major, ld
+ * lda; the lower triangle is read, or with upper != 0 the upper one, and the matrix is
+ * overwritten) by a two-stage reduction: dense -> band (blocked Householder panels, rocSOLVER
+ * geqrf/larft, BLAS-3 two-sided updates), band -> tridiagonal by bulge chasing, then Sturm
+ * bisection of every eigenvalue.  The routine the entanglement calls use for rho_A blocks of 1024
+ * rows and more in place of a one-stage dsyevd (the SVD role of entangle_entropy.h:231-353); no
+ * counterpart in the reference.  Synchronises the stream. */
+int ed_sym_eigvals(int64_t n, double *d_A, int64_t lda, int upper, double *d_w, void *stream);
 /* Kernel-time statistics of the three calls above, summed over calls since the last reset (no
  * counterpart in the reference: a measurement aid).  *gram_ms: the rho_A Gram kernels' durations,
  * *gram_flops: their SYRK FLOPs (k (k+1) K per real k x K block, 4x that for complex), *eig_ms:
