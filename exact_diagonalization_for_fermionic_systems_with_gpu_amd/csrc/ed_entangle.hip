@@ -691,10 +691,13 @@ static int block_spectra(int N, int Qu, int Qd, int64_t maskA, bool cplx, const 
             const int64_t kw = std::min<int64_t>(maxk, 2048);
             hipLaunchKernelGGL(k_warm_matrix, dim3((unsigned)((kw * kw + 255) / 256)), dim3(256), 0, st, kw,
                                cplx ? 1 : 0, dR);
+            // (its values go to dSall and its info to the second half of dinfo: dWall and the first
+            // half may already hold the results of the large blocks solved above)
             rocblas_status rs = cplx ? rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)kw,
-                                                        (rocblas_double_complex *)dR, (rocblas_int)kw, dWall, dE, dinfo)
+                                                        (rocblas_double_complex *)dR, (rocblas_int)kw, dSall, dE,
+                                                        dinfo + njobs)
                                      : rocsolver_dsyevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)kw, dR,
-                                                        (rocblas_int)kw, dWall, dE, dinfo);
+                                                        (rocblas_int)kw, dSall, dE, dinfo + njobs);
             if (rs == rocblas_status_success && want_sigma)
                 rs = cplx ? rocsolver_zgesvd(h, rocblas_svect_none, rocblas_svect_none, (rocblas_int)kw,
                                              (rocblas_int)kw, (rocblas_double_complex *)dR, (rocblas_int)kw, dSall,

@@ -108,3 +108,45 @@ def test_concurrent_block_solves_match_sequential(N, Q, nA, cplx, monkeypatch):
         lam_o, _ = E.block_spectra(N, Q, Q, (1 << nA) - 1, psi.cpu().numpy())
         assert np.allclose(lam_4, np.sort(lam_o)[::-1], atol=1e-13)
 
+
+
+_FIRST_CALL_SCRIPT = r"""
+import ctypes as C, os, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np, torch
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd import Lanczos
+from exact_diagonalization_for_fermionic_systems_with_gpu_amd._lib import call, dptr, stream_ptr
+torch.cuda.set_device(0)
+D = 48620 * 48620
+x = torch.empty(D, dtype=torch.float64, device="cuda")
+Lanczos(48620, 48620).random_vector(x)
+out = []
+for one_stream in (False, True):
+    if one_stream:
+        os.environ["ED_EE_PARALLEL_MIN"] = str(1 << 40)
+    s = np.zeros(1)
+    call("ed_entanglement_entropy_vn", 18, 9, 9, (1 << 9) - 1, 0, dptr(x), s.ctypes.data_as(C.POINTER(C.c_double)),
+         stream_ptr())
+    out.append(s[0])
+print("S", repr(float(out[0])), repr(float(out[1])), flush=True)
+"""
+
+
+def test_first_call_of_process_18_sites(tmp_path):
+    """The first multi-worker entanglement call of a process (the one that warms rocSOLVER up on the
+    calling thread before the worker threads start) at 18 sites, 9|9: the large blocks are solved
+    before the warm-up, whose values must not land on theirs.  Fresh process; the same S_vN as the
+    one-stream path."""
+    import os
+    import subprocess
+    import sys
+    script = tmp_path / "first_call.py"
+    script.write_text(_FIRST_CALL_SCRIPT)
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("entropy 18 sites, first call of a fresh process", flush=True)
+    r = subprocess.run([sys.executable, str(script), repo], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = [l for l in r.stdout.splitlines() if l.startswith("S ")][-1]
+    s_workers, s_one = (float(v) for v in line.split()[1:])
+    assert 0.0 < s_one < 18 * math.log(4.0)
+    assert abs(s_workers - s_one) <= 1e-10, (s_workers, s_one)
