@@ -391,7 +391,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the disorder-ensemble line (--ensemble realisations, --workers threads)")
     ap.add_argument("--workers", type=int, default=1, help="host threads of the disorder-ensemble line")
-    ap.add_argument("--ensemble-method", default="lanczos", choices=["lanczos", "restart"],
+    ap.add_argument("--ensemble-method", default="lanczos", choices=["batched", "lanczos", "restart"],
                     help="eigensolver of the disorder-ensemble line")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")

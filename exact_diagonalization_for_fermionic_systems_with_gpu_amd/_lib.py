@@ -93,6 +93,7 @@ SIGNATURES = {
     "ed_lanczos_lowest": (_i32, [_p, _i32, C.POINTER(LanczosCfg), _p, _i32, _pd, _p, _pi, _p]),
     "ed_lanczos_tridiagonal": (_i32, [_p, _i32, _i32, _p, _pd, _pd, _p]),
     "ed_tridiag_eigen": (_i32, [_i32, _pd, _pd, _pd, _pd]),
+    "ed_tridiag_lanczos_lowest": (_i32, [_i32, _pd, _pd, C.c_double, _i32, C.c_double, C.c_double, _i32, _pd, _pi, _pi]),
     "ed_lanczos_orthogonality": (_i32, [_i32, _pd, _pd, _pd]),
     "ed_lanczos_so_stats": (_i32, [_pi, _pi, _pi, _pd]),
     "ed_krylov_tridiagonal": (_i32, [_p, _p, _i32, _d, _pd, _pd, _pi, _p]),
@@ -104,6 +105,7 @@ SIGNATURES = {
     "ed_batch_create": (_i32, [_i32, _p, _i32, C.POINTER(_p)]),
     "ed_batch_destroy": (_i32, [_p]),
     "ed_batch_lowest": (_i32, [_p, C.POINTER(LanczosCfg), C.c_uint32, _i32, _pd, _pi, _pi, _p]),
+    "ed_batch_lowest_k": (_i32, [_p, C.POINTER(LanczosCfg), _i32, C.c_uint32, _i32, _pd, _pi, _pi, _p]),
     "ed_batch_set_shape": (_i32, [_p, _i32, _i32, _i32, _i32]),
     "ed_batch_autotune": (_i32, [_p, _i32, C.POINTER(BatchTuning), _p]),
     "ed_bond_bilinear": (_i32, [_p, _i32, _i32, _p, _p, _pd, _p]),

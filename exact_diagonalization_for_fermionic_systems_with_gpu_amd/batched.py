@@ -70,6 +70,20 @@ class BatchedLanczos:
         return np.array(ev[:]), np.array(ns[:]), np.array(st[:])
 
 
+    def lowest_k(self, nev: int, steps=400, tol=None, use_graph=True, seed=235):
+        """The nev lowest eigenvalues of every instance from one batched run of `steps` steps
+        (ed_batch_lowest_k: Cullum-Willoughby filter of each tridiagonal, ed_lanczos_eigen's
+        residual test on each kept value).  Returns (evals [n, nev], nsteps [n], status [n])."""
+        n = len(self.hubbards)
+        cfg = LanczosCfg(int(steps), 0.0 if tol is None else tol, 10, 0)
+        ev = np.zeros(n * nev)
+        ns = (C.c_int * n)()
+        st = (C.c_int * n)()
+        call("ed_batch_lowest_k", self._b, C.byref(cfg), int(nev), seed, int(use_graph),
+             ev.ctypes.data_as(C.POINTER(C.c_double)), ns, st, stream_ptr())
+        return ev.reshape(n, nev), np.array(ns[:]), np.array(st[:])
+
+
 def plain_lowest(hubbards, complex_vectors: bool, max_steps=1000, tol=None):
     """One instance after another: the reference's loop over samples/sectors."""
     evs, ns = [], []

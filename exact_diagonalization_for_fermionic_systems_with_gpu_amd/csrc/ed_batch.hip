@@ -741,8 +741,10 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
     return ED_OK;
 }
 
-extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
-                               int *nsteps, int *status, void *stream)
+// The batched solve; ha, hb (optional) get the a and b rows of every copy, step-major (nc per row,
+// max_steps + 3 rows; rows past a copy's last step are not set).
+static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
+                     int *nsteps, int *status, void *stream, std::vector<double> *ha, std::vector<double> *hb)
 {
     Batch *B = reinterpret_cast<Batch *>(bb);
     if (!B || !evals) return fail(ED_EINVAL, "ed_batch_lowest: bad arguments");
@@ -1001,9 +1003,67 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
             if (nsteps) nsteps[i] = hc[c].last + 1;
             if (status) status[i] = hc[c].conv ? ED_OK : ED_ENOCONV;
         }
+        if (ha && hb) {
+            ha->assign((size_t)nc * ld, 0.0);
+            hb->assign((size_t)nc * ld, 0.0);
+            if (hipMemcpyAsync(ha->data(), adev, sizeof(double) * (size_t)nc * ld, hipMemcpyDeviceToHost, st) !=
+                    hipSuccess ||
+                hipMemcpyAsync(hb->data(), bdev, sizeof(double) * (size_t)nc * ld, hipMemcpyDeviceToHost, st) !=
+                    hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_batch_lowest: tridiagonal readback failed");
+                break;
+            }
+        }
     } while (0);
     cleanup();
     return s;
+}
+
+extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
+                               int *nsteps, int *status, void *stream)
+{
+    return batch_run(bb, cfg, seed, use_graph, evals, nsteps, status, stream, nullptr, nullptr);
+}
+
+extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph,
+                                 double *evals, int *nsteps, int *status, void *stream)
+{
+    Batch *B = reinterpret_cast<Batch *>(bb);
+    if (!B || !evals || nev < 1 || nev > 64) return fail(ED_EINVAL, "ed_batch_lowest_k: bad arguments (nev 1..64)");
+    if (cfg && cfg->reorth) return fail(ED_EUNSUPPORTED, "ed_batch_lowest_k: no reorthogonalisation in the batch");
+    const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
+    const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
+    // every copy runs max_steps steps (a test no copy passes stops none of them; a breakdown, an
+    // invariant subspace, still ends a copy early)
+    const ed_lanczos_cfg run{maxN, 1e-300, 10, 0};
+    std::vector<double> ev((size_t)B->n), ha, hb;
+    std::vector<int> ns((size_t)B->n), st((size_t)B->n);
+    ED_TRY(batch_run(bb, &run, seed, use_graph, ev.data(), ns.data(), st.data(), stream, &ha, &hb));
+    const int nc = B->ncopies;
+    std::vector<double> a, e;
+    for (int i = 0; i < B->n; ++i) {
+        const int c = B->copy_of[i];
+        const int M = ns[(size_t)i]; // steps kept: a[0..M-1], b[0..M]
+        a.assign((size_t)M, 0.0);
+        e.assign((size_t)std::max(M, 1), 0.0);
+        std::vector<double> b((size_t)M + 1);
+        for (int k = 0; k < M; ++k) a[(size_t)k] = ha[(size_t)k * nc + c];
+        for (int k = 0; k <= M; ++k) b[(size_t)k] = hb[(size_t)k * nc + c];
+        double anorm = 0.0;
+        for (int k = 0; k < M; ++k)
+            anorm = k == 0 ? std::fabs(a[0] + b[1]) : std::fmax(anorm, b[k] + std::fabs(a[k] + b[k + 1]));
+        for (int k = 0; k + 1 < M; ++k) e[(size_t)k] = b[(size_t)k + 1];
+        const bool exact = st[(size_t)i] == ED_OK && M < maxN; // the recurrence ended in an invariant subspace
+        bool conv = false;
+        double *out = evals + (size_t)i * nev;
+        const int f = ed::cw_lowest(M, a.data(), e.data(), exact ? 0.0 : b[(size_t)M], nev, tol, anorm, false, out,
+                                    &conv);
+        for (int k = f; k < nev; ++k) out[k] = std::numeric_limits<double>::quiet_NaN();
+        if (nsteps) nsteps[i] = M;
+        if (status) status[i] = conv ? ED_OK : ED_ENOCONV;
+    }
+    return ED_OK;
 }
 
 extern "C" int ed_batch_set_shape(ed_batch bb, int dn_rows, int dn_panels, int up_rows, int up_panels)

@@ -154,5 +154,11 @@ int ritz_update(int64_t nreal, const double *v, double *f, double *evec, const d
 int dot_partials(int64_t nreal, const double *a, const double *b, double *partial, int64_t *nblocks,
                  hipStream_t st);
 int64_t blas_num_blocks(int64_t nreal);
+// The nev lowest eigenvalues of a Lanczos tridiagonal (a[0..M-1], couplings e[0..M-2]) with
+// copies of converged eigenvalues merged and spurious values dropped (Cullum-Willoughby), each
+// simple one tested |bres s[M-1]| < tol * anorm (ed_lanczos.hip); semi_orth: the recurrence ran
+// with selective orthogonalisation.  Returns the number found, *conv = all nev found and passing.
+int cw_lowest(int M, const double *a, const double *e, double bres, int nev, double tol, double anorm,
+              bool semi_orth, double *out, bool *conv);
 
 } // namespace ed

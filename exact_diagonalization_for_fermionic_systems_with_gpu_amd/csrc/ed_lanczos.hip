@@ -219,6 +219,67 @@ static int lanczos_step_vfree(const Hubbard &h, int64_t nreal, double *A, double
     return ED_OK;
 }
 
+// The nev lowest eigenvalues of a Lanczos tridiagonal T (a[0..M-1], couplings e[0..M-2]), after
+// the test of J. Cullum and R. A. Willoughby ("Lanczos Algorithms for Large Symmetric Eigenvalue
+// Computations", vol. 1, ch. 4): an eigenvalue of T that occurs more than once (within tol_m) is
+// a converged eigenvalue of H (a copy appears only once the eigenvalue has converged) and its
+// extra copies are dropped; a simple one that is not also an eigenvalue of T^ (T with its first
+// row and column removed) stands for an eigenvalue of H and needs |bres s[M-1]| < tol * anorm
+// (s its unit eigenvector of T, bres the coupling after the last step), the residual test of
+// ed_lanczos_eigen.  A simple eigenvalue of T that is also one of T^ has a tiny first component
+// s[0]: either a ghost (spurious) or an eigenvalue of H that the start vector barely touches
+// (an overlap of 1e-7 puts it within 1e-12 |T| of T^).  It counts as the latter when it passes
+// the residual test with no other eigenvalue of T within sqrt(tol) |T| (a converged simple Ritz
+// value is within its residual of an eigenvalue of H, Paige), otherwise as a ghost; an
+// unconverged one is dropped after a recurrence without reorthogonalisation (a ghost in transit,
+// new ones keep forming) and kept, holding the test back, after one with semi_orth set
+// (selective orthogonalisation forms no ghosts away from converged values).  (Within a cluster of
+// copies inverse iteration returns some mixture of their vectors, whose last component says
+// nothing about convergence.)  Returns the number found (at most nev); *conv = every one found
+// passes.
+int cw_lowest(int M, const double *a, const double *e, double bres, int nev, double tol, double anorm,
+              bool semi_orth, double *out, bool *conv)
+{
+    *conv = false;
+    if (M < 1) return 0;
+    std::vector<double> th((size_t)M), thh((size_t)std::max(M - 1, 1));
+    ed_tridiag_eigvals(M, a, e, th.data());
+    if (M > 1) ed_tridiag_eigvals(M - 1, a + 1, e + 1, thh.data());
+    // multiplicity and spurious-value tolerance: copies of a converged eigenvalue agree to a few
+    // hundred eps |T|, while distinct eigenvalues of H can lie 1e-8 apart
+    const double tm = 1e-12 * anorm;
+    const double tg = std::sqrt(tol) * anorm; // a ghost's distance to its twin while it converges
+    std::vector<double> ws(5 * (size_t)M), z((size_t)M);
+    std::vector<char> sw((size_t)M);
+    int found = 0;
+    bool ok = true;
+    for (int j = 0; j < M && found < nev;) {
+        int r = j + 1;
+        while (r < M && th[r] - th[r - 1] <= tm) ++r;
+        const double v = th[j];
+        bool keep = true;
+        if (r - j == 1) {
+            tri_inverse_iteration(M, a, e, v, z.data(), ws.data(), sw.data());
+            const bool converged = std::fabs(bres * z[M - 1]) < tol * anorm;
+            bool in_that = false;
+            if (M > 1) {
+                const double *q = std::lower_bound(thh.data(), thh.data() + (M - 1), v - tm);
+                in_that = q < thh.data() + (M - 1) && *q <= v + tm;
+            }
+            if (in_that) {
+                const bool isolated = (j == 0 || v - th[j - 1] > tg) && (r >= M || th[r] - v > tg);
+                if (converged) keep = isolated;
+                else keep = semi_orth;
+            }
+            if (keep) ok = ok && converged;
+        }
+        if (keep) out[found++] = v;
+        j = r;
+    }
+    *conv = ok && found == nev;
+    return found;
+}
+
 // ---- selective orthogonalisation (Parlett & Scott) -----------------------------------
 // B. N. Parlett and D. S. Scott, "The Lanczos algorithm with selective orthogonalization",
 // Math. Comp. 33 (1979) 217-238.  The Lanczos vectors lose orthogonality only along Ritz vectors
@@ -297,7 +358,12 @@ struct SelOrth {
     // lowest ones, so the basis need only stay orthogonal to the converged vectors at the low end)
     double theta_cap = std::numeric_limits<double>::infinity();
     int analyses = 0, purges = 0;
-    ~SelOrth() { dfree(coef); }
+    double *part = nullptr;       // device: dot partials of the duplicate check
+    ~SelOrth()
+    {
+        dfree(coef);
+        dfree(part);
+    }
     double *vec(int k) { return chunks[(size_t)k / kChunk] + (size_t)(k % kChunk) * (size_t)n; }
     int new_y(double **y) { return SoPool::get(t_so_pool.yv, t_so_pool.yv_cap, Y.size(), (size_t)n, y); }
     int store(int k, const double *v, hipStream_t st)
@@ -366,20 +432,50 @@ struct SelOrth {
         for (int i = 0; i < M; ++i) e[i] = b[i + 1];
         ql_eig(M + 1, d.data(), e.data(), nullptr);
         std::sort(d.begin(), d.end());
+        const int per = cvec ? 2 : 1; // entries of Y per kept Ritz vector (y, and i y for complex)
         for (int i = 0; i <= M; ++i) {
             const double th = d[i];
             if (th > theta_cap) break; // ascending
+            // a kept vector's own Ritz value is found again here, moved by at most rounding: skip
+            // it; one a little further away may still be the same vector (its value moves while it
+            // converges) or a distinct eigenvalue close by (1e-8 apart happens), which the
+            // overlap test below tells apart
             bool known = false;
-            for (double t : theta) known = known || std::fabs(t - th) <= std::sqrt(eps) * tn;
+            for (double t : theta) known = known || std::fabs(t - th) <= 1e-12 * tn;
             if (known) continue;
             inverse_iteration(M + 1, a, b + 1, th, z.data());
             if (!(std::fabs(b[M + 1] * z[M]) <= std::sqrt(eps) * tn)) continue;
             double *y = nullptr;
             ED_TRY(new_y(&y));
-            Y.push_back(y);
-            theta.push_back(th);
             ED_TRY(combine(M, z.data(), y, st));
             ED_TRY(ed_normalize(n, 0, y, nullptr, st));
+            const int nkept = (int)theta.size() / per;
+            int dup = -1;
+            if (nkept > 0) {
+                // |<y_q, y>| for every kept y_q (complex: from the real dots with y_q and i y_q)
+                const int64_t nb = blas_num_blocks(n);
+                if (!part) ED_TRY(dmalloc((void **)&part, sizeof(double) * (size_t)nb));
+                ED_TRY(reserve_coef(std::max(M + 1, (int)Y.size() + 2)));
+                for (size_t q = 0; q < Y.size(); ++q) {
+                    int64_t nd;
+                    ED_TRY(dot_partials(n, Y[q], y, part, &nd, st));
+                    ED_TRY(finalize_sum(part, nd, coef + q, 0, st));
+                }
+                std::vector<double> ov(Y.size());
+                ED_HIP(hipMemcpyAsync(ov.data(), coef, sizeof(double) * Y.size(), hipMemcpyDeviceToHost, st));
+                ED_HIP(hipStreamSynchronize(st));
+                for (int q = 0; q < nkept; ++q) {
+                    double o2 = 0.0;
+                    for (int c = 0; c < per; ++c) o2 += ov[(size_t)q * per + c] * ov[(size_t)q * per + c];
+                    if (o2 > 0.25) dup = q;
+                }
+            }
+            if (dup >= 0) { // the same vector again: remember its current value
+                for (int c = 0; c < per; ++c) theta[(size_t)dup * per + c] = th;
+                continue;
+            }
+            Y.push_back(y);
+            theta.push_back(th);
             if (cvec) {
                 double *iy = nullptr;
                 ED_TRY(new_y(&iy));
@@ -441,6 +537,19 @@ extern "C" int ed_tridiag_eigvals(int n, const double *d, const double *e, doubl
     ql_eig(n, dd.data(), ee.data(), nullptr);
     std::sort(dd.begin(), dd.end());
     for (int i = 0; i < n; ++i) w[i] = dd[i];
+    return ED_OK;
+}
+
+extern "C" int ed_tridiag_lanczos_lowest(int n, const double *d, const double *e, double bres, int nev, double tol,
+                                         double anorm, int semi_orth, double *out, int *found, int *conv)
+{
+    if (n < 1 || !d || (n > 1 && !e) || nev < 1 || !out || !(tol > 0.0) || !(anorm > 0.0))
+        return fail(ED_EINVAL, "ed_tridiag_lanczos_lowest: bad arguments");
+    for (int k = 0; k < nev; ++k) out[k] = std::numeric_limits<double>::quiet_NaN();
+    bool c = false;
+    const int f = cw_lowest(n, d, e, bres, nev, tol, anorm, semi_orth != 0, out, &c);
+    if (found) *found = f;
+    if (conv) *conv = c ? 1 : 0;
     return ED_OK;
 }
 
@@ -607,24 +716,25 @@ static int lanczos_eigen_impl(const Hubbard *h, int complex_vectors, const ed_la
             ed_tridiag_lowest(M + 1, a.data(), b.data() + 1, &lam, z.data());
             *eval = lam;
             lastN = M;
-            // nev > 1: the test of every one of the nev lowest Ritz pairs of T_{M+1}; with the
-            // Lanczos vectors kept semi-orthogonal (selective orthogonalisation) T has no spurious
-            // copies of converged eigenvalues, so its k-th lowest Ritz value stands for the k-th
-            // eigenvalue of H
+            // nev > 1: the nev lowest eigenvalues of T_{M+1} with copies of converged eigenvalues
+            // merged and converging ghosts dropped (cw_lowest), each simple one passing the
+            // residual test.  Selective orthogonalisation keeps most copies from forming, but not
+            // around eigenvalues closer together than the accuracy at which a Ritz vector counts
+            // as good (a pair 1e-8 apart in a disordered chain gave a third copy); it forms no
+            // ghosts elsewhere, so a value the start vector barely touches is kept.
             bool rest = true;
-            const int nk = std::min(nev, M + 1);
-            for (int k = 1; k < nk; ++k) {
-                eval[k] = bisect_kth(M + 1, a.data(), b.data() + 1, k);
-                std::vector<double> zk((size_t)M + 1);
-                inverse_iteration(M + 1, a.data(), b.data() + 1, eval[k], zk.data());
-                rest = rest && std::fabs(b[M + 1] * zk[M]) < tol * anorm;
+            const bool exact = M < N || !(b[M + 1] > 0.0);
+            if (nev > 1) {
+                std::vector<double> out((size_t)nev, std::numeric_limits<double>::quiet_NaN());
+                const int f = cw_lowest(M + 1, a.data(), b.data() + 1, exact ? 0.0 : b[M + 1], nev, tol, anorm,
+                                        true, out.data(), &rest);
+                (void)f;
+                for (int k = 0; k < nev; ++k) eval[k] = out[(size_t)k];
             }
-            for (int k = nk; k < nev; ++k) eval[k] = std::numeric_limits<double>::quiet_NaN();
-            if (M < N || !(b[M + 1] > 0.0)) {
+            if (exact) {
                 status = ED_OK; // exact invariant subspace: T_{M+1} eigenvalues are exact
                 stop = true;
-            } else if (nk == nev && rest && std::fabs(b[M + 1] * z[M]) < tol * anorm &&
-                       std::fabs(z[0]) > tol * anorm) {
+            } else if (rest && std::fabs(b[M + 1] * z[M]) < tol * anorm && std::fabs(z[0]) > tol * anorm) {
                 status = ED_OK;
                 stop = true;
             }
@@ -733,7 +843,13 @@ extern "C" int ed_lanczos_lowest(ed_hubbard hh, int complex_vectors, const ed_la
     int n2 = 0;
     const int s2 = lanczos_eigen_impl(h, complex_vectors, &c2, d_f, 1, &e0, d_evec, nullptr, nullptr, &n2, stream);
     if (nsteps) *nsteps = n1 + n2;
-    return s2;
+    if (s2 != ED_OK) return s2;
+    // the refined vector is the Ritz vector of T's lowest eigenvalue: its value must be evals[0]
+    // (a lowest value left out of evals by the spurious-value test would show up here)
+    if (!(std::fabs(e0 - evals[0]) <= 1e-8 * std::fmax(1.0, std::fabs(e0))))
+        return fail(ED_ENOCONV, "ed_lanczos_lowest: the ground state's value %.15g is not the lowest eigenvalue "
+                                "found (%.15g)", e0, evals[0]);
+    return ED_OK;
 }
 
 extern "C" int ed_lanczos_tridiagonal(ed_hubbard hh, int complex_vectors, int nsteps, const void *d_f,

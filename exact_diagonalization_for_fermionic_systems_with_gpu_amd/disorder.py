@@ -62,6 +62,7 @@ class EnsembleResult:
     grad_eps: np.ndarray     # [R, L]
     seconds: float
     seconds_batched_e0: float
+    fallbacks: int = 0       # method "batched": realisations whose batched run did not pass the test
 
     @property
     def realisations_per_s(self):
@@ -91,23 +92,32 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
                  tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
-                 max_steps=2000) -> EnsembleResult:
-    """workers > 1 solves that many realisations at once after the batched E0 launch, each worker
-    thread on its own stream.  method "lanczos": ed_lanczos_lowest (at most max_steps steps; tol
-    is its test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart":
-    the thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).
+                 max_steps=2000, batch_steps=400) -> EnsembleResult:
+    """workers > 1 solves that many realisations at once after the batched launch, each worker
+    thread on its own stream.  method "batched": the nev lowest of every realisation from one
+    batched run of batch_steps steps (ed_batch_lowest_k), then per realisation its ground state by
+    a plain Lanczos run with the Ritz vector rebuilt (ed_lanczos_eigen, test tolerance tol); a
+    realisation whose batched values fail the test falls back to "lanczos".  "lanczos": after a
+    batched E0 launch, ed_lanczos_lowest per realisation (at most max_steps steps; tol is its
+    test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart": the
+    thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).
     Without S_ref the entropies are taken after the solves, all realisations in one batched
     call."""
-    if method not in ("lanczos", "restart"):
-        raise ValueError("run_ensemble: method is 'lanczos' or 'restart'")
+    if method not in ("batched", "lanczos", "restart"):
+        raise ValueError("run_ensemble: method is 'batched', 'lanczos' or 'restart'")
     eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hs = [build_realisation(eps[r], L_sites, n_up, n_dn, U, t) for r in range(n_real)]
     tb = time.perf_counter()
-    e0b, _, _ = BatchedLanczos(hs, False).lowest(tol=1e-12)
+    if method == "batched":
+        evb, _, stb = BatchedLanczos(hs, False).lowest_k(nev, steps=batch_steps, tol=tol)
+        e0b = evb[:, 0].copy()
+    else:
+        e0b, _, _ = BatchedLanczos(hs, False).lowest(tol=1e-12)
     torch.cuda.synchronize()
     tb = time.perf_counter() - tb
+    fallbacks = 0
     evals = np.zeros((n_real, nev))
     S = np.zeros((n_real, 2))
     g = np.zeros((n_real, L_sites))
@@ -132,7 +142,14 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
                 psi = torch.empty_like(f)
                 lz = Lanczos(H.dimUP, H.dimDN, max_steps=max_steps, tol=tol)
                 lz.random_vector(f)
-                evals[r] = lz.lowest(H, f, nev, psi)
+                if method == "batched" and stb[r] == 0:
+                    evals[r] = evb[r]
+                    lz.eigen(H, f, psi) # the ground state only
+                else:
+                    if method == "batched":
+                        nonlocal fallbacks
+                        fallbacks += 1
+                    evals[r] = lz.lowest(H, f, nev, psi)
                 del f
             if with_sref:
                 S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, True)
@@ -169,4 +186,4 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
         S[:, 1] = np.nan
         del psis[:]
     torch.cuda.synchronize()
-    return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb)
+    return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb, fallbacks)

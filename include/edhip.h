@@ -280,14 +280,17 @@ int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cf
 /* The nev lowest eigenvalues of H by one Lanczos run with selective orthogonalisation (the role
  * of cuSpectra::SymEigsSolver's "SmallestAlge" solve, TotalSpin/include/SymEigsSolver.cuh:332-359,
  * which restarts a 2 nev + 1 vector Krylov basis; here one recurrence runs on, its vectors kept
- * semi-orthogonal against the converged Ritz vectors as with reorth = 2, so its tridiagonal has no
- * spurious copies and its k-th lowest Ritz value stands for the k-th eigenvalue).  Every
- * check_every steps each of the nev lowest Ritz pairs (theta_k, s_k) of T must pass
- * |b_{N+1} s_k[N]| < tol * anorm (ed_lanczos_eigen's test, applied to all nev); the lowest also
- * needs |s_0[0]| > tol * anorm.  evals[nev] ascending; d_evec (may be NULL) gets the normalised
- * ground state: V s_0, refined by a plain Lanczos run from it (ed_lanczos_eigen with the same
- * test, Ritz vector rebuilt), since the purges leave V s_0 off by about sqrt(eps); *nsteps counts
- * the steps of both runs.  cfg->reorth is ignored; nev 1..64.  The basis is kept on the device:
+ * semi-orthogonal against the converged Ritz vectors up to the nev-th lowest as with reorth = 2).
+ * Every check_every steps the nev lowest eigenvalues of T are taken with copies of converged
+ * eigenvalues merged and converging ghost copies dropped (the Cullum-Willoughby test of
+ * ed_batch_lowest_k: the purges keep most copies from forming, not those of eigenvalues closer
+ * together than the accuracy at which a Ritz vector counts as converged; a value the start
+ * vector barely touches is kept), and each simple one must pass |b_{N+1} s_k[N]| < tol * anorm
+ * (ed_lanczos_eigen's test); the lowest also needs |s_0[0]| > tol * anorm.  evals[nev]
+ * ascending; d_evec (may be NULL) gets the normalised ground state: V s_0, refined by a plain
+ * Lanczos run from it (ed_lanczos_eigen with the same test, Ritz vector rebuilt), since the
+ * purges leave V s_0 off by about sqrt(eps); the refined value must agree with evals[0] to 1e-8
+ * relative (ED_ENOCONV otherwise); *nsteps counts the steps of both runs.  cfg->reorth is ignored; nev 1..64.  The basis is kept on the device:
  * ED_ENOMEM when it does not fit.  Returns ED_ENOCONV (with the last Ritz values) when a test is
  * not met by max_steps.  Synchronises the stream. */
 int ed_lanczos_lowest(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cfg, void *d_f, int nev,
@@ -322,6 +325,14 @@ int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
  * Krylov::Make_tridiagonal_eigen calls it, HubMatrix_GPU.cuh:690-699): d[n], e[n-1]; w[n]
  * ascending; z[n*n] column-major, z[j*n + i] = component i of the unit eigenvector of w[j]. */
 int ed_tridiag_eigen(int n, const double *d, const double *e, double *w, double *z);
+/* The nev lowest eigenvalues of a Lanczos tridiagonal (d[n], e[n-1]) with bres = b_{n+1}, the
+ * coupling after the last step, by the filter ed_lanczos_lowest (semi_orth = 1) and
+ * ed_batch_lowest_k (semi_orth = 0) apply: copies merged, ghosts dropped, each kept simple value
+ * tested |bres s[n-1]| < tol * anorm.  out[nev] ascending (NaN past the values found); *found =
+ * the number found; *conv = 1 when all nev were found and pass.  No counterpart in the reference
+ * (its SymEigsSolver restarts instead of filtering); exported for tests of the filter. */
+int ed_tridiag_lanczos_lowest(int n, const double *d, const double *e, double bres, int nev, double tol,
+                              double anorm, int semi_orth, double *out, int *found, int *conv);
 
 /* Krylov<C>::tridiagonal (HubMatrix_GPU.cuh:660-680): Lanczos recurrence from the complex128
  * vector d_phi0 (used as it is, not normalised, not modified) without reorthogonalisation, at
@@ -374,6 +385,20 @@ int ed_batch_destroy(ed_batch b);
  * the same kernels without capture.  Synchronises. */
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
+/* The nev lowest eigenvalues of every instance from one batched run (the SymEigsSolver
+ * "SmallestAlge" role, SymEigsSolver.cuh:332-359, for many instances at once): every instance
+ * runs cfg->max_steps steps (fewer only when its recurrence ends in an invariant subspace), then
+ * its tridiagonal is filtered by the Cullum-Willoughby test (copies of converged eigenvalues
+ * merged; a simple value that is also an eigenvalue of T without its first row and column is
+ * dropped as spurious unless it passes the residual test with no other eigenvalue of T within
+ * sqrt(tol) anorm: that is an eigenvalue of H the start vector barely touches, an overlap of
+ * 1e-7 is enough to put it there) and each of the nev lowest kept values must pass
+ * ed_lanczos_eigen's residual test |b s_N| < tol * anorm.  evals[ninst * nev] (instance-major,
+ * ascending, NaN past the values
+ * found), nsteps[ninst], status[ninst]: ED_OK, or ED_ENOCONV when the test is not met.
+ * Synchronises. */
+int ed_batch_lowest_k(ed_batch b, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph, double *evals,
+                      int *nsteps, int *status, void *stream);
 /* Panel shape of the batch kernels: DN rows per panel (4 or 8) x DN panels per workgroup (1-16),
  * UP rows per panel (1 or 4) x UP panels per workgroup (1-64); default 4 x 4, 1 x 4.  The
  * eigenvalues do not depend on it, bit for bit (same per-element sums; the dot partials are per

@@ -105,13 +105,16 @@ def test_eigs_kagome12_vs_lanczos_and_arpack():
     assert np.allclose(ev, wa, rtol=1e-10, atol=0), (ev, wa)
 
 
-@pytest.mark.parametrize("method", ["lanczos", "restart"])
+@pytest.mark.parametrize("method", ["batched", "lanczos", "restart"])
 def test_disorder_ensemble_vs_dense_chain8(method):
     print(f"disorder ensemble chain8 x4 ({method})", flush=True)
     # the Lanczos test scales with |T| (about 3 |E0| here), the restart solver's with |theta|: S_ref
     # needs the ground state to ~1e-11, so the Lanczos run gets the tighter bar
     tol = 1e-14 if method == "lanczos" else 1e-13
     res = DS.run_ensemble(n_real=4, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=4, ncv=16, tol=tol, method=method)
+    # the batched run goes 400 steps on D = 4900, far past convergence: its tridiagonals hold
+    # copies of the converged eigenvalues, which the Cullum-Willoughby filter has to remove
+    assert res.fallbacks == 0
     for r in range(4):
         ops = DS.chain_disorder_ops(8, res.eps[r])
         Ho = O.Hubbard(O.Species(8, 4, ops), O.Species(8, 4, ops), 4.0)
@@ -260,3 +263,71 @@ def test_entropy_vn_batch_matches_per_state():
         # the oracle on one state
         lam, _ = EE.block_spectra(N, nu, nd, mask, states[0].cpu().numpy())
         assert abs(sb[0] - EE.entropy_from_lambda(lam)) < 1e-10
+
+
+def test_batched_lowest_k_against_per_instance():
+    """ed_batch_lowest_k on the 14-site chain (D = 1.18e7, one group of 6 realisations sharing the
+    operator structure): the 4 lowest of every realisation against ed_lanczos_lowest on each, and
+    the batched E0 against the plain Lanczos E0."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd.batched import BatchedLanczos
+    print("batched lowest_k chain14 x6", flush=True)
+    hs = [DS.build_realisation(DS.realisation_eps(r, 14, 8.0), 14, 7, 7) for r in range(6)]
+    ev, ns, st = BatchedLanczos(hs, False).lowest_k(4, steps=500, tol=1e-10)
+    # a realisation whose lowest levels lie close together may need more steps (ED_ENOCONV, the
+    # ensemble then solves it alone); the others must agree with the per-instance solver
+    assert (ns == 500).all() and (st == 0).sum() >= 4, (st, ns)
+    for r, H in enumerate(hs):
+        if st[r] != 0:
+            continue
+        f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+        lz = Lanczos(H.dimUP, H.dimDN, max_steps=2000, tol=1e-10)
+        lz.random_vector(f)
+        ref = lz.lowest(H, f, 4)
+        assert np.allclose(ev[r], ref, rtol=1e-10, atol=0), (r, ev[r], ref)
+
+
+def test_lanczos_lowest_near_degenerate_pairs_chain14():
+    """Realisation 25 of the 14-site W = 8 chain has two pairs of levels 1e-8 apart
+    (-46.81513937, -46.81513936, -46.00713629, -46.00713628): selective orthogonalisation counts a
+    Ritz vector good long before it resolves such a pair, and without the Cullum-Willoughby filter
+    the solver reported a third copy of the lowest pair.  Against the thick-restart solver and the
+    batched run."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd.batched import BatchedLanczos
+    print("lanczos lowest chain14 realisation 25 (near-degenerate pairs)", flush=True)
+    H = DS.build_realisation(DS.realisation_eps(25, 14, 8.0), 14, 7, 7)
+    f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+    lz = Lanczos(H.dimUP, H.dimDN, max_steps=3000, tol=1e-10)
+    lz.random_vector(f)
+    ev = lz.lowest(H, f, 4)
+    es = SymEigsSolver(HubbardOp(H), 4, 24)
+    es.init()
+    es.compute("SmallestAlge", 1000, 1e-12)
+    assert es.info() == "Successful"
+    ref = es.eigenvalues()
+    assert ref[1] - ref[0] < 1e-7 and ref[3] - ref[2] < 1e-7  # the two close pairs
+    assert np.allclose(ev, ref, rtol=1e-10, atol=0), (ev, ref)
+    evb, _, st = BatchedLanczos([H], False).lowest_k(4, steps=500, tol=1e-10)
+    assert st[0] == 0 and np.allclose(evb[0], ref, rtol=1e-10, atol=0), (evb, ref)
+
+
+def test_lanczos_lowest_start_vector_barely_touching_ground_state_chain14():
+    """Realisation 125 of the 14-site W = 8 chain: the seed-235 start vector overlaps its ground
+    state by 1.1e-7, so the converged lowest Ritz value is also an eigenvalue of T without its
+    first row and column; the spurious-value filter once dropped it and the solver returned the
+    first excited level (-11.7151) as E0.  Against the plain Lanczos E0 and the batched run."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd.batched import BatchedLanczos
+    print("lanczos lowest chain14 realisation 125 (start vector 1e-7 from the ground state)", flush=True)
+    H = DS.build_realisation(DS.realisation_eps(125, 14, 8.0), 14, 7, 7)
+    f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(f)
+    lz = Lanczos(H.dimUP, H.dimDN, max_steps=3000, tol=1e-12)
+    lz.random_vector(f)
+    f0 = f.clone()
+    e0 = lz.eigen(H, f, psi)
+    assert abs(float(torch.dot(f0, psi))) < 1e-6  # the case: a start vector nearly orthogonal to psi0
+    f.copy_(f0)
+    lw = Lanczos(H.dimUP, H.dimDN, max_steps=2000, tol=1e-10)
+    ev = lw.lowest(H, f, 4, psi)
+    assert abs(ev[0] - e0) < 1e-10 * abs(e0), (ev, e0)
+    evb, _, st = BatchedLanczos([H], False).lowest_k(4, steps=400, tol=1e-10)
+    assert st[0] == 0 and np.allclose(evb[0], ev, rtol=1e-10, atol=0), (evb, ev)
