@@ -105,7 +105,7 @@ SIGNATURES = {
     "ed_batch_create": (_i32, [_i32, _p, _i32, C.POINTER(_p)]),
     "ed_batch_destroy": (_i32, [_p]),
     "ed_batch_lowest": (_i32, [_p, C.POINTER(LanczosCfg), C.c_uint32, _i32, _pd, _pi, _pi, _p]),
-    "ed_batch_lowest_k": (_i32, [_p, C.POINTER(LanczosCfg), _i32, C.c_uint32, _i32, _pd, _pi, _pi, _p]),
+    "ed_batch_lowest_k": (_i32, [_p, C.POINTER(LanczosCfg), _i32, C.c_uint32, _i32, _pd, _pi, _pi, _p, _p]),
     "ed_batch_set_shape": (_i32, [_p, _i32, _i32, _i32, _i32]),
     "ed_batch_autotune": (_i32, [_p, _i32, C.POINTER(BatchTuning), _p]),
     "ed_bond_bilinear": (_i32, [_p, _i32, _i32, _p, _p, _pd, _p]),

@@ -70,17 +70,29 @@ class BatchedLanczos:
         return np.array(ev[:]), np.array(ns[:]), np.array(st[:])
 
 
-    def lowest_k(self, nev: int, steps=400, tol=None, use_graph=True, seed=235):
+    def lowest_k(self, nev: int, steps=400, tol=None, use_graph=True, seed=235, evecs=None):
         """The nev lowest eigenvalues of every instance from one batched run of `steps` steps
         (ed_batch_lowest_k: Cullum-Willoughby filter of each tridiagonal, ed_lanczos_eigen's
-        residual test on each kept value).  Returns (evals [n, nev], nsteps [n], status [n])."""
+        residual test on each kept value).  evecs (optional): one device tensor per instance
+        (its dim, float64 or complex128 as the batch) that gets the instance's normalised ground
+        state when its status is 0 (a second batched run rebuilds the Ritz vectors).  Returns
+        (evals [n, nev], nsteps [n], status [n])."""
         n = len(self.hubbards)
         cfg = LanczosCfg(int(steps), 0.0 if tol is None else tol, 10, 0)
         ev = np.zeros(n * nev)
         ns = (C.c_int * n)()
         st = (C.c_int * n)()
+        vp = None
+        if evecs is not None:
+            if len(evecs) != n:
+                raise ValueError("lowest_k: one evec per instance")
+            want = torch.complex128 if self.complex_vectors else torch.float64
+            for H, v in zip(self.hubbards, evecs):
+                if not (v.is_cuda and v.dtype == want and v.is_contiguous() and v.numel() == H.dim):
+                    raise ValueError(f"lowest_k: evecs must be contiguous {want} CUDA tensors of the instances' dims")
+            vp = (C.c_void_p * n)(*[v.data_ptr() for v in evecs])
         call("ed_batch_lowest_k", self._b, C.byref(cfg), int(nev), seed, int(use_graph),
-             ev.ctypes.data_as(C.POINTER(C.c_double)), ns, st, stream_ptr())
+             ev.ctypes.data_as(C.POINTER(C.c_double)), ns, st, vp, stream_ptr())
         return ev.reshape(n, nev), np.array(ns[:]), np.array(st[:])
 
 

@@ -286,7 +286,8 @@ __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr,
                                                   const int *__restrict__ active, const int *__restrict__ gact,
                                                   const double *__restrict__ adev, const double *__restrict__ bdev,
                                                   int lda, const int *__restrict__ dstep, int cvec, const double *x,
-                                                  double *f, double *partial)
+                                                  double *f, double *partial, const double *__restrict__ zdev,
+                                                  const int *__restrict__ lim, double *acc)
 {
     __shared__ double s_red[256];
     const int64_t T = blockIdx.x;
@@ -297,18 +298,24 @@ __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr,
     const int gi = g.cbase + k;
     double s = 0.0;
     if (k < g.ncopy && active[gi]) {
-        const double a = -adev[(int64_t)*dstep * lda + gi];
-        const double sc = 1.0 / bdev[(int64_t)*dstep * lda + gi];
+        const int N = *dstep;
+        const double a = -adev[(int64_t)N * lda + gi];
+        const double sc = 1.0 / bdev[(int64_t)N * lda + gi];
         const int NC = cvec ? 2 : 1;
+        // the rebuild run: evec += z_N v_N over the copy's first lim steps
+        const bool ritz = acc != nullptr && N < lim[gi];
+        const double zc = ritz ? zdev[(int64_t)N * lda + gi] : 0.0;
 #pragma unroll
         for (int j = 0; j < GCH; ++j) {
             const int64_t e = (ch * GCH + j) * g.CW + c;
             if (e < g.nelem) {
                 const int64_t i = g.voff + (e * g.G + k) * NC;
                 for (int q = 0; q < NC; ++q) {
-                    const double r = fma(a, sc * x[i + q], f[i + q]);
+                    const double v = sc * x[i + q];
+                    const double r = fma(a, v, f[i + q]);
                     f[i + q] = r;
                     s = fma(r, r, s);
+                    if (ritz) acc[i + q] = fma(zc, v, acc[i + q]);
                 }
             }
         }
@@ -364,6 +371,29 @@ __global__ void k_g_b0(int cbase, int ncopy, int G, const double *__restrict__ n
     if (k >= G) return;
     const double v = *nrm;
     bdev[cbase + k] = (k < ncopy && v > 0.0) ? v : 1.0;
+}
+
+// The rebuild run's stand-in for the convergence test: after a chunk, copies that have run their
+// lim steps are switched off
+__global__ void k_g_limit(int nc, const int *__restrict__ dchk, int chunk, int *active, const int *__restrict__ cgroup,
+                          int *gact, const int *__restrict__ lim, int *remaining)
+{
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= nc || !active[c]) return;
+    if (*dchk + chunk >= lim[c]) {
+        active[c] = 0;
+        atomicSub(&gact[cgroup[c]], 1);
+        atomicSub(remaining, 1);
+    }
+}
+
+// copy k of a group's interleaved vector -> dst (one instance, NC doubles per element)
+__global__ __launch_bounds__(256) void k_g_gather(int64_t nelem, int G, int k, int NC, const double *__restrict__ src,
+                                                  double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nelem) return;
+    for (int q = 0; q < NC; ++q) dst[e * NC + q] = src[(e * G + k) * NC + q];
 }
 
 struct Batch {
@@ -709,7 +739,7 @@ __global__ void k_g_chk_bump(int *dchk, int chunk) { dchk[0] += chunk; }
 template <typename V, typename TV>
 static int enqueue_step(const Batch &B, const double *x, const double *xp, double *y, double *partial,
                         const int *active, const int *gact, double *adev, double *bdev, int ld, int *dstep,
-                        bool first, hipStream_t st)
+                        bool first, const double *zdev, const int *lim, double *acc, hipStream_t st)
 {
     const int64_t per1 = (B.t1 + 7) / 8, per2 = (B.t2 + 7) / 8;
     if (B.gu == 4)
@@ -733,7 +763,7 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
     hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 0, adev,
                        ld, dstep);
     hipLaunchKernelGGL(k_g_update, dim3((unsigned)B.tc), dim3(256), 0, st, B.dgr, B.dtgc, B.tc, active, gact, adev, bdev,
-                       ld, dstep, B.cvec ? 1 : 0, x, y, partial);
+                       ld, dstep, B.cvec ? 1 : 0, x, y, partial, zdev, lim, acc);
     hipLaunchKernelGGL(k_g_finalize, dim3(B.ncopies), dim3(1024), 0, st, B.dgr, B.dcgroup, active, partial, 1, bdev,
                        ld, dstep);
     hipLaunchKernelGGL(k_g_bump, dim3(1), dim3(1), 0, st, dstep);
@@ -741,13 +771,24 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
     return ED_OK;
 }
 
+// Ritz vectors by a second run of the same recurrence (same start vectors, kernels and launch
+// order, so the same a and b bit for bit): copy c runs lim[c] steps and accumulates
+// sum_N z[N * (max_steps + 3) + c] v_N; the normalised result goes to evecs[instance].
+struct BatchRebuild {
+    const double *z = nullptr;    // host, step-major, nc per row
+    const int *lim = nullptr;     // host, per copy (0: not rebuilt)
+    void *const *evecs = nullptr; // per instance, device vectors of the instance's length
+};
+
 // The batched solve; ha, hb (optional) get the a and b rows of every copy, step-major (nc per row,
-// max_steps + 3 rows; rows past a copy's last step are not set).
+// max_steps + 3 rows; rows past a copy's last step are not set).  rb: the rebuild run instead
+// (evals, nsteps, status untouched).
 static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
-                     int *nsteps, int *status, void *stream, std::vector<double> *ha, std::vector<double> *hb)
+                     int *nsteps, int *status, void *stream, std::vector<double> *ha, std::vector<double> *hb,
+                     const BatchRebuild *rb = nullptr)
 {
     Batch *B = reinterpret_cast<Batch *>(bb);
-    if (!B || !evals) return fail(ED_EINVAL, "ed_batch_lowest: bad arguments");
+    if (!B || (!evals && !rb)) return fail(ED_EINVAL, "ed_batch_lowest: bad arguments");
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
     const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
     const int every = 10;
@@ -779,6 +820,8 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
     CopyCheck *cc = nullptr;
     double *dpart = nullptr; // start-vector norm partials + result
     int64_t npart = 0;
+    double *zdev = nullptr, *acc = nullptr; // rebuild: Ritz coordinates, accumulated vectors
+    int *ldev = nullptr;
     int *pact = nullptr; // pinned: active flags, step counter, copies still running
     // graphs of kGraphChunks chunks (and of one chunk, for the tail) per buffer rotation phase
     constexpr int kGraphChunks = 5;
@@ -814,6 +857,9 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
         dfree(ws);
         dfree(wsf);
         dfree(cc);
+        dfree(zdev);
+        dfree(acc);
+        dfree(ldev);
         if (pact) (void)hipHostFree(pact);
         (void)hipStreamDestroy(st);
         (void)hipEventDestroy(ev);
@@ -836,6 +882,18 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
             (s = dmalloc((void **)&wsf, (size_t)nc * ldz)) ||
             (s = dmalloc((void **)&cc, sizeof(CopyCheck) * (size_t)nc)))
             break;
+        if (rb) {
+            if ((s = dmalloc((void **)&zdev, sizeof(double) * (size_t)nc * ld)) ||
+                (s = dmalloc((void **)&ldev, sizeof(int) * (size_t)nc)) ||
+                (s = dmalloc((void **)&acc, sizeof(double) * (size_t)B->total_real)))
+                break;
+            if (hipMemcpyAsync(zdev, rb->z, sizeof(double) * (size_t)nc * ld, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(ldev, rb->lim, sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemsetAsync(acc, 0, sizeof(double) * (size_t)B->total_real, st) != hipSuccess) {
+                s = fail(ED_EHIP, "ed_batch_lowest: rebuild upload failed");
+                break;
+            }
+        }
         if (hipHostMalloc((void **)&pact, sizeof(int) * (size_t)(nc + 3 + B->ng), hipHostMallocDefault) != hipSuccess) {
             s = fail(ED_ENOMEM, "ed_batch_lowest: pinned staging allocation failed");
             break;
@@ -877,7 +935,7 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
         int *pgact = pact + nc + 3; // running copies per group
         for (int q = 0; q < B->ng; ++q) pgact[q] = 0;
         for (int c = 0; c < nc; ++c) {
-            pact[c] = B->inst_of[c] >= 0 ? 1 : 0;
+            pact[c] = B->inst_of[c] >= 0 && (!rb || rb->lim[c] > 0) ? 1 : 0;
             running += pact[c];
             pgact[B->cgroup[c]] += pact[c];
         }
@@ -902,11 +960,11 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
                 double *Y = buf[(ph + k + 1) % 3];
                 const bool first = (n0 + k) == 0;
                 int r = B->cvec ? (B->ctab ? enqueue_step<cplx, cplx>(*B, X, XP, Y, partial, active, gact, adev, bdev,
-                                                                     nc, dstep, first, st)
+                                                                     nc, dstep, first, zdev, ldev, acc, st)
                                            : enqueue_step<cplx, double>(*B, X, XP, Y, partial, active, gact, adev,
-                                                                       bdev, nc, dstep, first, st))
+                                                                       bdev, nc, dstep, first, zdev, ldev, acc, st))
                                 : enqueue_step<double, double>(*B, X, XP, Y, partial, active, gact, adev, bdev, nc,
-                                                               dstep, first, st);
+                                                               dstep, first, zdev, ldev, acc, st);
                 if (r) return r;
             }
             return ED_OK;
@@ -923,7 +981,11 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
                 if (int r = enqueue(phase, N, chunk)) return r;
                 ED_HIP(hipEventRecord(ev_chunk, st));
                 ED_HIP(hipStreamWaitEvent(side, ev_chunk, 0));
-                if (chk_lds <= kCheckLdsMax)
+                if (rb)
+                    hipLaunchKernelGGL(k_g_limit, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, side, nc,
+                                       (const int *)dchk, chunk, active, (const int *)B->dcgroup, gact,
+                                       (const int *)ldev, dremaining);
+                else if (chk_lds <= kCheckLdsMax)
                     hipLaunchKernelGGL(k_g_check<true>, dim3((unsigned)nc), dim3(64), chk_lds, side, nc, nc,
                                        (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
                                        (const int *)B->dcgroup, gact, tol, maxN, cc, ws, wsf, ldz, dremaining);
@@ -991,6 +1053,20 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
             if (pact[nc + slot] <= 0) stop = true;
         }
         if (s) break;
+        if (rb) {
+            // copy -> instance vector, normalised
+            for (int i = 0; i < B->n && !s; ++i) {
+                const int c = B->copy_of[i];
+                if (rb->lim[c] <= 0) continue;
+                const GGroup &g = B->host[(size_t)B->cgroup[c]];
+                hipLaunchKernelGGL(k_g_gather, dim3((unsigned)((g.nelem + 255) / 256)), dim3(256), 0, st, g.nelem, g.G,
+                                   c - g.cbase, NC, (const double *)(acc + g.voff), (double *)rb->evecs[i]);
+                if (hipGetLastError() != hipSuccess) s = fail(ED_EHIP, "ed_batch_lowest: gather failed");
+                else s = ed_normalize(g.nelem, B->cvec ? 1 : 0, rb->evecs[i], nullptr, st);
+            }
+            if (!s && hipStreamSynchronize(st) != hipSuccess) s = fail(ED_EHIP, "ed_batch_lowest: rebuild failed");
+            break;
+        }
         std::vector<CopyCheck> hc((size_t)nc);
         if (hipMemcpyAsync(hc.data(), cc, sizeof(CopyCheck) * (size_t)nc, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess) {
@@ -1026,8 +1102,37 @@ extern "C" int ed_batch_lowest(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t 
     return batch_run(bb, cfg, seed, use_graph, evals, nsteps, status, stream, nullptr, nullptr);
 }
 
+// The first T_n (n a multiple of ten, or the whole T_M after an invariant subspace) whose lowest
+// Ritz pair passes ed_lanczos_eigen's test |b_n s[n-1]| < tol anorm, |s[0]| > tol anorm: the
+// step count n at which a single-instance run stops, its eigenvector z and eigenvalue; 0 if none.
+static int ground_state_steps(const double *a, const double *b, int M, bool exact, double tol, std::vector<double> &z,
+                              double *lam)
+{
+    std::vector<double> ws, zz;
+    std::vector<char> sw;
+    double anorm = 0.0;
+    int done = 0, n = 0;
+    while (n < M) {
+        n = std::min(n + 10, M);
+        for (int k = done; k < n; ++k)
+            anorm = k == 0 ? std::fabs(a[0] + b[1]) : std::fmax(anorm, b[k] + std::fabs(a[k] + b[k + 1]));
+        done = n;
+        const double l = tri_bisect_kth(n, a, b + 1, 0);
+        ws.assign(5 * (size_t)n, 0.0);
+        zz.assign((size_t)n, 0.0);
+        sw.assign((size_t)n, 0);
+        tri_inverse_iteration(n, a, b + 1, l, zz.data(), ws.data(), sw.data());
+        if ((exact && n == M) || (std::fabs(b[n] * zz[(size_t)n - 1]) < tol * anorm && std::fabs(zz[0]) > tol * anorm)) {
+            z = zz;
+            *lam = l;
+            return n;
+        }
+    }
+    return 0;
+}
+
 extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph,
-                                 double *evals, int *nsteps, int *status, void *stream)
+                                 double *evals, int *nsteps, int *status, void *const *d_evecs, void *stream)
 {
     Batch *B = reinterpret_cast<Batch *>(bb);
     if (!B || !evals || nev < 1 || nev > 64) return fail(ED_EINVAL, "ed_batch_lowest_k: bad arguments (nev 1..64)");
@@ -1042,6 +1147,8 @@ extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev
     ED_TRY(batch_run(bb, &run, seed, use_graph, ev.data(), ns.data(), st.data(), stream, &ha, &hb));
     const int nc = B->ncopies;
     std::vector<double> a, e;
+    std::vector<int> lim((size_t)nc, 0);             // rebuild steps per copy
+    std::vector<std::vector<double>> zrows((size_t)B->n); // Ritz coordinates per instance
     for (int i = 0; i < B->n; ++i) {
         const int c = B->copy_of[i];
         const int M = ns[(size_t)i]; // steps kept: a[0..M-1], b[0..M]
@@ -1062,8 +1169,39 @@ extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev
         for (int k = f; k < nev; ++k) out[k] = std::numeric_limits<double>::quiet_NaN();
         if (nsteps) nsteps[i] = M;
         if (status) status[i] = conv ? ED_OK : ED_ENOCONV;
+        if (!d_evecs) continue;
+        // the ground state of an instance whose nev pass: from T_n at the step where the lowest
+        // pair first passes, whose value must be the filter's lowest
+        double lam = 0.0;
+        std::vector<double> z;
+        const int n0 = conv ? ground_state_steps(a.data(), b.data(), M, exact, tol, z, &lam) : 0;
+        if (n0 > 0 && !(std::fabs(lam - out[0]) <= 1e-8 * std::fmax(1.0, std::fabs(out[0])))) {
+            if (status) status[i] = ED_ENOCONV;
+            continue;
+        }
+        if (n0 == 0) {
+            if (status) status[i] = ED_ENOCONV;
+            continue;
+        }
+        lim[(size_t)c] = n0;
+        zrows[(size_t)i] = std::move(z);
     }
-    return ED_OK;
+    if (!d_evecs) return ED_OK;
+    int maxlim = 0;
+    for (int l : lim) maxlim = std::max(maxlim, l);
+    if (maxlim == 0) return ED_OK;
+    const int ld = maxlim + 3;
+    std::vector<double> zh((size_t)nc * ld, 0.0);
+    for (int i = 0; i < B->n; ++i) {
+        const int c = B->copy_of[i];
+        for (int k = 0; k < lim[(size_t)c]; ++k) zh[(size_t)k * nc + c] = zrows[(size_t)i][(size_t)k];
+    }
+    BatchRebuild rb;
+    rb.z = zh.data();
+    rb.lim = lim.data();
+    rb.evecs = d_evecs;
+    const ed_lanczos_cfg rcfg{maxlim, 1e-300, 10, 0};
+    return batch_run(bb, &rcfg, seed, use_graph, nullptr, nullptr, nullptr, stream, nullptr, nullptr, &rb);
 }
 
 extern "C" int ed_batch_set_shape(ed_batch bb, int dn_rows, int dn_panels, int up_rows, int up_panels)

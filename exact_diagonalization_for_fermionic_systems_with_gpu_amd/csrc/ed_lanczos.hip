@@ -724,7 +724,10 @@ static int lanczos_eigen_impl(const Hubbard *h, int complex_vectors, const ed_la
             // ghosts elsewhere, so a value the start vector barely touches is kept.
             bool rest = true;
             const bool exact = M < N || !(b[M + 1] > 0.0);
-            if (nev > 1) {
+            const bool low_ok = std::fabs(b[M + 1] * z[M]) < tol * anorm && std::fabs(z[0]) > tol * anorm;
+            // the filter (two tridiagonal eigenvalue solves) runs once the lowest value passes, and
+            // at the last step for the values returned
+            if (nev > 1 && (exact || low_ok || N == maxN - 1)) {
                 std::vector<double> out((size_t)nev, std::numeric_limits<double>::quiet_NaN());
                 const int f = cw_lowest(M + 1, a.data(), b.data() + 1, exact ? 0.0 : b[M + 1], nev, tol, anorm,
                                         true, out.data(), &rest);
@@ -734,7 +737,7 @@ static int lanczos_eigen_impl(const Hubbard *h, int complex_vectors, const ed_la
             if (exact) {
                 status = ED_OK; // exact invariant subspace: T_{M+1} eigenvalues are exact
                 stop = true;
-            } else if (rest && std::fabs(b[M + 1] * z[M]) < tol * anorm && std::fabs(z[0]) > tol * anorm) {
+            } else if (rest && low_ok) {
                 status = ED_OK;
                 stop = true;
             }

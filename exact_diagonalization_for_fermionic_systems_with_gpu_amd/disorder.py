@@ -94,10 +94,10 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
                  tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
                  max_steps=2000, batch_steps=400) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched launch, each worker
-    thread on its own stream.  method "batched": the nev lowest of every realisation from one
-    batched run of batch_steps steps (ed_batch_lowest_k), then per realisation its ground state by
-    a plain Lanczos run with the Ritz vector rebuilt (ed_lanczos_eigen, test tolerance tol); a
-    realisation whose batched values fail the test falls back to "lanczos".  "lanczos": after a
+    thread on its own stream.  method "batched": the nev lowest of every realisation and its
+    ground state from one batched run of batch_steps steps and a batched rebuild of the Ritz
+    vectors (ed_batch_lowest_k, test tolerance tol); a realisation whose batched values fail the
+    test falls back to "lanczos".  "lanczos": after a
     batched E0 launch, ed_lanczos_lowest per realisation (at most max_steps steps; tol is its
     test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart": the
     thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).
@@ -110,8 +110,10 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
     t0 = time.perf_counter()
     hs = [build_realisation(eps[r], L_sites, n_up, n_dn, U, t) for r in range(n_real)]
     tb = time.perf_counter()
+    psib = None
     if method == "batched":
-        evb, _, stb = BatchedLanczos(hs, False).lowest_k(nev, steps=batch_steps, tol=tol)
+        psib = [torch.empty(h.dim, dtype=torch.float64, device="cuda") for h in hs]
+        evb, _, stb = BatchedLanczos(hs, False).lowest_k(nev, steps=batch_steps, tol=tol, evecs=psib)
         e0b = evb[:, 0].copy()
     else:
         e0b, _, _ = BatchedLanczos(hs, False).lowest(tol=1e-12)
@@ -137,19 +139,20 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
                 evals[r] = es.eigenvalues()
                 psi = es.eigenvectors()[0]
                 del es
+            elif method == "batched" and stb[r] == 0:
+                evals[r] = evb[r]
+                psi = psib[r]  # rebuilt by the batched call
+                psib[r] = None
             else:
+                if method == "batched":
+                    nonlocal fallbacks
+                    fallbacks += 1
+                    psib[r] = None
                 f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
                 psi = torch.empty_like(f)
                 lz = Lanczos(H.dimUP, H.dimDN, max_steps=max_steps, tol=tol)
                 lz.random_vector(f)
-                if method == "batched" and stb[r] == 0:
-                    evals[r] = evb[r]
-                    lz.eigen(H, f, psi) # the ground state only
-                else:
-                    if method == "batched":
-                        nonlocal fallbacks
-                        fallbacks += 1
-                    evals[r] = lz.lowest(H, f, nev, psi)
+                evals[r] = lz.lowest(H, f, nev, psi)
                 del f
             if with_sref:
                 S[r] = half_chain_entropy(H, n_up, n_dn, psi, L_sites // 2, True)

@@ -394,11 +394,16 @@ int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int us
  * sqrt(tol) anorm: that is an eigenvalue of H the start vector barely touches, an overlap of
  * 1e-7 is enough to put it there) and each of the nev lowest kept values must pass
  * ed_lanczos_eigen's residual test |b s_N| < tol * anorm.  evals[ninst * nev] (instance-major,
- * ascending, NaN past the values
- * found), nsteps[ninst], status[ninst]: ED_OK, or ED_ENOCONV when the test is not met.
- * Synchronises. */
+ * ascending, NaN past the values found), nsteps[ninst], status[ninst]: ED_OK, or ED_ENOCONV when
+ * the test is not met.  d_evecs (NULL: none) holds ninst device vectors of the instances'
+ * lengths (FP64 or complex128 as the batch): each instance with status ED_OK gets its normalised
+ * ground state, the Ritz vector of T_n at the first check (every ten steps) where the lowest pair
+ * passes the test -- where ed_lanczos_eigen would stop -- by a second batched run of the same
+ * recurrence (the same a and b bit for bit) that accumulates sum z_k v_k, each instance for its
+ * own n steps; an instance whose T_n value is not the filter's lowest (to 1e-8 relative) gets
+ * ED_ENOCONV and no vector.  Synchronises. */
 int ed_batch_lowest_k(ed_batch b, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph, double *evals,
-                      int *nsteps, int *status, void *stream);
+                      int *nsteps, int *status, void *const *d_evecs, void *stream);
 /* Panel shape of the batch kernels: DN rows per panel (4 or 8) x DN panels per workgroup (1-16),
  * UP rows per panel (1 or 4) x UP panels per workgroup (1-64); default 4 x 4, 1 x 4.  The
  * eigenvalues do not depend on it, bit for bit (same per-element sums; the dot partials are per

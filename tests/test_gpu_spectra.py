@@ -154,10 +154,17 @@ def test_disorder_gradient_vs_finite_differences():
 
 
 @pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
-def test_chain14_against_fixture():
+@pytest.mark.parametrize("method", ["batched", "lanczos"])
+def test_chain14_against_fixture(method):
     gold = json.load(open(GOLD))
-    print("disorder chain14 7/7 x2 vs fixture", flush=True)
-    res = DS.run_ensemble(n_real=2, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=20, tol=1e-14)
+    print(f"disorder chain14 7/7 x2 vs fixture ({method})", flush=True)
+    # S_ref to 1e-8 needs the ground state to ~1e-10: the Lanczos test at 1e-14 |T|; the batched
+    # run's test at 1e-12 |T| (its 500 steps meet it)
+    tol = 1e-12 if method == "batched" else 1e-14
+    res = DS.run_ensemble(n_real=2, L_sites=14, n_up=7, n_dn=7, W=8.0, nev=4, ncv=20, tol=tol, method=method,
+                          batch_steps=500)
+    if method == "batched":
+        assert res.fallbacks == 0
     for rec in gold["realisations"]:
         r = rec["r"]
         assert np.allclose(res.eps[r], rec["eps"], rtol=0, atol=0)
@@ -268,11 +275,13 @@ def test_entropy_vn_batch_matches_per_state():
 def test_batched_lowest_k_against_per_instance():
     """ed_batch_lowest_k on the 14-site chain (D = 1.18e7, one group of 6 realisations sharing the
     operator structure): the 4 lowest of every realisation against ed_lanczos_lowest on each, and
-    the batched E0 against the plain Lanczos E0."""
+    the batched ground states (the rebuild run) against the plain Lanczos run's Ritz vector from
+    the same start vector (ed_lanczos_eigen, same test), with their residuals."""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd.batched import BatchedLanczos
-    print("batched lowest_k chain14 x6", flush=True)
+    print("batched lowest_k chain14 x6 with ground states", flush=True)
     hs = [DS.build_realisation(DS.realisation_eps(r, 14, 8.0), 14, 7, 7) for r in range(6)]
-    ev, ns, st = BatchedLanczos(hs, False).lowest_k(4, steps=500, tol=1e-10)
+    psib = [torch.full((H.dim,), float("nan"), dtype=torch.float64, device="cuda") for H in hs]
+    ev, ns, st = BatchedLanczos(hs, False).lowest_k(4, steps=500, tol=1e-10, evecs=psib)
     # a realisation whose lowest levels lie close together may need more steps (ED_ENOCONV, the
     # ensemble then solves it alone); the others must agree with the per-instance solver
     assert (ns == 500).all() and (st == 0).sum() >= 4, (st, ns)
@@ -284,6 +293,15 @@ def test_batched_lowest_k_against_per_instance():
         lz.random_vector(f)
         ref = lz.lowest(H, f, 4)
         assert np.allclose(ev[r], ref, rtol=1e-10, atol=0), (r, ev[r], ref)
+        psi = torch.empty_like(f)
+        lz.random_vector(f)
+        e0 = lz.eigen(H, f, psi)
+        pb = psib[r]
+        assert abs(float(torch.linalg.vector_norm(pb)) - 1.0) < 1e-12
+        if ev[r][1] - ev[r][0] > 1e-6:  # within a near-degenerate pair the vector is any mixture
+            assert abs(abs(float(torch.dot(pb, psi))) - 1.0) < 1e-9, (r, float(torch.dot(pb, psi)))
+        res = float(torch.linalg.vector_norm(H.apply(pb) - ev[r][0] * pb))
+        assert res < 1e-7 and abs(e0 - ev[r][0]) < 1e-10 * abs(e0), (r, res, e0, ev[r][0])
 
 
 def test_lanczos_lowest_near_degenerate_pairs_chain14():
