@@ -316,9 +316,9 @@ def batched_drude_line(with_cpu=False):
 
 def ensemble_line(n_real, ncv=24, workers=1, method="batched"):
     """14-site open chain, 7/7, U = 4, W = 8.  Method "batched": the 4 lowest eigenvalues of all
-    realisations from one batched Lanczos launch of 400 steps and their ground states from a
-    batched rebuild (ed_batch_lowest_k), a realisation whose values miss the test solved alone as
-    with "lanczos"; "lanczos": E0 of all in one batched launch, then per realisation one Lanczos
+    realisations from one batched Lanczos launch (each realisation stops once its 4 lowest pass
+    the test, at most 600 steps) and their ground states from a batched rebuild
+    (ed_batch_lowest_k), a realisation whose values miss the test solved alone as with "lanczos"; "lanczos": E0 of all in one batched launch, then per realisation one Lanczos
     run with selective orthogonalisation, ground state refined by a plain run; "restart": the
     thick-restart solver with ncv = 24 basis vectors.  Then dE0/d eps_i per realisation (`workers`
     at a time on separate streams) and the 7|7 entropy S_vN of every ground state, each block size

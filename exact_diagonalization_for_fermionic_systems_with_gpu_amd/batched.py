@@ -70,10 +70,10 @@ class BatchedLanczos:
         return np.array(ev[:]), np.array(ns[:]), np.array(st[:])
 
 
-    def lowest_k(self, nev: int, steps=400, tol=None, use_graph=True, seed=235, evecs=None):
-        """The nev lowest eigenvalues of every instance from one batched run of `steps` steps
-        (ed_batch_lowest_k: Cullum-Willoughby filter of each tridiagonal, ed_lanczos_eigen's
-        residual test on each kept value).  evecs (optional): one device tensor per instance
+    def lowest_k(self, nev: int, steps=600, tol=None, use_graph=True, seed=235, evecs=None):
+        """The nev lowest eigenvalues of every instance from one batched run of at most `steps`
+        steps, each instance stopped once its nev lowest pass (ed_batch_lowest_k: Cullum-Willoughby
+        filter of each tridiagonal, ed_lanczos_eigen's residual test on each kept value).  evecs (optional): one device tensor per instance
         (its dim, float64 or complex128 as the batch) that gets the instance's normalised ground
         state when its status is 0 (a second batched run rebuilds the Ritz vectors).  Returns
         (evals [n, nev], nsteps [n], status [n])."""

@@ -732,6 +732,146 @@ __global__ __launch_bounds__(64) void k_g_check(int nc, int ld, const double *__
     }
 }
 
+// The k-th smallest eigenvalue of T (d[n], e[n-1]) in the bracket (lo, hi] by 64-point
+// multisection of the Sturm count, down to adjacent doubles; uniform over the wavefront
+__device__ inline double wave_kth(int n, const double *d, const double *e, int k, double lo, double hi, int lane)
+{
+#pragma clang fp contract(off)
+    for (int round = 0; round < 64; ++round) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break; // adjacent doubles (uniform)
+        const double w = hi - lo;
+        const double xj = lo + w * ((double)(lane + 1) / 65.0);
+        const unsigned long long above = __ballot(tri_sturm_below(n, d, e, xj) > k);
+        if (above == 0ull) {
+            lo = lo + w * (64.0 / 65.0);
+        } else {
+            const int j = __ffsll((long long)above) - 1;
+            const double xh = lo + w * ((double)(j + 1) / 65.0), xl = j > 0 ? lo + w * ((double)j / 65.0) : lo;
+            lo = xl;
+            hi = xh;
+        }
+    }
+    return 0.5 * (lo + hi);
+}
+
+// The nev-lowest test of every active copy after a chunk (ed_batch_lowest_k's run): the filter of
+// cw_lowest (semi_orth off) on the copy's T_n, with eigenvalues by multisection instead of QL:
+// copies within 1e-12 anorm merged; a simple value on an eigenvalue of T^ (T without its first
+// row and column: Sturm counts of T^ at v -+ 1e-12 anorm) holds the test back when it has
+// converged (until its copy forms) or lies more than 1e-3 anorm above the converged values kept
+// below it, and is dropped as a ghost in transit otherwise; every other simple one must pass
+// |b_n s[n-1]| < tol anorm (inverse iteration on lane 0).  A copy whose nev lowest all pass, or
+// whose recurrence ends in an invariant subspace, or that has run max_steps, is switched off.
+// Only when to stop is decided here: the host filters the final tridiagonals again.  Arrays as in
+// k_g_check.
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_g_check_k(int nc, int ld, const double *__restrict__ adev,
+                                                  const double *__restrict__ bdev, const int *__restrict__ dchk,
+                                                  int chunk, int *active, const int *__restrict__ cgroup, int *gact,
+                                                  double tol, int max_steps, int nev, CopyCheck *cc, double *ws,
+                                                  char *wsf, int ldz, int *remaining)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double s_chk[];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    if (c >= nc || !active[c]) return; // uniform over the wavefront
+    const int Ne = *dchk + chunk, Nl = Ne - 1;
+    double *a = LDS ? s_chk : ws + (size_t)c * 7 * ldz, *b = a + ldz, *w5 = b + ldz;
+    char *sw = LDS ? reinterpret_cast<char *>(s_chk + 7 * (size_t)ldz) : wsf + (size_t)c * ldz;
+    for (int k = lane; k <= Nl; k += 64) a[k] = adev[(int64_t)k * ld + c];
+    for (int k = lane; k <= Ne; k += 64) b[k] = bdev[(int64_t)k * ld + c];
+    __syncthreads();
+    CopyCheck st = cc[c];
+    int M = Nl;
+    for (int k = st.processed; k <= Nl; ++k)
+        if (!(b[k + 1] > 0.0) || !isfinite(a[k])) {
+            M = k;
+            break;
+        }
+    for (int k = st.processed; k <= M; ++k) {
+        if (k == 0) st.anorm = fabs(a[0] + b[1]);
+        else st.anorm = fmax(st.anorm, b[k] + fabs(a[k] + b[k + 1]));
+    }
+    st.processed = M + 1;
+    const int n = M + 1;
+    const double *d = a, *e = b + 1;
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
+    }
+    const double pad = 1e-12 * (hi - lo + 1.0);
+    lo -= pad;
+    hi += pad;
+    const bool brk = M < Nl || !(b[M + 1] > 0.0);
+    const double bres = brk ? 0.0 : b[M + 1];
+    const double tm = 1e-12 * st.anorm;
+    int found = 0;
+    bool ok = true;
+    int j = 0;
+    double thj = wave_kth(n, d, e, 0, lo, hi, lane), genuine_hi = -INFINITY;
+    const double tw = 1e-3 * st.anorm;
+    const double th0 = thj;
+    while (j < n && found < nev) {
+        int r = j + 1;
+        double last = thj; // the cluster's largest member
+        double thr = r < n ? wave_kth(n, d, e, r, lo, hi, lane) : INFINITY;
+        while (r < n && thr - last <= tm) {
+            last = thr;
+            ++r;
+            thr = r < n ? wave_kth(n, d, e, r, lo, hi, lane) : INFINITY;
+        }
+        bool keep = true, genuine = true;
+        if (r - j == 1) {
+            int pass = 0;
+            if (lane == 0) {
+                double *z = w5 + 4 * (size_t)n;
+                tri_inverse_iteration(n, d, e, thj, z, w5, sw);
+                pass = fabs(bres * z[n - 1]) < tol * st.anorm ? 1 : 0;
+            }
+            const bool conv1 = __shfl(pass, 0) != 0;
+            bool in_that = false;
+            if (n > 1) {
+                const int cnt = lane < 2 ? tri_sturm_below(n - 1, a + 1, b + 2, lane == 0 ? thj - tm : thj + tm) : 0;
+                in_that = __shfl(cnt, 1) > __shfl(cnt, 0);
+            }
+            bool cv = conv1;
+            if (in_that) {
+                if (cv) cv = false; // held back until its copy forms
+                else keep = thj - genuine_hi > tw;
+            }
+            if (keep) ok = ok && cv;
+            genuine = keep && cv;
+        }
+        if (keep) ++found;
+        if (genuine) genuine_hi = last;
+        j = r;
+        thj = thr;
+    }
+    if (lane != 0) return;
+    const bool pass = ok && found == nev;
+    st.ev = th0;
+    st.last = M;
+    bool stop = false, conv = false;
+    if (brk) {
+        stop = true;
+        conv = pass;
+    } else if (pass) {
+        stop = conv = true;
+    } else if (Ne >= max_steps) {
+        stop = true;
+    }
+    if (conv) st.conv = 1;
+    cc[c] = st;
+    if (stop) {
+        active[c] = 0;
+        atomicSub(&gact[cgroup[c]], 1);
+        atomicSub(remaining, 1);
+    }
+}
+
 // the checks' own step count (the step kernels' counter runs ahead while a check runs beside them)
 __global__ void k_g_chk_bump(int *dchk, int chunk) { dchk[0] += chunk; }
 
@@ -785,7 +925,7 @@ struct BatchRebuild {
 // (evals, nsteps, status untouched).
 static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                      int *nsteps, int *status, void *stream, std::vector<double> *ha, std::vector<double> *hb,
-                     const BatchRebuild *rb = nullptr)
+                     const BatchRebuild *rb = nullptr, int nev = 1)
 {
     Batch *B = reinterpret_cast<Batch *>(bb);
     if (!B || (!evals && !rb)) return fail(ED_EINVAL, "ed_batch_lowest: bad arguments");
@@ -985,6 +1125,14 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
                     hipLaunchKernelGGL(k_g_limit, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, side, nc,
                                        (const int *)dchk, chunk, active, (const int *)B->dcgroup, gact,
                                        (const int *)ldev, dremaining);
+                else if (nev > 1 && chk_lds <= kCheckLdsMax)
+                    hipLaunchKernelGGL(k_g_check_k<true>, dim3((unsigned)nc), dim3(64), chk_lds, side, nc, nc,
+                                       (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
+                                       (const int *)B->dcgroup, gact, tol, maxN, nev, cc, ws, wsf, ldz, dremaining);
+                else if (nev > 1)
+                    hipLaunchKernelGGL(k_g_check_k<false>, dim3((unsigned)nc), dim3(64), 0, side, nc, nc,
+                                       (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
+                                       (const int *)B->dcgroup, gact, tol, maxN, nev, cc, ws, wsf, ldz, dremaining);
                 else if (chk_lds <= kCheckLdsMax)
                     hipLaunchKernelGGL(k_g_check<true>, dim3((unsigned)nc), dim3(64), chk_lds, side, nc, nc,
                                        (const double *)adev, (const double *)bdev, (const int *)dchk, chunk, active,
@@ -1139,12 +1287,12 @@ extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev
     if (cfg && cfg->reorth) return fail(ED_EUNSUPPORTED, "ed_batch_lowest_k: no reorthogonalisation in the batch");
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
     const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
-    // every copy runs max_steps steps (a test no copy passes stops none of them; a breakdown, an
-    // invariant subspace, still ends a copy early)
-    const ed_lanczos_cfg run{maxN, 1e-300, 10, 0};
+    // each copy runs until its nev lowest pass the device test (k_g_check_k), its recurrence ends
+    // in an invariant subspace, or max_steps
+    const ed_lanczos_cfg run{maxN, tol, 10, 0};
     std::vector<double> ev((size_t)B->n), ha, hb;
     std::vector<int> ns((size_t)B->n), st((size_t)B->n);
-    ED_TRY(batch_run(bb, &run, seed, use_graph, ev.data(), ns.data(), st.data(), stream, &ha, &hb));
+    ED_TRY(batch_run(bb, &run, seed, use_graph, ev.data(), ns.data(), st.data(), stream, &ha, &hb, nullptr, nev));
     const int nc = B->ncopies;
     std::vector<double> a, e;
     std::vector<int> lim((size_t)nc, 0);             // rebuild steps per copy
@@ -1161,7 +1309,7 @@ extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev
         for (int k = 0; k < M; ++k)
             anorm = k == 0 ? std::fabs(a[0] + b[1]) : std::fmax(anorm, b[k] + std::fabs(a[k] + b[k + 1]));
         for (int k = 0; k + 1 < M; ++k) e[(size_t)k] = b[(size_t)k + 1];
-        const bool exact = st[(size_t)i] == ED_OK && M < maxN; // the recurrence ended in an invariant subspace
+        const bool exact = !(b[(size_t)M] > 0.0); // the recurrence ended in an invariant subspace
         bool conv = false;
         double *out = evals + (size_t)i * nev;
         const int f = ed::cw_lowest(M, a.data(), e.data(), exact ? 0.0 : b[(size_t)M], nev, tol, anorm, false, out,

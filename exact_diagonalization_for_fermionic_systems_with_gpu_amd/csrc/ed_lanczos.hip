@@ -221,22 +221,26 @@ static int lanczos_step_vfree(const Hubbard &h, int64_t nreal, double *A, double
 
 // The nev lowest eigenvalues of a Lanczos tridiagonal T (a[0..M-1], couplings e[0..M-2]), after
 // the test of J. Cullum and R. A. Willoughby ("Lanczos Algorithms for Large Symmetric Eigenvalue
-// Computations", vol. 1, ch. 4): an eigenvalue of T that occurs more than once (within tol_m) is
-// a converged eigenvalue of H (a copy appears only once the eigenvalue has converged) and its
-// extra copies are dropped; a simple one that is not also an eigenvalue of T^ (T with its first
-// row and column removed) stands for an eigenvalue of H and needs |bres s[M-1]| < tol * anorm
-// (s its unit eigenvector of T, bres the coupling after the last step), the residual test of
-// ed_lanczos_eigen.  A simple eigenvalue of T that is also one of T^ has a tiny first component
-// s[0]: either a ghost (spurious) or an eigenvalue of H that the start vector barely touches
-// (an overlap of 1e-7 puts it within 1e-12 |T| of T^).  It counts as the latter when it passes
-// the residual test with no other eigenvalue of T within sqrt(tol) |T| (a converged simple Ritz
-// value is within its residual of an eigenvalue of H, Paige), otherwise as a ghost; an
-// unconverged one is dropped after a recurrence without reorthogonalisation (a ghost in transit,
-// new ones keep forming) and kept, holding the test back, after one with semi_orth set
-// (selective orthogonalisation forms no ghosts away from converged values).  (Within a cluster of
-// copies inverse iteration returns some mixture of their vectors, whose last component says
-// nothing about convergence.)  Returns the number found (at most nev); *conv = every one found
-// passes.
+// Computations", vol. 1, ch. 4).  Eigenvalues of T within tol_m of each other form a cluster: two
+// or more copies are a converged eigenvalue of H (a copy appears only once the eigenvalue has
+// converged) and count once.  A simple eigenvalue that is not also an eigenvalue of T^ (T with its
+// first row and column removed) stands for an eigenvalue of H and must pass |bres s[M-1]| <
+// tol * anorm (s its unit eigenvector of T, bres the coupling after the last step), the residual
+// test of ed_lanczos_eigen.  A simple eigenvalue that is also one of T^ has a tiny first component
+// s[0]: a ghost, or an eigenvalue of H that the start vector barely touches (an overlap of 1e-7
+// puts it within 1e-12 |T| of T^); in a disordered 14-site chain a ground state 1e-7 from the
+// start vector converged after its excited levels.
+//   * after a recurrence without reorthogonalisation: converged, it holds the test back until a
+//     copy of it forms and merges (or max_steps ends the run); not converged, it is a ghost in
+//     transit and dropped when it lies within 1e-3 |T| above a converged value kept below it (a
+//     ghost converges from above onto its twin), and otherwise holds the test back: it may be an
+//     eigenvalue coming in, and T's lowest eigenvalue cannot be a ghost at all;
+//   * after selective orthogonalisation (semi_orth), which forms ghosts only around eigenvalues
+//     closer together than its accuracy: converged, a ghost when the converged value just below
+//     it lies within sqrt(tol) |T|, otherwise kept; not converged, it holds the test back.
+// Left open: an eigenvalue the start vector barely touches that lies within 1e-3 |T| above
+// another one (1e-7 apart in a test) is taken for a ghost until it converges.  Returns the number
+// found (at most nev); *conv = every one found passes.
 int cw_lowest(int M, const double *a, const double *e, double bres, int nev, double tol, double anorm,
               bool semi_orth, double *out, bool *conv)
 {
@@ -253,27 +257,38 @@ int cw_lowest(int M, const double *a, const double *e, double bres, int nev, dou
     std::vector<char> sw((size_t)M);
     int found = 0;
     bool ok = true;
+    // the previous cluster: its largest member, and whether it stands for a converged eigenvalue
+    double prev_hi = -std::numeric_limits<double>::infinity();
+    bool prev_genuine = false;
+    // the largest converged value kept so far, and the window above it in which an unconverged
+    // value on T^ is taken for a ghost approaching it
+    double genuine_hi = -std::numeric_limits<double>::infinity();
+    const double tw = 1e-3 * anorm;
     for (int j = 0; j < M && found < nev;) {
         int r = j + 1;
         while (r < M && th[r] - th[r - 1] <= tm) ++r;
         const double v = th[j];
-        bool keep = true;
+        bool keep = true, genuine = true;
         if (r - j == 1) {
             tri_inverse_iteration(M, a, e, v, z.data(), ws.data(), sw.data());
-            const bool converged = std::fabs(bres * z[M - 1]) < tol * anorm;
+            bool converged = std::fabs(bres * z[M - 1]) < tol * anorm;
             bool in_that = false;
             if (M > 1) {
                 const double *q = std::lower_bound(thh.data(), thh.data() + (M - 1), v - tm);
                 in_that = q < thh.data() + (M - 1) && *q <= v + tm;
             }
             if (in_that) {
-                const bool isolated = (j == 0 || v - th[j - 1] > tg) && (r >= M || th[r] - v > tg);
-                if (converged) keep = isolated;
-                else keep = semi_orth;
+                if (converged && !semi_orth) converged = false; // held back until its copy forms
+                else if (converged) keep = !(prev_genuine && v - prev_hi <= tg);
+                else keep = semi_orth || v - genuine_hi > tw;
             }
             if (keep) ok = ok && converged;
+            genuine = keep && converged;
         }
         if (keep) out[found++] = v;
+        prev_hi = th[r - 1];
+        prev_genuine = genuine;
+        if (genuine) genuine_hi = th[r - 1];
         j = r;
     }
     *conv = ok && found == nev;

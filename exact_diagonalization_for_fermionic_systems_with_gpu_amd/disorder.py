@@ -92,12 +92,12 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
                  tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
-                 max_steps=2000, batch_steps=400) -> EnsembleResult:
+                 max_steps=2000, batch_steps=600) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched launch, each worker
     thread on its own stream.  method "batched": the nev lowest of every realisation and its
-    ground state from one batched run of batch_steps steps and a batched rebuild of the Ritz
-    vectors (ed_batch_lowest_k, test tolerance tol); a realisation whose batched values fail the
-    test falls back to "lanczos".  "lanczos": after a
+    ground state from one batched run of at most batch_steps steps (each realisation stops once
+    its nev lowest pass) and a batched rebuild of the Ritz vectors (ed_batch_lowest_k, test
+    tolerance tol); a realisation whose batched values fail the test falls back to "lanczos".  "lanczos": after a
     batched E0 launch, ed_lanczos_lowest per realisation (at most max_steps steps; tol is its
     test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart": the
     thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).

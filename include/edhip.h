@@ -282,8 +282,8 @@ int ed_lanczos_eigen(ed_hubbard h, int complex_vectors, const ed_lanczos_cfg *cf
  * which restarts a 2 nev + 1 vector Krylov basis; here one recurrence runs on, its vectors kept
  * semi-orthogonal against the converged Ritz vectors up to the nev-th lowest as with reorth = 2).
  * Every check_every steps the nev lowest eigenvalues of T are taken with copies of converged
- * eigenvalues merged and converging ghost copies dropped (the Cullum-Willoughby test of
- * ed_batch_lowest_k: the purges keep most copies from forming, not those of eigenvalues closer
+ * eigenvalues merged and converging ghost copies dropped (ed_tridiag_lanczos_lowest with
+ * semi_orth = 1: the purges keep most copies from forming, not those of eigenvalues closer
  * together than the accuracy at which a Ritz vector counts as converged; a value the start
  * vector barely touches is kept), and each simple one must pass |b_{N+1} s_k[N]| < tol * anorm
  * (ed_lanczos_eigen's test); the lowest also needs |s_0[0]| > tol * anorm.  evals[nev]
@@ -326,9 +326,12 @@ int ed_tridiag_eigvals(int n, const double *d, const double *e, double *w);
  * ascending; z[n*n] column-major, z[j*n + i] = component i of the unit eigenvector of w[j]. */
 int ed_tridiag_eigen(int n, const double *d, const double *e, double *w, double *z);
 /* The nev lowest eigenvalues of a Lanczos tridiagonal (d[n], e[n-1]) with bres = b_{n+1}, the
- * coupling after the last step, by the filter ed_lanczos_lowest (semi_orth = 1) and
- * ed_batch_lowest_k (semi_orth = 0) apply: copies merged, ghosts dropped, each kept simple value
- * tested |bres s[n-1]| < tol * anorm.  out[nev] ascending (NaN past the values found); *found =
+ * coupling after the last step, by the filter ed_lanczos_lowest (semi_orth = 1: the recurrence
+ * ran with selective orthogonalisation) and ed_batch_lowest_k (semi_orth = 0: without
+ * reorthogonalisation) apply, after Cullum and Willoughby: copies within 1e-12 anorm merged,
+ * ghosts dropped, values on eigenvalues of T without its first row and column (ghosts, or
+ * eigenvalues the start vector barely touches) held back or dropped by the rules of each mode,
+ * each kept simple value tested |bres s[n-1]| < tol * anorm.  out[nev] ascending (NaN past the values found); *found =
  * the number found; *conv = 1 when all nev were found and pass.  No counterpart in the reference
  * (its SymEigsSolver restarts instead of filtering); exported for tests of the filter. */
 int ed_tridiag_lanczos_lowest(int n, const double *d, const double *e, double bres, int nev, double tol,
@@ -386,13 +389,16 @@ int ed_batch_destroy(ed_batch b);
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
 /* The nev lowest eigenvalues of every instance from one batched run (the SymEigsSolver
- * "SmallestAlge" role, SymEigsSolver.cuh:332-359, for many instances at once): every instance
- * runs cfg->max_steps steps (fewer only when its recurrence ends in an invariant subspace), then
- * its tridiagonal is filtered by the Cullum-Willoughby test (copies of converged eigenvalues
- * merged; a simple value that is also an eigenvalue of T without its first row and column is
- * dropped as spurious unless it passes the residual test with no other eigenvalue of T within
- * sqrt(tol) anorm: that is an eigenvalue of H the start vector barely touches, an overlap of
- * 1e-7 is enough to put it there) and each of the nev lowest kept values must pass
+ * "SmallestAlge" role, SymEigsSolver.cuh:332-359, for many instances at once): every ten steps
+ * a device test applies the filter below to each running instance (eigenvalues by Sturm
+ * multisection) and switches off those whose nev lowest pass; an instance runs at most
+ * cfg->max_steps steps (fewer when its recurrence ends in an invariant subspace).  Then its
+ * tridiagonal is filtered on the host by the same test (ed_tridiag_lanczos_lowest with
+ * semi_orth = 0: copies of converged eigenvalues merged; a simple value that is also an
+ * eigenvalue of T without its first row and column -- a ghost, or an eigenvalue the start vector
+ * barely touches, 1e-7 is enough -- holds the test back when it has converged, until its copy
+ * forms, and when it lies more than 1e-3 anorm above the converged values below it, and is
+ * dropped as a ghost in transit otherwise) and each of the nev lowest kept values must pass
  * ed_lanczos_eigen's residual test |b s_N| < tol * anorm.  evals[ninst * nev] (instance-major,
  * ascending, NaN past the values found), nsteps[ninst], status[ninst]: ED_OK, or ED_ENOCONV when
  * the test is not met.  d_evecs (NULL: none) holds ninst device vectors of the instances'

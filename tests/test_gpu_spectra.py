@@ -282,9 +282,10 @@ def test_batched_lowest_k_against_per_instance():
     hs = [DS.build_realisation(DS.realisation_eps(r, 14, 8.0), 14, 7, 7) for r in range(6)]
     psib = [torch.full((H.dim,), float("nan"), dtype=torch.float64, device="cuda") for H in hs]
     ev, ns, st = BatchedLanczos(hs, False).lowest_k(4, steps=500, tol=1e-10, evecs=psib)
-    # a realisation whose lowest levels lie close together may need more steps (ED_ENOCONV, the
-    # ensemble then solves it alone); the others must agree with the per-instance solver
-    assert (ns == 500).all() and (st == 0).sum() >= 4, (st, ns)
+    # the device test stops each realisation once its 4 lowest pass (240-330 steps here); one whose
+    # lowest levels lie close together may need more than 500 (ED_ENOCONV, the ensemble then
+    # solves it alone); the others must agree with the per-instance solver
+    assert (ns <= 500).all() and (st == 0).sum() >= 4 and (ns[st == 0] < 500).all(), (st, ns)
     for r, H in enumerate(hs):
         if st[r] != 0:
             continue

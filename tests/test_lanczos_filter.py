@@ -4,11 +4,15 @@ matrices with known spectra:
 
 * a start vector that touches the ground state by 1e-8 (full reorthogonalisation, so T has no
   ghosts): the converged lowest Ritz value is then also an eigenvalue of T without its first row
-  and column (checked), and the plain Cullum-Willoughby test would call it spurious; the filter
-  keeps it, in both modes.  A disordered 14-site chain had a start vector 1.1e-7 from its ground
-  state, and the solver reported its first excited level as E0;
-* a long run without reorthogonalisation (ghost copies of the low end present, checked): the
-  nev values are the nev lowest eigenvalues of the matrix, each once.
+  and column (checked), and the plain Cullum-Willoughby test would call it spurious.  After
+  selective orthogonalisation the filter keeps it; after a plain recurrence it holds the test
+  back (until a copy forms).  A disordered 14-site chain had a start vector 1.1e-7 from its
+  ground state, and the solver reported its first excited level as E0;
+* a long run without reorthogonalisation (ghost copies of the low end present, checked): every
+  check that reports convergence reports the nev lowest eigenvalues of the matrix, each once;
+* sweeps over start vectors that barely touch one of the four lowest eigenvalues (1e-12 to 1e-6),
+  with and without reorthogonalisation: no check reports convergence with a wrong set once every
+  one of the four has a Ritz value near it.
 """
 import numpy as np
 import pytest
@@ -72,7 +76,7 @@ def test_ground_state_barely_touched_by_start_vector_is_kept(semi_orth):
     assert abs(Z[0, 0]) < 1e-6
     assert np.abs(wh - w[0]).min() < 1e-12 * anorm
     out, found, conv, _ = lowest(a, b, 3, 1e-10, semi_orth)
-    assert found == 3 and conv
+    assert found == 3 and conv == bool(semi_orth)  # a plain recurrence waits for its copy
     assert np.allclose(out, lam[:3], rtol=0, atol=1e-11), out
 
 
@@ -85,9 +89,36 @@ def test_ghost_copies_are_dropped_after_plain_recurrence():
     w = sla.eigvalsh_tridiagonal(a, b[1:-1])
     # loss of orthogonality has made copies of the lowest eigenvalue
     assert (np.abs(w - lam[0]) < 1e-9).sum() >= 2
-    out, found, conv, _ = lowest(a, b, 4, 1e-10, 0)
-    assert found == 4 and conv
-    assert np.allclose(out, lam[:4], rtol=0, atol=1e-10), out
+    reports = 0
+    for m in range(300, 601, 10):
+        out, found, conv, _ = lowest(a[:m], b[:m + 1], 4, 1e-10, 0)
+        if conv:
+            reports += 1
+            assert np.allclose(out, lam[:4], rtol=0, atol=1e-10), (m, out)
+    assert reports >= 5
+
+
+@pytest.mark.parametrize("reorth", [False, True])
+def test_barely_touched_levels_never_give_a_wrong_set(reorth):
+    n = 1200
+    reports = 0
+    for trial in range(6):
+        rng = np.random.default_rng(300 + trial)
+        low = np.sort(rng.uniform(-1.0, -0.85, 5))
+        lam = np.concatenate([low, np.sort(rng.uniform(-0.8, 1.0, n - 5))])
+        f = rng.standard_normal(n)
+        f /= np.linalg.norm(f)
+        f[trial % 4] = 10.0 ** rng.uniform(-12, -6)
+        a, b = lanczos(lam, f, 300, reorth)
+        for m in range(20, 301, 10):
+            out, found, conv, _ = lowest(a[:m], b[:m + 1], 4, 1e-10, int(reorth))
+            if not conv:
+                continue
+            w = sla.eigvalsh_tridiagonal(a[:m], b[1:m])
+            if all(np.abs(w - lam[q]).min() < 1e-4 for q in range(4)):
+                reports += 1
+                assert np.allclose(out, lam[:4], rtol=0, atol=1e-9), (trial, m, out - lam[:4])
+    assert reports >= 20
 
 
 def test_unconverged_lowest_holds_the_test_back():
