@@ -32,7 +32,7 @@
 namespace ed {
 
 constexpr int kSyB = 64;   // half-bandwidth of the band form (stage 1 panel width)
-constexpr int kSyT = 256;  // threads of a bulge-chasing step
+constexpr int kSyT = 512;  // threads of a bulge-chasing step
 
 // count symmetric matrices of order n (ld lda, lda * n doubles apart) whose eigenvalues go to w
 // (n per matrix, ascending)
@@ -195,7 +195,7 @@ struct SbMat {
 __global__ __launch_bounds__(kSyT) void k_sb2st_step(const SbMat *__restrict__ mats, int nmats, int64_t ts)
 {
     constexpr int B = kSyB;
-    __shared__ double C[B][B + 1], D[B][B + 1];
+    __shared__ double D[B][B + 1];
     __shared__ double v[B], v2[B], p[B], red[kSyT];
     __shared__ double sc[4];
     int lo = 0, hi = nmats - 1; // the last matrix with slot0 <= blockIdx.x
@@ -232,9 +232,15 @@ __global__ __launch_bounds__(kSyT) void k_sb2st_step(const SbMat *__restrict__ m
         __syncthreads();
         const double tau = sc[0];
         if (t < L) A[(st + t) + s * lda] = t == 0 ? sc[1] : 0.0;
+        // the lower triangle column by column (contiguous in memory), mirrored in LDS
         for (int e = t; e < L * L; e += kSyT) {
             const int i = e % L, j = e / L;
-            D[i][j] = A[(st + std::max(i, j)) + (st + std::min(i, j)) * lda];
+            if (i >= j) D[i][j] = A[(st + i) + (st + j) * lda];
+        }
+        __syncthreads();
+        for (int e = t; e < L * L; e += kSyT) {
+            const int i = e % L, j = e / L;
+            if (i < j) D[i][j] = D[j][i];
         }
         __syncthreads();
         sy_two_sided(D, L, v, tau, p, red);
@@ -247,47 +253,59 @@ __global__ __launch_bounds__(kSyT) void k_sb2st_step(const SbMat *__restrict__ m
         return;
     }
     // chase: the previous reflector (columns st..ed) applied from the right to the block below,
-    // a new one from that block's first column, applied from the left and to the next diagonal block
+    // a new one from that block's first column, applied from the left and to the next diagonal block.
+    // The block below, C, stays in registers: thread (i = t % B, part = t / B) holds row i's
+    // columns j = part + PARTS q, so that only the diagonal block takes LDS (four workgroups per CU).
+    constexpr int PARTS = kSyT / B, NQ = B / PARTS;
     const int64_t st = s + 1 + (k - 1) * B, ed = std::min<int64_t>(s + k * B, n - 1);
     const int64_t j1 = s + k * B + 1, j2 = std::min<int64_t>(s + (k + 1) * B, n - 1);
     const int Lp = (int)(ed - st + 1), Lc = (int)(j2 - j1 + 1);
+    const int ri = t % B, part = t / B;
     // every load of the step up front: the reflector, the block below and the next diagonal block
-    if (t < Lp) v[t] = rv[t];
+    // (its lower triangle, contiguous columns, mirrored in LDS below)
+    if (t < B) v[t] = t < Lp ? rv[t] : 0.0; // zero past Lp: the products below run over all B columns
     if (t == 0) sc[0] = rv[B];
-    for (int e = t; e < Lc * Lp; e += kSyT) {
-        const int i = e % Lc, j = e / Lc;
-        C[i][j] = A[(j1 + i) + (st + j) * lda];
+    double c[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int j = part + PARTS * q;
+        c[q] = (ri < Lc && j < Lp) ? A[(j1 + ri) + (st + j) * lda] : 0.0;
     }
     for (int e = t; e < Lc * Lc; e += kSyT) {
         const int i = e % Lc, j = e / Lc;
-        D[i][j] = A[(j1 + std::max(i, j)) + (j1 + std::min(i, j)) * lda];
+        if (i >= j) D[i][j] = A[(j1 + i) + (j1 + j) * lda];
     }
     __syncthreads();
+    for (int e = t; e < Lc * Lc; e += kSyT) {
+        const int i = e % Lc, j = e / Lc;
+        if (i < j) D[i][j] = D[j][i];
+    }
     const double tau = sc[0];
     // u = C v; C -= tau u v^T
     {
-        constexpr int PARTS = kSyT / B;
-        const int i = t % B, part = t / B;
         double acc = 0.0;
-        if (i < Lc)
-            for (int j = part; j < Lp; j += PARTS) acc = fma(C[i][j], v[j], acc);
-        red[part * B + i] = acc;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc = fma(c[q], v[part + PARTS * q], acc); // c is 0 past Lp
+        red[part * B + ri] = acc;
     }
     __syncthreads();
     if (t < B) {
         double acc = 0.0;
         if (t < Lc)
-            for (int q = 0; q < kSyT / B; ++q) acc += red[q * B + t];
+            for (int q = 0; q < PARTS; ++q) acc += red[q * B + t];
         p[t] = tau * acc;
     }
     __syncthreads();
-    for (int e = t; e < Lc * Lp; e += kSyT) {
-        const int i = e % Lc, j = e / Lc;
-        C[i][j] -= p[i] * v[j];
+    {
+        const double pi = p[ri];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int j = part + PARTS * q;
+            if (ri < Lc && j < Lp) c[q] -= pi * v[j];
+        }
     }
-    __syncthreads();
-    // new reflector from C[:, 0]
-    if (t < Lc) v2[t] = C[t][0];
+    // new reflector from C[:, 0] (held by the threads of part 0, q = 0)
+    if (part == 0 && ri < Lc) v2[ri] = c[0];
     __syncthreads();
     if (t < 64) {
         double tau2, beta2;
@@ -299,30 +317,30 @@ __global__ __launch_bounds__(kSyT) void k_sb2st_step(const SbMat *__restrict__ m
     }
     __syncthreads();
     const double tau2 = sc[2];
-    // C[:, 1:] <- H2 C[:, 1:]: r_j = v2 . C[:, j]; C[i][j] -= tau2 v2_i r_j
+    // C[:, 1:] <- H2 C[:, 1:]: r_j = v2 . C[:, j] summed over the wavefront holding column j;
+    // C[i][j] -= tau2 v2_i r_j
     {
-        constexpr int PARTS = kSyT / B;
-        const int j = t % B, part = t / B;
-        double acc = 0.0;
-        if (j >= 1 && j < Lp)
-            for (int i = part; i < Lc; i += PARTS) acc = fma(v2[i], C[i][j], acc);
-        red[part * B + j] = acc;
+        const double vi = ri < Lc ? v2[ri] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double r = wave_sum(vi * c[q]);
+            const int j = part + PARTS * q;
+            if (ri == 0) p[j] = tau2 * r;
+        }
     }
     __syncthreads();
-    if (t < B) {
-        double acc = 0.0;
-        for (int q = 0; q < kSyT / B; ++q) acc += red[q * B + t];
-        p[t] = tau2 * acc;
+    {
+        const double vi = ri < Lc ? v2[ri] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int j = part + PARTS * q;
+            if (ri < Lc && j < Lp) {
+                const double cv = j == 0 ? (ri == 0 ? sc[3] : 0.0) : c[q] - vi * p[j];
+                A[(j1 + ri) + (st + j) * lda] = cv;
+            }
+        }
     }
-    __syncthreads();
-    for (int e = t; e < Lc * Lp; e += kSyT) {
-        const int i = e % Lc, j = e / Lc;
-        double c;
-        if (j == 0) c = i == 0 ? sc[3] : 0.0;
-        else c = C[i][j] - v2[i] * p[j];
-        A[(j1 + i) + (st + j) * lda] = c;
-    }
-    // the next diagonal block, two-sided (p is free again once every thread has stored C)
+    // the next diagonal block, two-sided (p is free again once every thread has read it)
     __syncthreads();
     sy_two_sided(D, Lc, v2, tau2, p, red);
     for (int e = t; e < Lc * Lc; e += kSyT) {
