@@ -103,3 +103,33 @@ def test_batch_shapes_bitwise_and_autotune():
     assert t["dn_rows"] in (4, 8) and t["up_rows"] in (1, 4) and t["seconds"] > 0
     ev, ns, st = bl.lowest(use_graph=True)
     assert np.array_equal(ev.view(np.uint64), ref[0].view(np.uint64))
+
+
+def test_lowest_k_complex_ground_states():
+    """ed_batch_lowest_k with ground states on complex instances (Lieb-12 at flux 2 pi / 16, five
+    sectors): values against the plain Lanczos E0, ground states against its Ritz vector from the
+    same start vector, residual |H psi - E0 psi|.  With a 20-step cap nothing passes: every
+    instance reports ED_ENOCONV and its vector is left as it was."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import Lanczos
+    print("batched lowest_k complex lieb12 with ground states", flush=True)
+    inst = BT.lieb12_instances(U=4.0, nflux=16, sectors=[3, 5, 6, 7, 9], fluxes=[1])
+    hs = [h for (_, _, h) in inst]
+    psib = [torch.full((h.dim,), complex("nan"), dtype=torch.complex128, device="cuda") for h in hs]
+    ev, ns, st = BT.BatchedLanczos(hs, True).lowest_k(2, steps=600, tol=1e-10, evecs=psib)
+    assert np.all(st == 0), (st, ns)
+    for h, e, pb in zip(hs, ev, psib):
+        f = torch.empty(h.dim, dtype=torch.complex128, device="cuda")
+        psi = torch.empty_like(f)
+        lz = Lanczos(h.dimUP, h.dimDN, max_steps=2000, tol=1e-10)
+        lz.random_vector(f)
+        e0 = lz.eigen(h, f, psi)
+        assert abs(e[0] - e0) < 1e-10 * max(1.0, abs(e0)), (e, e0)
+        assert abs(float(torch.linalg.vector_norm(pb)) - 1.0) < 1e-12
+        if e[1] - e[0] > 1e-6:
+            assert abs(abs(complex(torch.vdot(pb, psi))) - 1.0) < 1e-9
+        res = float(torch.linalg.vector_norm(h.apply(pb) - e[0] * pb))
+        assert res < 1e-7, res
+    short = [torch.full((h.dim,), complex("nan"), dtype=torch.complex128, device="cuda") for h in hs]
+    ev2, ns2, st2 = BT.BatchedLanczos(hs, True).lowest_k(2, steps=20, tol=1e-10, evecs=short)
+    assert np.all(st2 != 0) and np.all(ns2 == 20), (st2, ns2)
+    assert all(bool(torch.isnan(v.real).all()) for v in short)
