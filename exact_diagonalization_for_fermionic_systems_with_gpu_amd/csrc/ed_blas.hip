@@ -112,19 +112,31 @@ int lanczos_update(int64_t nreal, const double *v, double *f, const double *adev
 
 // y = -a[step]*(f/b[step]) + y ; partial sum y^2 -- k_lanczos_update with v = (1/b[step]) f, the
 // product the H.v pass forms when it scales f, so the same bits as with a stored v
+// 16 B per lane (element pairs), nontemporal: the three streams are read or written once per
+// step and do not fit any cache at the sizes this runs on
 __global__ __launch_bounds__(kBB) void k_lanczos_update_vf(int64_t n, const double *__restrict__ f,
                                                            double *__restrict__ y, const double *__restrict__ adev,
                                                            const double *__restrict__ bdev, int step,
                                                            double *__restrict__ partial)
 {
+    typedef double d2 __attribute__((ext_vector_type(2)));
     __shared__ double red[kBB / 64];
     const double c = -adev[step], sc = 1.0 / bdev[step];
-    const int64_t base = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kChunk + 2 * (int64_t)threadIdx.x;
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < kChunk / kBB; ++k) {
-        const int64_t i = base + (int64_t)k * kBB;
-        if (i < n) {
+    for (int k = 0; k < kChunk / (2 * kBB); ++k) {
+        const int64_t i = base + (int64_t)k * (2 * kBB);
+        if (i + 1 < n) {
+            const d2 fv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(f + i));
+            const d2 yv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(y + i));
+            d2 r;
+            r.x = fma(c, sc * fv.x, yv.x);
+            r.y = fma(c, sc * fv.y, yv.y);
+            __builtin_nontemporal_store(r, reinterpret_cast<d2 *>(y + i));
+            s = fma(r.x, r.x, s);
+            s = fma(r.y, r.y, s);
+        } else if (i < n) {
             const double r = fma(c, sc * f[i], y[i]);
             y[i] = r;
             s = fma(r, r, s);
