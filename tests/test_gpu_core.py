@@ -5,7 +5,7 @@ buffers.
 
 Bars: integer tables and the start vector bit-exact; H.v within 1e-14 of max|y|
 (same per-element summation order, FMA contraction differs); E0 within 1e-10 relative
-same number of Lanczos steps, Ritz vectors parallel to 1e-8.
+(the parity bar), same number of Lanczos steps, Ritz vectors parallel to 1e-8.
 """
 import math
 

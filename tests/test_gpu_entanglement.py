@@ -1,6 +1,6 @@
 """GPU entanglement entropy (per-block rho_A with FP64 MFMA + rocSOLVER) against the oracle:
 the reference's dense first-appearance matrix + SVD on small systems, and the block oracle on
-the 12-site Kagome ground state (6|6 sites).  Bar: 1e-8 relative on S (north-star)."""
+the 12-site Kagome ground state (6|6 sites).  Bar: 1e-8 relative on S."""
 import math
 
 import numpy as np
