@@ -2,8 +2,8 @@
  * ed_oracle.c -- CPU ORACLE for the exact-diagonalization hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
- * reference arithmetic, used as the checker in tests/, in
- * __graft_entry__.smoke() and as bench.py's cpu_baseline leg.  Nothing in
+ * reference arithmetic, which tests/ and __graft_entry__.smoke() check the
+ * GPU results against and bench.py times as its cpu_baseline leg.  Nothing in
  * the product package links, loads or calls it; the product path fails
  * loudly when its HIP library is missing instead of falling back here.
  *
