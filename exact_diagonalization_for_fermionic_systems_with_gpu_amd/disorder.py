@@ -92,42 +92,39 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
                  tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
-                 max_steps=2000, batch_steps=600) -> EnsembleResult:
+                 max_steps=2000, batch_steps=600, batch_size=128) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched launch, each worker
     thread on its own stream.  method "batched": the nev lowest of every realisation and its
     ground state from one batched run of at most batch_steps steps (each realisation stops once
     its nev lowest pass) and a batched rebuild of the Ritz vectors (ed_batch_lowest_k, test
-    tolerance tol); a realisation whose batched values fail the test falls back to "lanczos".  "lanczos": after a
-    batched E0 launch, ed_lanczos_lowest per realisation (at most max_steps steps; tol is its
-    test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs); "restart": the
-    thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test with tol).
-    Without S_ref the entropies are taken after the solves, all realisations in one batched
-    call."""
+    tolerance tol); a realisation whose batched values fail the test falls back to "lanczos".
+    "lanczos": after a batched E0 launch, ed_lanczos_lowest per realisation (at most max_steps
+    steps; tol is its test's tolerance, |b s_N| < tol |T| for each of the nev lowest Ritz pairs);
+    "restart": the thick-restart solver with ncv basis vectors and maxit restarts (Spectra's test
+    with tol).  Without S_ref the entropies are taken after the solves, all realisations of a
+    batch in one batched call.  Realisations go through in batches of at most batch_size, so that
+    the device memory held (the batch's vectors and ground states: about 6 batch_size vectors of
+    the sector) does not grow with n_real."""
     if method not in ("batched", "lanczos", "restart"):
         raise ValueError("run_ensemble: method is 'batched', 'lanczos' or 'restart'")
+    if batch_size < 1:
+        raise ValueError("run_ensemble: batch_size >= 1")
     eps = np.stack([realisation_eps(r, L_sites, W, seed0) for r in range(n_real)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hs = [build_realisation(eps[r], L_sites, n_up, n_dn, U, t) for r in range(n_real)]
-    tb = time.perf_counter()
-    psib = None
-    if method == "batched":
-        psib = [torch.empty(h.dim, dtype=torch.float64, device="cuda") for h in hs]
-        evb, _, stb = BatchedLanczos(hs, False).lowest_k(nev, steps=batch_steps, tol=tol, evecs=psib)
-        e0b = evb[:, 0].copy()
-    else:
-        e0b, _, _ = BatchedLanczos(hs, False).lowest(tol=1e-12)
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - tb
+    tb = 0.0
     fallbacks = 0
+    e0b = np.zeros(n_real)
     evals = np.zeros((n_real, nev))
     S = np.zeros((n_real, 2))
     g = np.zeros((n_real, L_sites))
-
-    psis = [None] * n_real  # ground states kept for the batched entropy (without S_ref)
+    hs, psib, psis, evb, stb = {}, {}, {}, {}, {}
+    import threading
+    lock = threading.Lock()
 
     def solve(r, stream):
         # the library releases the GIL in every call; each worker thread drives its own stream
+        nonlocal fallbacks
         with torch.cuda.stream(stream):
             H = hs[r]
             if method == "restart":
@@ -141,13 +138,12 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
                 del es
             elif method == "batched" and stb[r] == 0:
                 evals[r] = evb[r]
-                psi = psib[r]  # rebuilt by the batched call
-                psib[r] = None
+                psi = psib.pop(r)  # rebuilt by the batched call
             else:
                 if method == "batched":
-                    nonlocal fallbacks
-                    fallbacks += 1
-                    psib[r] = None
+                    with lock:
+                        fallbacks += 1
+                    psib.pop(r, None)
                 f = torch.empty(H.dim, dtype=torch.float64, device="cuda")
                 psi = torch.empty_like(f)
                 lz = Lanczos(H.dimUP, H.dimDN, max_steps=max_steps, tol=tol)
@@ -164,29 +160,46 @@ def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, se
             if log is not None:
                 log(f"realisation {r}: E0 {evals[r, 0]:.12f}, {time.perf_counter() - t0:.2f} s elapsed")
 
-    if workers <= 1:
-        for r in range(n_real):
-            solve(r, torch.cuda.current_stream())
-    else:
-        import threading
-        from concurrent.futures import ThreadPoolExecutor
-        local = threading.local()
+    for c0 in range(0, n_real, batch_size):
+        idx = list(range(c0, min(n_real, c0 + batch_size)))
+        for r in idx:
+            hs[r] = build_realisation(eps[r], L_sites, n_up, n_dn, U, t)
+        hl = [hs[r] for r in idx]
+        tc = time.perf_counter()
+        if method == "batched":
+            vecs = [torch.empty(h.dim, dtype=torch.float64, device="cuda") for h in hl]
+            ev, _, st = BatchedLanczos(hl, False).lowest_k(nev, steps=batch_steps, tol=tol, evecs=vecs)
+            for q, r in enumerate(idx):
+                psib[r], evb[r], stb[r], e0b[r] = vecs[q], ev[q], st[q], ev[q, 0]
+            del vecs
+        else:
+            e0b[idx], _, _ = BatchedLanczos(hl, False).lowest(tol=1e-12)
+        torch.cuda.synchronize()
+        tb += time.perf_counter() - tc
+        if workers <= 1:
+            for r in idx:
+                solve(r, torch.cuda.current_stream())
+        else:
+            from concurrent.futures import ThreadPoolExecutor
+            local = threading.local()
 
-        def task(r):
-            if not hasattr(local, "stream"):
-                local.stream = torch.cuda.Stream()  # one stream per worker thread
-            solve(r, local.stream)
+            def task(r):
+                if not hasattr(local, "stream"):
+                    local.stream = torch.cuda.Stream()  # one stream per worker thread
+                solve(r, local.stream)
 
-        # realisation 0 runs on this thread first: every library routine the workers call (and
-        # the rocBLAS/rocSOLVER kernels behind them) is then loaded before several threads make
-        # their first calls at once
-        task(0)
-        with ThreadPoolExecutor(workers) as ex:
-            for f in [ex.submit(task, r) for r in range(1, n_real)]:
-                f.result()
-    if not with_sref:
-        S[:, 0] = entropy_vn_batch(L_sites, n_up, n_dn, (1 << (L_sites // 2)) - 1, psis)
-        S[:, 1] = np.nan
-        del psis[:]
+            # the batch's first realisation runs on this thread first: every library routine the
+            # workers call (and the rocBLAS/rocSOLVER kernels behind them) is then loaded before
+            # several threads make their first calls at once
+            task(idx[0])
+            with ThreadPoolExecutor(workers) as ex:
+                for f in [ex.submit(task, r) for r in idx[1:]]:
+                    f.result()
+        if not with_sref:
+            S[idx, 0] = entropy_vn_batch(L_sites, n_up, n_dn, (1 << (L_sites // 2)) - 1, [psis[r] for r in idx])
+            S[idx, 1] = np.nan
+            psis.clear()
+        hs.clear()
+        psib.clear()
     torch.cuda.synchronize()
     return EnsembleResult(eps, e0b, evals, S, g, time.perf_counter() - t0, tb, fallbacks)

@@ -175,6 +175,19 @@ def test_chain14_against_fixture(method):
         assert np.allclose(res.grad_eps[r], rec["dE0_deps"], atol=1e-8)
 
 
+def test_disorder_ensemble_in_batches_matches_one_batch():
+    """Realisations taken in batches of 4 (two batched launches) give what one batch of all 6 gives,
+    to the rounding of the other group size (the batch's partial sums are per tile of 256 / G
+    columns)."""
+    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, tol=1e-12, method="batched", with_sref=False)
+    a = DS.run_ensemble(**kw)
+    b = DS.run_ensemble(**kw, batch_size=4)
+    assert np.allclose(a.evals, b.evals, rtol=1e-12, atol=0)
+    assert np.allclose(a.entropy[:, 0], b.entropy[:, 0], rtol=0, atol=1e-9)
+    assert np.allclose(a.grad_eps, b.grad_eps, rtol=0, atol=1e-9)
+    assert np.allclose(a.e0_batched, b.e0_batched, rtol=1e-12, atol=0)
+
+
 def test_disorder_ensemble_concurrent_workers_match_serial():
     """Several realisations solved at once on separate streams (one rocBLAS handle and scratch per
     host thread) give the serial results."""
