@@ -1285,6 +1285,9 @@ extern "C" int ed_batch_lowest_k(ed_batch bb, const ed_lanczos_cfg *cfg, int nev
     Batch *B = reinterpret_cast<Batch *>(bb);
     if (!B || !evals || nev < 1 || nev > 64) return fail(ED_EINVAL, "ed_batch_lowest_k: bad arguments (nev 1..64)");
     if (cfg && cfg->reorth) return fail(ED_EUNSUPPORTED, "ed_batch_lowest_k: no reorthogonalisation in the batch");
+    if (d_evecs)
+        for (int i = 0; i < B->n; ++i)
+            if (!d_evecs[i]) return fail(ED_EINVAL, "ed_batch_lowest_k: d_evecs[%d] is NULL", i);
     const int maxN = cfg && cfg->max_steps > 0 ? cfg->max_steps : 1000;
     const double tol = cfg && cfg->tol > 0 ? cfg->tol : 0.01 * std::sqrt(std::numeric_limits<double>::epsilon());
     // each copy runs until its nev lowest pass the device test (k_g_check_k), its recurrence ends
