@@ -61,6 +61,9 @@ class BatchedLanczos:
                 "seconds": t.seconds, "steps": t.steps, "candidate_s": [round(v, 4) for v in t.candidate_s]}
 
     def lowest(self, max_steps=1000, tol=None, use_graph=True, seed=235):
+        """E0 of every instance (ed_batch_lowest).  use_graph: 0/False eager launches, 1/True
+        fifty-step graph replays, 2 one graph launch for every step after the first ten up to
+        max_steps.  Returns (evals [n], nsteps [n], status [n])."""
         n = len(self.hubbards)
         cfg = LanczosCfg(max_steps, 0.0 if tol is None else tol, 10, 0)
         ev = (C.c_double * n)()
@@ -96,8 +99,9 @@ class BatchedLanczos:
         return ev.reshape(n, nev), np.array(ns[:]), np.array(st[:])
 
 
-def plain_lowest(hubbards, complex_vectors: bool, max_steps=1000, tol=None):
-    """One instance after another: the reference's loop over samples/sectors."""
+def plain_lowest(hubbards, complex_vectors: bool, max_steps=1000, tol=None, strict=False):
+    """One instance after another: the reference's loop over samples/sectors.  strict: raise when
+    an instance misses the convergence test (otherwise its last Ritz value is returned)."""
     evs, ns = [], []
     for H in hubbards:
         dt = torch.complex128 if complex_vectors else torch.float64
@@ -109,7 +113,7 @@ def plain_lowest(hubbards, complex_vectors: bool, max_steps=1000, tol=None):
         n = C.c_int()
         st = lib().ed_lanczos_eigen(H.handle, int(complex_vectors), C.byref(cfg), C.c_void_p(f.data_ptr()),
                                     C.byref(e), None, None, None, C.byref(n), stream_ptr())
-        if st not in (0, 4):
+        if st not in (0, 4) or (strict and st != 0):
             from ._lib import check
             check(st)
         evs.append(e.value)
@@ -143,9 +147,14 @@ def lieb12_instances(U=4.0, nflux=16, t=1.0, n_total=12, sectors=None, fluxes=No
     return out
 
 
-def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=True, max_steps=1000, tol=None, shape=None):
+def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=2, max_steps=1000, tol=None, shape=None,
+                       graph_steps=300):
     """All 13 sectors x nflux fluxes.  Returns (E0[13, nflux], D[13], seconds, nsteps).  shape: the
-    batch kernels' panel shape (dn_rows, dn_panels, up_rows, up_panels), e.g. an autotuned one."""
+    batch kernels' panel shape (dn_rows, dn_panels, up_rows, up_panels), e.g. an autotuned one.
+    max_steps: the step cap of the plain loop, the reference's maxN = 1000
+    (HubMatrix_GPU.cuh:563).  The batched solve runs as one graph launch after its first ten steps
+    (use_graph 2), capped at graph_steps = 300 (the instances converge in at most 130); an
+    instance that does not converge raises, in either loop."""
     inst = lieb12_instances(U, nflux)
     hs = [h for (_, _, h) in inst]
     torch.cuda.synchronize()
@@ -154,9 +163,12 @@ def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=True, max_steps=
         bl = BatchedLanczos(hs, True)
         if shape is not None:
             bl.set_shape(*shape)
-        ev, ns, st = bl.lowest(max_steps, tol, use_graph)
+        cap = min(max_steps, graph_steps) if use_graph == 2 else max_steps
+        ev, ns, st = bl.lowest(cap, tol, use_graph)
+        if np.any(st != 0):
+            raise RuntimeError(f"lieb12_drude_sweep: {int(np.sum(st != 0))} instances did not converge in {cap} steps")
     else:
-        ev, ns = plain_lowest(hs, True, max_steps, tol)
+        ev, ns = plain_lowest(hs, True, max_steps, tol, strict=True)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     E0 = np.zeros((13, nflux))

@@ -965,12 +965,12 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
     int *pact = nullptr; // pinned: active flags, step counter, copies still running
     // graphs of kGraphChunks chunks (and of one chunk, for the tail) per buffer rotation phase
     constexpr int kGraphChunks = 5;
-    hipGraphExec_t gexec[2][3] = {};
-    hipGraph_t graph[2][3] = {};
+    hipGraphExec_t gexec[3][3] = {}; // [fifty steps, ten steps, the rest of the solve][phase]
+    hipGraph_t graph[3][3] = {};
     int s = ED_OK;
     auto cleanup = [&] {
         (void)hipStreamSynchronize(st);
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < 3; ++q)
             for (int k = 0; k < 3; ++k) {
                 if (gexec[q][k]) (void)hipGraphExecDestroy(gexec[q][k]);
                 if (graph[q][k]) (void)hipGraphDestroy(graph[q][k]);
@@ -1156,7 +1156,9 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
         // and the host reads it once the next submission is queued.
         auto submit = [&]() -> int {
             const int left = (maxN - N) / every; // whole chunks left
-            const int q = left >= kGraphChunks ? 0 : 1, nch = q == 0 ? kGraphChunks : 1;
+            // use_graph 2: every whole chunk left in one graph, launched once
+            const int q = use_graph == 2 ? 2 : left >= kGraphChunks ? 0 : 1,
+                      nch = q == 2 ? left : q == 0 ? kGraphChunks : 1;
             if (!use_graph || left == 0) return enqueue_chunks(left == 0 ? 1 : nch);
             if (!gexec[q][phase]) {
                 const int N0 = N, ph0 = phase;

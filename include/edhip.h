@@ -385,7 +385,10 @@ typedef struct ed_batch_s *ed_batch;
 int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vectors, ed_batch *out);
 int ed_batch_destroy(ed_batch b);
 /* evals[ninst], nsteps[ninst], status[ninst] (ED_OK or ED_ENOCONV) out; use_graph = 0 launches
- * the same kernels without capture.  Synchronises. */
+ * the same kernels without capture; 1 replays fifty-step graphs; 2 captures every step after the
+ * first ten, up to cfg->max_steps, in one graph and launches it once (the steps after the last
+ * instance has stopped find no tile to work on and return at once: set max_steps near the
+ * expected step count).  The results do not depend on it, bit for bit.  Synchronises. */
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
 /* The nev lowest eigenvalues of every instance from one batched run (the SymEigsSolver

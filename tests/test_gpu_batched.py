@@ -16,10 +16,10 @@ pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lieb12_drude_oracle.json")))
 
 
-def _sweep_subset(use_graph):
+def _sweep_subset(use_graph, max_steps=1000):
     inst = BT.lieb12_instances(U=4.0, nflux=16, fluxes=[0, 1, 15])
     hs = [h for (_, _, h) in inst]
-    ev, ns, st = BT.BatchedLanczos(hs, True).lowest(use_graph=use_graph)
+    ev, ns, st = BT.BatchedLanczos(hs, True).lowest(max_steps, use_graph=use_graph)
     return inst, hs, ev, ns, st
 
 
@@ -50,10 +50,14 @@ def test_batched_matches_plain_and_oracle():
 
 
 def test_graph_replay_equals_eager():
-    _, _, ev_g, ns_g, _ = _sweep_subset(True)
-    _, _, ev_e, ns_e, _ = _sweep_subset(False)
-    assert np.array_equal(ns_g, ns_e)
+    """fifty-step graph replays, one graph for the whole solve (capped at 300 steps) and eager
+    launches: the same step counts and eigenvalues bit for bit"""
+    _, _, ev_g, ns_g, _ = _sweep_subset(1)
+    _, _, ev_e, ns_e, _ = _sweep_subset(0)
+    _, _, ev_w, ns_w, st_w = _sweep_subset(2, max_steps=300)
+    assert np.array_equal(ns_g, ns_e) and np.array_equal(ns_w, ns_e) and np.all(st_w == 0)
     assert np.array_equal(ev_g.view(np.uint64), ev_e.view(np.uint64))
+    assert np.array_equal(ev_w.view(np.uint64), ev_e.view(np.uint64))
 
 
 def test_full_sweep_all_instances():
