@@ -263,9 +263,9 @@ def entropy_only_line(N=18, Q=9, seed=5):
 
 def batched_drude_line(with_cpu=False):
     """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
-    weight per sector; batched (ed_batch_lowest: the first ten steps eagerly, then every step up
-    to the 300-step cap in one hipGraph launch) vs the plain per-instance loop on the same GPU,
-    both checked against the oracle's Drude weights (tests/golden fixture)."""
+    weight per sector; batched (ed_batch_lowest: the device convergence test every ten steps,
+    fifty-step hipGraph replays) vs the plain per-instance loop on the same GPU, both checked
+    against the oracle's Drude weights (tests/golden fixture)."""
     import numpy as np
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import batched as BT
     gold = json.load(open(os.path.join(REPO, "tests", "golden", "lieb12_drude_oracle.json")))
@@ -312,7 +312,7 @@ def batched_drude_line(with_cpu=False):
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
             "max_abs_e0_batched_minus_plain": e0_diff, "max_abs_e0_flux_asymmetry": e0_asym,
             "batch_shape": {k: tune[k] for k in ("dn_rows", "dn_panels", "up_rows", "up_panels")},
-            "graph": "one launch per sweep after the first ten steps (cap 300 steps)",
+            "graph": "fifty-step graph replays",
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 

@@ -147,14 +147,14 @@ def lieb12_instances(U=4.0, nflux=16, t=1.0, n_total=12, sectors=None, fluxes=No
     return out
 
 
-def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=2, max_steps=1000, tol=None, shape=None,
+def lieb12_drude_sweep(U=4.0, nflux=16, batched=True, use_graph=1, max_steps=1000, tol=None, shape=None,
                        graph_steps=300):
     """All 13 sectors x nflux fluxes.  Returns (E0[13, nflux], D[13], seconds, nsteps).  shape: the
     batch kernels' panel shape (dn_rows, dn_panels, up_rows, up_panels), e.g. an autotuned one.
-    max_steps: the step cap of the plain loop, the reference's maxN = 1000
-    (HubMatrix_GPU.cuh:563).  The batched solve runs as one graph launch after its first ten steps
-    (use_graph 2), capped at graph_steps = 300 (the instances converge in at most 130); an
-    instance that does not converge raises, in either loop."""
+    max_steps: the step cap of both loops, the reference's maxN = 1000 (HubMatrix_GPU.cuh:563).
+    use_graph 1 (default): fifty-step graph replays; 2: the batched solve after its first ten
+    steps as one graph launch, capped at graph_steps (the instances converge in at most 130 steps).
+    An instance that does not converge raises, in either loop."""
     inst = lieb12_instances(U, nflux)
     hs = [h for (_, _, h) in inst]
     torch.cuda.synchronize()
