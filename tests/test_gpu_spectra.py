@@ -188,10 +188,12 @@ def test_disorder_ensemble_in_batches_matches_one_batch():
     assert np.allclose(a.e0_batched, b.e0_batched, rtol=1e-12, atol=0)
 
 
-def test_disorder_ensemble_concurrent_workers_match_serial():
-    """Several realisations solved at once on separate streams (one rocBLAS handle and scratch per
-    host thread) give the serial results."""
-    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, ncv=12, tol=1e-12)
+@pytest.mark.parametrize("method", ["lanczos", "batched"])
+def test_disorder_ensemble_concurrent_workers_match_serial(method):
+    """Several realisations solved (method "lanczos") or post-processed (entropies, densities) at
+    once on separate streams (one rocBLAS handle and scratch per host thread) give the serial
+    results."""
+    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, ncv=12, tol=1e-12, method=method)
     a = DS.run_ensemble(**kw, workers=1)
     b = DS.run_ensemble(**kw, workers=3)
     assert np.allclose(a.evals, b.evals, rtol=1e-12, atol=0)
