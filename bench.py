@@ -572,10 +572,21 @@ def main():
     if not args.no_extras:
         del x, y
         torch.cuda.empty_cache()
-        out["batched_drude"] = batched_drude_line(with_cpu=not args.no_cpu)
-        out["kagome12_ground_state"] = kagome12_line(with_cpu=not args.no_cpu)
-        out["disorder_ensemble"] = ensemble_line(args.ensemble, workers=args.workers, method=args.ensemble_method)
-        out["krylov_quench"] = krylov_line()
+
+        def extra(name, fn):
+            # a Python-level failure of one secondary line is recorded in its place; the headline
+            # line above is printed either way
+            try:
+                out[name] = fn()
+            except Exception as e:  # noqa: BLE001
+                log(f"[bench] {name} failed: {type(e).__name__}: {e}")
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
+
+        extra("batched_drude", lambda: batched_drude_line(with_cpu=not args.no_cpu))
+        extra("kagome12_ground_state", lambda: kagome12_line(with_cpu=not args.no_cpu))
+        extra("disorder_ensemble", lambda: ensemble_line(args.ensemble, workers=args.workers,
+                                                         method=args.ensemble_method))
+        extra("krylov_quench", krylov_line)
     print(json.dumps(out), flush=True)
 
 
