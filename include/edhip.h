@@ -388,7 +388,9 @@ int ed_batch_destroy(ed_batch b);
  * the same kernels without capture; 1 replays fifty-step graphs; 2 captures every step after the
  * first ten, up to cfg->max_steps, in one graph and launches it once (the steps after the last
  * instance has stopped find no tile to work on and return at once: set max_steps near the
- * expected step count).  The results do not depend on it, bit for bit.  Synchronises. */
+ * expected step count; not used by the library's own sweeps, since a run that used it for the
+ * Lieb-12 sweep inside a longer job ended in a GPU memory fault of unknown cause).  The results
+ * do not depend on it, bit for bit.  Synchronises. */
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
 /* The nev lowest eigenvalues of every instance from one batched run (the SymEigsSolver
