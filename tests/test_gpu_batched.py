@@ -146,3 +146,28 @@ def test_lowest_k_complex_ground_states():
     ev2, ns2, st2 = BT.BatchedLanczos(hs, True).lowest_k(2, steps=20, tol=1e-10, evecs=short)
     assert np.all(st2 != 0) and np.all(ns2 == 20), (st2, ns2)
     assert all(bool(torch.isnan(v.real).all()) for v in short)
+
+
+def test_lowest_k_small_sectors_end_in_invariant_subspaces():
+    """Sectors of dimension 1, 144 and 4356 (Lieb-12, N_up = 0, 1, 2 at flux 2 pi / 16): the two
+    larger recurrences end in an invariant subspace before 300 steps or converge, their values
+    and rebuilt ground states against the plain Lanczos solve; the dimension-1 sector has no
+    second eigenvalue, so it reports ED_ENOCONV and keeps its vector."""
+    from exact_diagonalization_for_fermionic_systems_with_gpu_amd import Lanczos
+    print("batched lowest_k small sectors", flush=True)
+    inst = BT.lieb12_instances(U=4.0, nflux=16, sectors=[0, 1, 2], fluxes=[1])
+    hs = [h for (_, _, h) in inst]
+    assert [h.dim for h in hs] == [1, 144, 4356]
+    psib = [torch.full((h.dim,), complex("nan"), dtype=torch.complex128, device="cuda") for h in hs]
+    ev, ns, st = BT.BatchedLanczos(hs, True).lowest_k(2, steps=300, tol=1e-10, evecs=psib)
+    assert st[0] != 0 and bool(torch.isnan(psib[0].real).all())
+    assert st[1] == 0 and st[2] == 0, (st, ns)
+    for h, e, pb in zip(hs[1:], ev[1:], psib[1:]):
+        f = torch.empty(h.dim, dtype=torch.complex128, device="cuda")
+        psi = torch.empty_like(f)
+        lz = Lanczos(h.dimUP, h.dimDN, max_steps=2000, tol=1e-10)
+        lz.random_vector(f)
+        e0 = lz.eigen(h, f, psi)
+        assert abs(e[0] - e0) < 1e-10 * max(1.0, abs(e0)), (e, e0)
+        res = float(torch.linalg.vector_norm(h.apply(pb) - e[0] * pb))
+        assert abs(float(torch.linalg.vector_norm(pb)) - 1.0) < 1e-12 and res < 1e-7, res
