@@ -91,7 +91,7 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
-                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="batched",
+                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
                  max_steps=2000, batch_steps=600, batch_size=128) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched launch, each worker
     thread on its own stream.  method "batched": the nev lowest of every realisation and its
