@@ -412,7 +412,9 @@ int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int us
  * passes the test -- where ed_lanczos_eigen would stop -- by a second batched run of the same
  * recurrence (the same a and b bit for bit) that accumulates sum z_k v_k, each instance for its
  * own n steps; an instance whose T_n value is not the filter's lowest (to 1e-8 relative) gets
- * ED_ENOCONV and no vector.  Synchronises. */
+ * ED_ENOCONV and no vector.  Synchronises.  The library's disorder ensemble does not use this
+ * call by default: one 128-instance run of it (max_steps 500) ended in a GPU memory fault whose
+ * cause has not been found. */
 int ed_batch_lowest_k(ed_batch b, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph, double *evals,
                       int *nsteps, int *status, void *const *d_evecs, void *stream);
 /* Panel shape of the batch kernels: DN rows per panel (4 or 8) x DN panels per workgroup (1-16),
