@@ -316,7 +316,7 @@ def batched_drude_line(with_cpu=False):
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
-def ensemble_line(n_real, ncv=24, workers=1, method="lanczos"):
+def ensemble_line(n_real, ncv=24, workers=1, method="batched"):
     """14-site open chain, 7/7, U = 4, W = 8.  Method "batched": the 4 lowest eigenvalues of all
     realisations from one batched Lanczos launch (each realisation stops once its 4 lowest pass
     the test, at most 600 steps) and their ground states from a batched rebuild
@@ -395,7 +395,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the disorder-ensemble line (--ensemble realisations, --workers threads)")
     ap.add_argument("--workers", type=int, default=1, help="host threads of the disorder-ensemble line")
-    ap.add_argument("--ensemble-method", default="lanczos", choices=["batched", "lanczos", "restart"],
+    ap.add_argument("--ensemble-method", default="batched", choices=["batched", "lanczos", "restart"],
                     help="eigensolver of the disorder-ensemble line")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")

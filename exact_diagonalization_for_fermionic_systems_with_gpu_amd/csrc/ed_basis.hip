@@ -122,6 +122,10 @@ int dmalloc(void **p, size_t bytes)
     if (bytes == 0) bytes = 256;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (release_idle_thread_pools() > 0) e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return fail(ED_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));

@@ -81,6 +81,19 @@ __device__ __forceinline__ void stage_table(const double *src, int nops, int G, 
     for (int q = threadIdx.x; q < nops * G * 2; q += 256) dst[q] = s[q];
 }
 
+// Whether a workgroup's group still has a running copy, the same answer for every thread of the
+// workgroup.  The check kernel that runs beside a step launch can switch the group's last copy off
+// at any moment, and each wave reading the count itself would read it at its own time: one wave
+// could leave while another stays on and reads the row descriptors and the count that the leaver
+// should have written to LDS, gathering from wherever those stale words point.  Thread 0 reads
+// the count once and the workgroup leaves or stays as a whole.
+__device__ __forceinline__ bool group_running(const int *gact, int grp, int *s_flag)
+{
+    if (threadIdx.x == 0) *s_flag = gact[grp];
+    __syncthreads();
+    return *s_flag != 0;
+}
+
 // descriptor of call op in the row/column of state s with rank self: (col << 8 | code), or ~0
 __device__ __forceinline__ uint32_t gdesc(const OtfOp &op, int o, uint32_t s, uint32_t self,
                                           const int32_t *__restrict__ addr)
@@ -106,9 +119,11 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
     extern __shared__ double s_upv[]; // value table, TV [nops_up][G][2]
     __shared__ uint32_t s_desc[GUMAX][64];
     __shared__ int s_cnt[GUMAX];
+    __shared__ int s_run;
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
-    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
+    if (T >= ntiles) return; // past the tiles (uniform)
+    if (!group_running(gact, tgrp[T], &s_run)) return; // every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     if (!g.has_up) return;
     TV *s_vt = reinterpret_cast<TV *>(s_upv);
@@ -180,9 +195,11 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
 {
     extern __shared__ double s_dnv[]; // value table TV [nops_dn][G][2], then descriptors [nops_dn][CW]
     __shared__ double s_red[256];
+    __shared__ int s_run;
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
-    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
+    if (T >= ntiles) return; // past the tiles (uniform)
+    if (!group_running(gact, tgrp[T], &s_run)) return; // every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     TV *s_vt = reinterpret_cast<TV *>(s_dnv);
     uint32_t *s_gd = reinterpret_cast<uint32_t *>(s_vt + g.a.dn.nops * g.G * 2);
@@ -290,8 +307,10 @@ __global__ __launch_bounds__(256) void k_g_update(const GGroup *__restrict__ gr,
                                                   const int *__restrict__ lim, double *acc)
 {
     __shared__ double s_red[256];
+    __shared__ int s_run;
     const int64_t T = blockIdx.x;
-    if (T >= ntiles || !gact[tgrp[T]]) return; // past the tiles, or every copy of the group is done
+    if (T >= ntiles) return; // past the tiles (uniform)
+    if (!group_running(gact, tgrp[T], &s_run)) return; // every copy of the group is done
     const GGroup g = gr[tgrp[T]];
     const int64_t ch = T - g.c_off;
     const int k = threadIdx.x & (g.G - 1), c = threadIdx.x >> g.lg;
@@ -336,8 +355,10 @@ __global__ __launch_bounds__(1024) void k_g_finalize(const GGroup *__restrict__ 
                                                      int which, double *dst, int ld, const int *dstep)
 {
     __shared__ double red[16];
+    __shared__ int s_run;
     const int gi = blockIdx.x;
-    if (!active[gi]) return;
+    // one read of the copy's flag for the whole workgroup (a check beside this launch may clear it)
+    if (!group_running(active, gi, &s_run)) return;
     const GGroup &g = gr[cgroup[gi]];
     const int k = gi - g.cbase;
     const int64_t np = which == 0 ? g.nu_dn : g.c_n;
@@ -1113,7 +1134,8 @@ static int batch_run(ed_batch bb, const ed_lanczos_cfg *cfg, uint32_t seed, int 
         // device-side convergence test runs on the side stream while the next chunk steps on (a
         // copy found converged is switched off from the chunk after; a kernel that still sees the
         // old flag does wasted but harmless work: a copy's kernels touch only its own vectors,
-        // partial slots and a/b rows, and the test reads only the rows of finished steps).  The
+        // partial slots and a/b rows, and the test reads only the rows of finished steps; each
+        // workgroup reads a group's running count once, group_running).  The
         // main stream waits for the last test.  Advances N and the rotation phase.
         auto enqueue_chunks = [&](int nch) -> int {
             for (int q = 0; q < nch && N < maxN; ++q) {

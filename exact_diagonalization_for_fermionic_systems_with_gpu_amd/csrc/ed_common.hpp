@@ -71,9 +71,14 @@ struct Binom {
 const Binom &binom_host();
 int64_t binom(int n, int k);
 
-// device scratch allocation with error reporting
+// device scratch allocation with error reporting; when hipMalloc fails, the calling thread's
+// idle grow-only pools (release_idle_thread_pools) are released and the allocation tried once more
 int dmalloc(void **p, size_t bytes);
 void dfree(void *p);
+// Frees the calling thread's grow-only pools that no call of the thread is using (the Lanczos
+// basis kept for selective orthogonalisation, which grows to the longest solve: up to max_steps
+// vectors); returns the bytes released.
+size_t release_idle_thread_pools();
 
 // Grow-only device workspace of one host thread.  hipMalloc and hipFree wait for the whole
 // device, so entry points that run many times on concurrent streams (the disorder-ensemble

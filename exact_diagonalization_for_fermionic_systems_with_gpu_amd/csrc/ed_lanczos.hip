@@ -327,17 +327,35 @@ __global__ __launch_bounds__(256) void k_times_i(int64_t ncplx, const double *__
 
 // The basis chunks and good Ritz vectors come from a grow-only pool of the calling thread, reused by
 // its next solve (a solve with a few hundred vectors would otherwise map and unmap gigabytes per
-// call); released when the thread exits (not during process teardown, ed_thread.hpp).
+// call); released when the thread exits (not during process teardown, ed_thread.hpp), or between
+// solves when a device allocation of the thread fails (release_idle_thread_pools).
 struct SoPool {
     std::vector<double *> chunk; // basis chunks
     std::vector<size_t> chunk_cap;
     std::vector<double *> yv;    // Ritz vectors
     std::vector<size_t> yv_cap;
+    int users = 0;               // solves of this thread holding buffers of the pool
+    size_t release()
+    {
+        size_t bytes = 0;
+        for (size_t i = 0; i < chunk.size(); ++i) {
+            dfree(chunk[i]);
+            bytes += chunk_cap[i] * sizeof(double);
+        }
+        for (size_t i = 0; i < yv.size(); ++i) {
+            dfree(yv[i]);
+            bytes += yv_cap[i] * sizeof(double);
+        }
+        chunk.clear();
+        chunk_cap.clear();
+        yv.clear();
+        yv_cap.clear();
+        return bytes;
+    }
     ~SoPool()
     {
         if (process_exiting()) return;
-        for (double *c : chunk) dfree(c);
-        for (double *y : yv) dfree(y);
+        release();
     }
     // the i-th buffer of list v (capacities cap) with room for `doubles` doubles
     static int get(std::vector<double *> &v, std::vector<size_t> &cap, size_t i, size_t doubles, double **out)
@@ -359,6 +377,8 @@ struct SoPool {
 };
 static thread_local SoPool t_so_pool;
 
+size_t release_idle_thread_pools() { return t_so_pool.users == 0 ? t_so_pool.release() : 0; }
+
 struct SelOrth {
     static constexpr int kChunk = 16;
     int64_t n = 0; // doubles per vector
@@ -374,8 +394,12 @@ struct SelOrth {
     double theta_cap = std::numeric_limits<double>::infinity();
     int analyses = 0, purges = 0;
     double *part = nullptr;       // device: dot partials of the duplicate check
+    SelOrth() { ++t_so_pool.users; }
+    SelOrth(const SelOrth &) = delete;
+    SelOrth &operator=(const SelOrth &) = delete;
     ~SelOrth()
     {
+        --t_so_pool.users;
         dfree(coef);
         dfree(part);
     }

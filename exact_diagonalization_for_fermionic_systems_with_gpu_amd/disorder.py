@@ -5,9 +5,11 @@ potential Make_N(i, eps_i) per species).
 For each seeded realisation eps_i ~ Uniform[-W, W] (the same potential for both species;
 numpy PCG64 seeded with seed0 + r) this module computes
   * E0 of every realisation in one batched Lanczos launch (ed_batch_lowest),
-  * the nev lowest eigenvalues and the ground state (the cuSpectra solver role): by default one
-    Lanczos run per realisation with selective orthogonalisation (ed_lanczos_lowest, method
-    "lanczos"), or the thick-restart solver (ed_eigs_lowest, method "restart"),
+  * the nev lowest eigenvalues and the ground state (the cuSpectra solver role): by default from
+    one batched Lanczos run of all realisations and a batched rebuild of their ground states
+    (ed_batch_lowest_k, method "batched"), or one Lanczos run per realisation with selective
+    orthogonalisation (ed_lanczos_lowest, method "lanczos"), or the thick-restart solver
+    (ed_eigs_lowest, method "restart"),
   * the half-chain entanglement entropy S_vN of the ground state from the particle-number blocks
     of rho_A -- for all realisations at once, each block size solved for every state in one
     batched call (ed_entanglement_entropy_vn_batch) -- and optionally the reference's
@@ -91,7 +93,7 @@ def half_chain_entropy(H: Hubbard, n_up: int, n_dn: int, psi: torch.Tensor, LA: 
 
 
 def run_ensemble(n_real=128, L_sites=14, n_up=7, n_dn=7, U=4.0, t=1.0, W=8.0, seed0=2024, nev=4, ncv=16,
-                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="lanczos",
+                 tol=1e-10, maxit=1000, log=None, with_sref=True, workers=1, method="batched",
                  max_steps=2000, batch_steps=600, batch_size=128) -> EnsembleResult:
     """workers > 1 solves that many realisations at once after the batched launch, each worker
     thread on its own stream.  method "batched": the nev lowest of every realisation and its

@@ -388,9 +388,8 @@ int ed_batch_destroy(ed_batch b);
  * the same kernels without capture; 1 replays fifty-step graphs; 2 captures every step after the
  * first ten, up to cfg->max_steps, in one graph and launches it once (the steps after the last
  * instance has stopped find no tile to work on and return at once: set max_steps near the
- * expected step count; not used by the library's own sweeps, since a run that used it for the
- * Lieb-12 sweep inside a longer job ended in a GPU memory fault of unknown cause).  The results
- * do not depend on it, bit for bit.  Synchronises. */
+ * expected step count; the library's own sweeps use 1, which is as fast).  The results do not
+ * depend on it, bit for bit.  Synchronises. */
 int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int use_graph, double *evals,
                     int *nsteps, int *status, void *stream);
 /* The nev lowest eigenvalues of every instance from one batched run (the SymEigsSolver
@@ -412,9 +411,7 @@ int ed_batch_lowest(ed_batch b, const ed_lanczos_cfg *cfg, uint32_t seed, int us
  * passes the test -- where ed_lanczos_eigen would stop -- by a second batched run of the same
  * recurrence (the same a and b bit for bit) that accumulates sum z_k v_k, each instance for its
  * own n steps; an instance whose T_n value is not the filter's lowest (to 1e-8 relative) gets
- * ED_ENOCONV and no vector.  Synchronises.  The library's disorder ensemble does not use this
- * call by default: one 128-instance run of it (max_steps 500) ended in a GPU memory fault whose
- * cause has not been found. */
+ * ED_ENOCONV and no vector.  Synchronises. */
 int ed_batch_lowest_k(ed_batch b, const ed_lanczos_cfg *cfg, int nev, uint32_t seed, int use_graph, double *evals,
                       int *nsteps, int *status, void *const *d_evecs, void *stream);
 /* Panel shape of the batch kernels: DN rows per panel (4 or 8) x DN panels per workgroup (1-16),

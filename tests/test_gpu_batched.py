@@ -58,8 +58,6 @@ def test_graph_replay_equals_eager():
     assert np.array_equal(ev_g.view(np.uint64), ev_e.view(np.uint64))
 
 
-@pytest.mark.skip(reason="use_graph = 2 (one graph for the whole solve) is off by default: a full bench.py run with "
-                         "the Drude sweep in this mode ended in a GPU memory fault whose cause has not been found")
 def test_one_graph_solve_equals_eager():
     """one graph for the whole solve after the first ten steps (capped at 300 steps) against eager
     launches: the same step counts and eigenvalues bit for bit"""
@@ -118,8 +116,6 @@ def test_batch_shapes_bitwise_and_autotune():
     assert np.array_equal(ev.view(np.uint64), ref[0].view(np.uint64))
 
 
-@pytest.mark.skip(reason="ed_batch_lowest_k is off by default: a 128-realisation run of it ended in a GPU memory "
-                         "fault whose cause has not been found")
 def test_lowest_k_complex_ground_states():
     """ed_batch_lowest_k with ground states on complex instances (Lieb-12 at flux 2 pi / 16, five
     sectors): values against the plain Lanczos E0, ground states against its Ritz vector from the
@@ -150,8 +146,6 @@ def test_lowest_k_complex_ground_states():
     assert all(bool(torch.isnan(v.real).all()) for v in short)
 
 
-@pytest.mark.skip(reason="ed_batch_lowest_k is off by default: a 128-realisation run of it ended in a GPU memory "
-                         "fault whose cause has not been found")
 def test_lowest_k_small_sectors_end_in_invariant_subspaces():
     """Sectors of dimension 1, 144 and 4356 (Lieb-12, N_up = 0, 1, 2 at flux 2 pi / 16): the two
     larger recurrences end in an invariant subspace before 300 steps or converge, their values

@@ -26,11 +26,6 @@ from exact_diagonalization_for_fermionic_systems_with_gpu_amd.spectra import Hub
 
 pytestmark = pytest.mark.gpu
 
-# ed_batch_lowest_k (the batched nev-lowest solve with its ground-state rebuild) is off by default:
-# a 128-realisation ensemble run of it capped at 500 steps ended in a GPU memory fault whose cause
-# has not been found
-SKIP_BATCHED_NEV = pytest.mark.skip(reason="ed_batch_lowest_k is off by default: a 128-realisation run of it "
-                                           "ended in a GPU memory fault whose cause has not been found")
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "chain14_eigs_oracle.json")
 
 
@@ -111,7 +106,7 @@ def test_eigs_kagome12_vs_lanczos_and_arpack():
     assert np.allclose(ev, wa, rtol=1e-10, atol=0), (ev, wa)
 
 
-@pytest.mark.parametrize("method", [pytest.param("batched", marks=SKIP_BATCHED_NEV), "lanczos", "restart"])
+@pytest.mark.parametrize("method", ["batched", "lanczos", "restart"])
 def test_disorder_ensemble_vs_dense_chain8(method):
     print(f"disorder ensemble chain8 x4 ({method})", flush=True)
     # the Lanczos test scales with |T| (about 3 |E0| here), the restart solver's with |theta|: S_ref
@@ -160,7 +155,7 @@ def test_disorder_gradient_vs_finite_differences():
 
 
 @pytest.mark.skipif(not os.path.exists(GOLD), reason="fixture not generated")
-@pytest.mark.parametrize("method", [pytest.param("batched", marks=SKIP_BATCHED_NEV), "lanczos"])
+@pytest.mark.parametrize("method", ["batched", "lanczos"])
 def test_chain14_against_fixture(method):
     gold = json.load(open(GOLD))
     print(f"disorder chain14 7/7 x2 vs fixture ({method})", flush=True)
@@ -181,11 +176,12 @@ def test_chain14_against_fixture(method):
         assert np.allclose(res.grad_eps[r], rec["dE0_deps"], atol=1e-8)
 
 
-def test_disorder_ensemble_in_batches_matches_one_batch():
-    """Realisations taken in batches of 4 (two batched E0 launches) give what one batch of all 6
-    gives, to the rounding of the other group size in the batched E0 (the batch's partial sums are
-    per tile of 256 / G columns)."""
-    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, tol=1e-12, method="lanczos", with_sref=False)
+@pytest.mark.parametrize("method", ["batched", "lanczos"])
+def test_disorder_ensemble_in_batches_matches_one_batch(method):
+    """Realisations taken in batches of 4 (two batched launches) give what one batch of all 6
+    gives, to the rounding of the other group size in the batched runs (the batch's partial sums
+    are per tile of 256 / G columns)."""
+    kw = dict(n_real=6, L_sites=8, n_up=4, n_dn=4, W=8.0, nev=3, tol=1e-12, method=method, with_sref=False)
     a = DS.run_ensemble(**kw)
     b = DS.run_ensemble(**kw, batch_size=4)
     assert np.allclose(a.evals, b.evals, rtol=1e-12, atol=0)
@@ -194,7 +190,7 @@ def test_disorder_ensemble_in_batches_matches_one_batch():
     assert np.allclose(a.e0_batched, b.e0_batched, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("method", ["lanczos", pytest.param("batched", marks=SKIP_BATCHED_NEV)])
+@pytest.mark.parametrize("method", ["lanczos", "batched"])
 def test_disorder_ensemble_concurrent_workers_match_serial(method):
     """Several realisations solved (method "lanczos") or post-processed (entropies, densities) at
     once on separate streams (one rocBLAS handle and scratch per host thread) give the serial
@@ -293,7 +289,6 @@ def test_entropy_vn_batch_matches_per_state():
         assert abs(sb[0] - EE.entropy_from_lambda(lam)) < 1e-10
 
 
-@SKIP_BATCHED_NEV
 def test_batched_lowest_k_against_per_instance():
     """ed_batch_lowest_k on the 14-site chain (D = 1.18e7, one group of 6 realisations sharing the
     operator structure): the 4 lowest of every realisation against ed_lanczos_lowest on each, and
@@ -347,7 +342,6 @@ def test_lanczos_lowest_near_degenerate_pairs_chain14():
     assert np.allclose(ev, ref, rtol=1e-10, atol=0), (ev, ref)
 
 
-@SKIP_BATCHED_NEV
 def test_batched_lowest_k_near_degenerate_pairs_chain14():
     """realisation 25 (two pairs of levels 1e-8 apart) by the batched solve, against the
     thick-restart solver"""
@@ -381,7 +375,6 @@ def test_lanczos_lowest_start_vector_barely_touching_ground_state_chain14():
     assert abs(ev[0] - e0) < 1e-10 * abs(e0), (ev, e0)
 
 
-@SKIP_BATCHED_NEV
 def test_batched_lowest_k_start_vector_barely_touching_ground_state_chain14():
     """realisation 125 (start vector 1.1e-7 from the ground state) by the batched solve"""
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd.batched import BatchedLanczos
