@@ -60,6 +60,7 @@ struct GGroup {
     int64_t c_off, c_n;      // update chunks (GCH * CW elements)
     int64_t p_off;           // partials: max(nu_dn, c_n) * G
     int64_t nu_dn;           // DN dot partials per copy: 4-row units x column blocks, any panel shape
+    int up_gather, dn_gather; // calls 0..gather-1 gather; the rest are diagonal (the own amplitude)
 };
 
 
@@ -110,7 +111,7 @@ __device__ __forceinline__ uint32_t gdesc(const OtfOp &op, int o, uint32_t s, ui
 // Tiles run slab-major (all rows of a CW-column slab, then the next slab), so the rows the
 // hops gather from -- dim_up x CW x G amplitudes, 3.7 MB for the 924 x 924 x 16 sector --
 // stay in the XCD's L2 while the slab is processed.
-template <typename V, typename TV, int GU>
+template <typename V, typename TV, int GU, bool DIAG>
 __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, const int *__restrict__ tgrp, int64_t ntiles, int64_t per_xcd,
                                               const int *__restrict__ active, const int *__restrict__ gact,
                                               const double *__restrict__ bdev, int ldb,
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
 {
     extern __shared__ double s_upv[]; // value table, TV [nops_up][G][2]
     __shared__ uint32_t s_desc[GUMAX][64];
-    __shared__ int s_cnt[GUMAX];
+    __shared__ int s_cnt[GUMAX], s_nh[GUMAX];
     __shared__ int s_run;
     const int64_t b = blockIdx.x;
     const int64_t T = (b & 7) * per_xcd + (b >> 3);
@@ -153,27 +154,37 @@ __global__ __launch_bounds__(256) void k_g_up(const GGroup *__restrict__ gr, con
             const bool has = dsc != 0xffffffffu;
             const uint64_t bal = __ballot(has);
             if (has) s_desc[r][__popcll(bal & ((1ull << lane) - 1ull))] = dsc;
-            if (lane == 0) s_cnt[r] = __popcll(bal);
+            if (lane == 0) {
+                s_cnt[r] = __popcll(bal);
+                // entries of the gathering calls; the rest (trailing diagonal calls) point at the row
+                if (DIAG) s_nh[r] = g.up_gather >= 64 ? __popcll(bal) : __popcll(bal & ((1ull << g.up_gather) - 1ull));
+            }
         }
         __syncthreads();
         if (!live) continue;
         for (int r = 0; r < GU; ++r) {
             const int64_t u = u0 + r;
             if (u >= du) break;
-            const int n = s_cnt[r];
+            const int n = s_cnt[r], nh = DIAG ? s_nh[r] : n;
             V acc = VO<V>::zero();
-            for (int j0 = 0; j0 < n; j0 += 8) {
+            for (int j0 = 0; j0 < nh; j0 += 8) {
                 V xl[8];
                 uint32_t ds[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    ds[q] = (j0 + q < n) ? s_desc[r][j0 + q] : 0u;
-                    if (j0 + q < n) xl[q] = xv[(((ds[q] >> 8) * ddu + du32) << lg) + ku];
+                    ds[q] = (j0 + q < nh) ? s_desc[r][j0 + q] : 0u;
+                    if (j0 + q < nh) xl[q] = xv[(((ds[q] >> 8) * ddu + du32) << lg) + ku];
                 }
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                    if (j0 + q < n)
+                    if (j0 + q < nh)
                         acc = omac(gval<TV>(s_vt, g.G, ds[q] & 0xffu, k), VO<V>::scale(sc, xl[q]), acc);
+            }
+            // the trailing diagonal calls' entries (Make_N, Insert_2Nop after the last hop) read
+            // the row's own amplitude: one load for all of them (the same value, the same sum)
+            if (DIAG && n > nh) {
+                const V xs = VO<V>::scale(sc, xv[(((uint32_t)u * ddu + du32) << lg) + ku]);
+                for (int j = nh; j < n; ++j) acc = omac(gval<TV>(s_vt, g.G, s_desc[r][j] & 0xffu, k), xs, acc);
             }
             yv[(((uint32_t)u * ddu + du32) << lg) + ku] = acc;
         }
@@ -246,13 +257,14 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
             // unchanged bit for bit (finite x; the sums start at +0 or the pass-1 value, never -0)
             constexpr int KB = 4;
             const TV tzero = tv_make<TV>(0.0, 0.0);
-            for (int o0 = 0; o0 < nops; o0 += KB) {
+            const int nhop = g.dn_gather; // calls from nhop on are diagonal: no gathers, below
+            for (int o0 = 0; o0 < nhop; o0 += KB) {
                 uint32_t col[KB];
                 TV av[KB];
                 V xl[KB][GR];
 #pragma unroll
                 for (int q = 0; q < KB; ++q) {
-                    const uint32_t e = (o0 + q < nops) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
+                    const uint32_t e = (o0 + q < nhop) ? s_gd[(o0 + q) * g.CW + c] : 0xffffffffu;
                     const bool on = e != 0xffffffffu;
                     col[q] = on ? (e >> 8) : (uint32_t)d;
                     const TV a = gval<TV>(s_vt, g.G, on ? (e & 0xffu) : 0u, k);
@@ -266,6 +278,21 @@ __global__ __launch_bounds__(256) void k_g_dn(const GGroup *__restrict__ gr, con
                 for (int q = 0; q < KB; ++q)
 #pragma unroll
                     for (int r = 0; r < GR; ++r) acc[r] = omac(av[q], VO<V>::scale(sc, xl[q][r]), acc[r]);
+            }
+            // the trailing diagonal calls (Make_N, Insert_2Nop after the last hop) all read the
+            // row's own amplitude: loaded once per row here instead of once per call (the same
+            // value; a column without the call's element adds 0 times it as above: the same sums)
+            if (nhop < nops) {
+#pragma unroll
+                for (int r = 0; r < GR; ++r) {
+                    const V xr = VO<V>::scale(sc, xv[((rowbase[r] + (uint32_t)d) << lg) + ku]);
+                    for (int o = nhop; o < nops; ++o) {
+                        const uint32_t e = s_gd[o * g.CW + c];
+                        const bool on = e != 0xffffffffu;
+                        const TV a = gval<TV>(s_vt, g.G, on ? (e & 0xffu) : 0u, k);
+                        acc[r] = omac(on ? a : tzero, xr, acc[r]);
+                    }
+                }
             }
 #pragma unroll
             for (int r = 0; r < GR; ++r) {
@@ -433,6 +460,7 @@ struct Batch {
     int maxnops_dn = 1;
     size_t lds_up = 0, lds_dn = 0; // dynamic LDS of the UP / DN kernels (largest group)
     int gr = 4, gu = 1;            // DN / UP rows per panel (4 or 8 / 1 or 4)
+    bool up_diag = false;          // some group's UP calls end in diagonal calls (k_g_up<..., DIAG>)
     int kp = 4, kpu = 4;           // DN / UP panels per workgroup (ed_batch_autotune, ed_batch_set_shape)
     ~Batch()
     {
@@ -600,6 +628,16 @@ extern "C" int ed_batch_create(int ninst, const ed_hubbard *ops, int complex_vec
         g.c_n = (g.nelem + (int64_t)GCH * g.CW - 1) / ((int64_t)GCH * g.CW);
         tc += g.c_n;
         B->maxnops_dn = std::max(B->maxnops_dn, h->otf.dn.nops);
+        // calls from up_gather / dn_gather on are all diagonal (Make_N / Insert_2Nop after the
+        // last hop): the kernels take the row's own amplitude once for them
+        auto gather_end = [](const OtfOp *ops, int nops) {
+            int e = nops;
+            while (e > 0 && (ops[e - 1].kind == ED_OP_DENSITY || ops[e - 1].kind == ED_OP_DENSITY2)) --e;
+            return e;
+        };
+        g.up_gather = gather_end(opu[mem[0]].data(), h->otf.up.nops);
+        g.dn_gather = gather_end(opd[mem[0]].data(), h->otf.dn.nops);
+        if (g.up_gather < h->otf.up.nops) B->up_diag = true;
         // value tables: [call][copy] x {vf, vb}
         auto table = [&](bool up) {
             const int nops = up ? h->otf.up.nops : h->otf.dn.nops;
@@ -903,12 +941,18 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
                         bool first, const double *zdev, const int *lim, double *acc, hipStream_t st)
 {
     const int64_t per1 = (B.t1 + 7) / 8, per2 = (B.t2 + 7) / 8;
-    if (B.gu == 4)
-        hipLaunchKernelGGL((k_g_up<V, TV, 4>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, gact, bdev, ld, dstep, x, y, B.kpu);
-    else
-        hipLaunchKernelGGL((k_g_up<V, TV, 1>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1,
-                           B.t1, per1, active, gact, bdev, ld, dstep, x, y, B.kpu);
+#define ED_GUP(R_, D_)                                                                                        \
+    hipLaunchKernelGGL((k_g_up<V, TV, R_, D_>), dim3((unsigned)(per1 * 8)), dim3(256), B.lds_up, st, B.dgr, B.dtg1, \
+                       B.t1, per1, active, gact, bdev, ld, dstep, x, y, B.kpu)
+    // DIAG: some group has trailing diagonal UP calls (one load of the row's own amplitude for them)
+    if (B.gu == 4) {
+        if (B.up_diag) ED_GUP(4, true);
+        else ED_GUP(4, false);
+    } else {
+        if (B.up_diag) ED_GUP(1, true);
+        else ED_GUP(1, false);
+    }
+#undef ED_GUP
     const size_t sh = B.lds_dn;
 #define ED_GDN(AX, R_)                                                                                        \
     hipLaunchKernelGGL((k_g_dn<V, TV, AX, R_>), dim3((unsigned)(per2 * 8)), dim3(256), sh, st, B.dgr, B.dtg2, B.t2, \
