@@ -978,7 +978,8 @@ static int enqueue_step(const Batch &B, const double *x, const double *xp, doubl
 
 // Ritz vectors by a second run of the same recurrence (same start vectors, kernels and launch
 // order, so the same a and b bit for bit): copy c runs lim[c] steps and accumulates
-// sum_N z[N * (max_steps + 3) + c] v_N; the normalised result goes to evecs[instance].
+// sum_N z[N * ncopies + c] v_N (z step-major, max_steps + 3 rows); the normalised result goes to
+// evecs[instance].
 struct BatchRebuild {
     const double *z = nullptr;    // host, step-major, nc per row
     const int *lim = nullptr;     // host, per copy (0: not rebuilt)
