@@ -261,11 +261,12 @@ def entropy_only_line(N=18, Q=9, seed=5):
     return out
 
 
-def batched_drude_line(with_cpu=False):
+def batched_drude_line(with_cpu=False, use_graph=1):
     """Lieb-12 twist sweep: 13 sectors x 16 fluxes = 208 complex128 Lanczos instances, one Drude
     weight per sector; batched (ed_batch_lowest: the device convergence test every ten steps,
-    fifty-step hipGraph replays) vs the plain per-instance loop on the same GPU, both checked
-    against the oracle's Drude weights (tests/golden fixture)."""
+    fifty-step hipGraph replays, or with use_graph 2 every step after the first ten in one graph
+    launch) vs the plain per-instance loop on the same GPU, both checked against the oracle's
+    Drude weights (tests/golden fixture)."""
     import numpy as np
     from exact_diagonalization_for_fermionic_systems_with_gpu_amd import batched as BT
     gold = json.load(open(os.path.join(REPO, "tests", "golden", "lieb12_drude_oracle.json")))
@@ -274,12 +275,12 @@ def batched_drude_line(with_cpu=False):
     tune = BT.BatchedLanczos([h for (_, _, h) in BT.lieb12_instances(4.0, 16)], True).autotune(steps=30)
     shape = (tune["dn_rows"], tune["dn_panels"], tune["up_rows"], tune["up_panels"])
     log(f"[bench] batch autotune: {tune}")
-    BT.lieb12_drude_sweep(batched=True, shape=shape)  # warm-up (allocations, graph capture paths)
+    BT.lieb12_drude_sweep(batched=True, shape=shape, use_graph=use_graph)  # warm-up (allocations, graph paths)
     BT.lieb12_drude_sweep(batched=False)
     # three timed sweeps of each, alternating; the fastest of each is reported
     runs_b, runs_p = [], []
     for _ in range(3):
-        runs_b.append(BT.lieb12_drude_sweep(batched=True, shape=shape))
+        runs_b.append(BT.lieb12_drude_sweep(batched=True, shape=shape, use_graph=use_graph))
         runs_p.append(BT.lieb12_drude_sweep(batched=False))
     E0b, Db, tb, _ = min(runs_b, key=lambda r: r[2])
     E0p, Dp, tp, _ = min(runs_p, key=lambda r: r[2])
@@ -312,7 +313,7 @@ def batched_drude_line(with_cpu=False):
             "speedup": round(tp / tb, 3), "max_rel_err_vs_oracle": err, "instances": 208,
             "max_abs_e0_batched_minus_plain": e0_diff, "max_abs_e0_flux_asymmetry": e0_asym,
             "batch_shape": {k: tune[k] for k in ("dn_rows", "dn_panels", "up_rows", "up_panels")},
-            "graph": "fifty-step graph replays",
+            "graph": "fifty-step graph replays" if use_graph == 1 else "one graph launch after the first ten steps",
             "workload": "lieb12_N12_13sectors_16fluxes_U4_complex128"}
 
 
@@ -397,6 +398,8 @@ def main():
     ap.add_argument("--workers", type=int, default=1, help="host threads of the disorder-ensemble line")
     ap.add_argument("--ensemble-method", default="batched", choices=["batched", "lanczos", "restart"],
                     help="eigensolver of the disorder-ensemble line")
+    ap.add_argument("--drude-graph", type=int, default=1, choices=[1, 2],
+                    help="batched Drude sweep: 1 fifty-step graph replays, 2 one graph for the whole solve")
     ap.add_argument("--drude-only", action="store_true",
                     help="run only the batched Drude sweep line (for profiling its kernels)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -409,7 +412,7 @@ def main():
         sys.exit(2)
 
     if args.drude_only:
-        print(json.dumps({"batched_drude": batched_drude_line()}), flush=True)
+        print(json.dumps({"batched_drude": batched_drude_line(use_graph=args.drude_graph)}), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps({"disorder_ensemble": ensemble_line(args.ensemble, workers=args.workers,
@@ -582,7 +585,7 @@ def main():
                 log(f"[bench] {name} failed: {type(e).__name__}: {e}")
                 out[name] = {"error": f"{type(e).__name__}: {e}"}
 
-        extra("batched_drude", lambda: batched_drude_line(with_cpu=not args.no_cpu))
+        extra("batched_drude", lambda: batched_drude_line(with_cpu=not args.no_cpu, use_graph=args.drude_graph))
         extra("kagome12_ground_state", lambda: kagome12_line(with_cpu=not args.no_cpu))
         extra("disorder_ensemble", lambda: ensemble_line(args.ensemble, workers=args.workers,
                                                          method=args.ensemble_method))
