@@ -334,7 +334,7 @@ struct SoPool {
     std::vector<size_t> chunk_cap;
     std::vector<double *> yv;    // Ritz vectors
     std::vector<size_t> yv_cap;
-    int users = 0;               // solves of this thread holding buffers of the pool
+    int users = 0;               // solves of this thread that have taken buffers of the pool
     size_t release()
     {
         size_t bytes = 0;
@@ -394,19 +394,34 @@ struct SelOrth {
     double theta_cap = std::numeric_limits<double>::infinity();
     int analyses = 0, purges = 0;
     double *part = nullptr;       // device: dot partials of the duplicate check
-    SelOrth() { ++t_so_pool.users; }
+    bool holds = false;           // counted in t_so_pool.users (from its first take on)
+    SelOrth() = default;
     SelOrth(const SelOrth &) = delete;
     SelOrth &operator=(const SelOrth &) = delete;
     ~SelOrth()
     {
-        --t_so_pool.users;
+        if (holds) --t_so_pool.users;
         dfree(coef);
         dfree(part);
     }
+    // before the first take: the pool is in use from here on (a failed allocation meanwhile,
+    // its own included, must not release it)
+    void hold()
+    {
+        if (!holds) {
+            holds = true;
+            ++t_so_pool.users;
+        }
+    }
     double *vec(int k) { return chunks[(size_t)k / kChunk] + (size_t)(k % kChunk) * (size_t)n; }
-    int new_y(double **y) { return SoPool::get(t_so_pool.yv, t_so_pool.yv_cap, Y.size(), (size_t)n, y); }
+    int new_y(double **y)
+    {
+        hold();
+        return SoPool::get(t_so_pool.yv, t_so_pool.yv_cap, Y.size(), (size_t)n, y);
+    }
     int store(int k, const double *v, hipStream_t st)
     {
+        hold();
         while ((int)chunks.size() * kChunk <= k) {
             double *c = nullptr;
             if (SoPool::get(t_so_pool.chunk, t_so_pool.chunk_cap, chunks.size(), (size_t)n * kChunk, &c) != ED_OK)
